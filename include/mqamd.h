@@ -1,0 +1,176 @@
+/*
+ * mqamd.h - C ABI of the MI355X (gfx950) state-vector backend.
+ *
+ * The library owns every device buffer; the caller owns every host buffer and no pointer
+ * escapes a call.  All entry points return an int status (MQA_OK == 0) and leave a
+ * thread-local message readable through mqa_last_error().  Calls on one simulator handle
+ * are serialised by the caller (the reference documents the same restriction:
+ * ccsrc/include/simulator/vector/vector_state.tpp:715 mutates the rng inside const methods).
+ *
+ * State layout: interleaved complex (re, im), little-endian qubit order - amplitude index
+ * bit q <=> qubit q (reference: ccsrc/lib/simulator/vector/detail/cpu_common/
+ * cpu_vector_core_policy.cpp:72).  c64 states hold float pairs, c128 states double pairs.
+ *
+ * Each entry point names the reference interface it replaces (file:line, relative to the
+ * mindquantum 0.12.0 source tree).
+ */
+#ifndef MQAMD_H_
+#define MQAMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped to Python exceptions by the host layer) ---- */
+#define MQA_OK 0
+#define MQA_ERR_ARG 1         /* invalid argument         -> ValueError   */
+#define MQA_ERR_OOM 2         /* device allocation failed -> MemoryError  */
+#define MQA_ERR_HIP 3         /* HIP runtime failure      -> RuntimeError */
+#define MQA_ERR_UNSUPPORTED 4 /* gate kind not supported  -> ValueError   */
+#define MQA_ERR_RUNTIME 5     /* other runtime failure    -> RuntimeError */
+
+/* ---- precision ---- */
+#define MQA_C64 0  /* complex64: float amplitudes, float arithmetic  */
+#define MQA_C128 1 /* complex128: double amplitudes, double arithmetic */
+
+/* ---- gate kinds (reference: ccsrc/include/ops/gate_id.h:25-70) ---- */
+#define MQA_GATE_I 0
+#define MQA_GATE_X 1
+#define MQA_GATE_Y 2
+#define MQA_GATE_Z 3
+#define MQA_GATE_H 4
+#define MQA_GATE_S 5
+#define MQA_GATE_SDAG 6
+#define MQA_GATE_T 7
+#define MQA_GATE_TDAG 8
+#define MQA_GATE_SX 9
+#define MQA_GATE_SXDAG 10
+#define MQA_GATE_SWAP 11
+#define MQA_GATE_ISWAP 12
+#define MQA_GATE_ISWAPDAG 13
+#define MQA_GATE_MATRIX 14 /* fixed dense 2^k x 2^k matrix, k = n_obj <= MQA_MAX_OBJ */
+/* parameterised gates: angle = const_val + sum_t coeff[t] * params[param[t]] */
+#define MQA_GATE_RX 32
+#define MQA_GATE_RY 33
+#define MQA_GATE_RZ 34
+#define MQA_GATE_PS 35 /* PhaseShift diag(1, e^{i a}) */
+#define MQA_GATE_GP 36 /* GlobalPhase e^{-i a} I      */
+#define MQA_GATE_RXX 37
+#define MQA_GATE_RYY 38
+#define MQA_GATE_RZZ 39
+
+#define MQA_MAX_OBJ 4
+
+/* One gate of a circuit.  64 bytes, no padding surprises across compilers. */
+typedef struct mqa_gate {
+    int32_t kind;       /* MQA_GATE_* */
+    int32_t n_obj;      /* number of target qubits (1..MQA_MAX_OBJ) */
+    int32_t obj[6];     /* target qubits; obj[i] <=> bit i of the gate's matrix index */
+    uint64_t ctrl_mask; /* control qubits as a bit mask */
+    double const_val;   /* constant part of the angle (parameterised kinds) */
+    int32_t term_begin; /* first (param, coeff) term of the angle */
+    int32_t n_term;     /* number of terms */
+    int32_t mat_begin;  /* MATRIX: first complex entry in the matrix pool (row-major) */
+    int32_t requires_grad; /* 0: the gate gets no derivative tap */
+} mqa_gate;
+
+/* One Pauli term c * X^x Y^y Z^z (masks are disjoint) with a real coefficient
+ * (reference: ccsrc/include/core/mq_base_types.h:67-76, mindquantum/core/operators/hamiltonian.py:94-101). */
+typedef struct mqa_pauli_term {
+    uint64_t x_mask;
+    uint64_t y_mask;
+    uint64_t z_mask;
+    double coeff;
+} mqa_pauli_term;
+
+typedef struct mqa_sim mqa_sim;
+typedef struct mqa_circuit mqa_circuit;
+typedef struct mqa_ham mqa_ham;
+
+/* ---- library ---- */
+/* Message of the last failed call on this thread (empty when none). */
+const char* mqa_last_error(void);
+/* Library version string. */
+const char* mqa_version(void);
+/* Number of visible HIP devices (0 without a GPU).  Never fails. */
+int mqa_device_count(void);
+/* Tuning knob ("tile_qubits_c64", "tile_qubits_c128", "adj_tile_qubits_c64",
+ * "adj_tile_qubits_c128", "group_width", "lowq", "fuse"); value < 0 restores the default. */
+int mqa_set_option(const char* key, int64_t value);
+int mqa_get_option(const char* key, int64_t* value);
+
+/* Kernel timing: with the option "profile" set to 1, HIP events are recorded on the
+ * simulator's own stream around every launch of the named kernel family
+ * ("tile_pass_fwd", "tile_pass_adj", "hamiltonian") and summed here, together with the
+ * algorithmic HBM bytes of those launches (each state read and written once per pass).
+ * No counterpart in the reference (it has no device timing). */
+int mqa_profile_get(const char* kernel, int64_t* launches, double* total_ms, double* bytes);
+int mqa_profile_reset(void);
+
+/* ---- circuit (reference: mindquantum/core/circuit/circuit.py:1108-1130 get_cpp_obj) ----
+ * Host-side object: validated gate list plus the cached tile-pass schedules.
+ * term_param/term_coeff hold the linear angle terms referenced by gates; mats holds
+ * interleaved complex matrices referenced by MATRIX gates.  n_params bounds param indices.
+ * param_requires_grad (n_params flags, NULL = all) drops no-grad parameters from the
+ * jacobian, as ParameterResolver::GetRequiresGradParameters does (basic_gate.h:208-242). */
+int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* term_param, const double* term_coeff,
+                       int32_t n_terms, const double* mats, int64_t n_mat_entries, int32_t n_params,
+                       const uint8_t* param_requires_grad, mqa_circuit** out);
+int mqa_circuit_destroy(mqa_circuit* circ);
+/* Export the tile-pass schedule (host only, no GPU needed) for inspection and tests.
+ * Layout of out (int64 words): [n_passes, then per pass: local_mask, n_ops, gate ids...].
+ * adjoint != 0 exports the reverse (back-sweep) schedule.  *len gets the words needed. */
+int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint, int64_t* out,
+                         int64_t cap, int64_t* len);
+
+/* ---- Hamiltonian (reference: ccsrc/include/ops/hamiltonian.h:30-55, how_to_ == ORIGIN) ---- */
+int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out);
+int mqa_ham_destroy(mqa_ham* ham);
+
+/* ---- simulator (reference: ccsrc/include/python/vector/bind_vec_state.h:46-84, BindSim) ---- */
+/* VectorState(n_qubits, seed) (vector_state.tpp:63-68); the state starts as |0...0>. */
+int mqa_sim_create(int32_t n_qubits, int32_t dtype, uint32_t seed, mqa_sim** out);
+int mqa_sim_destroy(mqa_sim* sim);
+/* reset() (vector_state.tpp Reset): back to |0...0>. */
+int mqa_sim_reset(mqa_sim* sim);
+/* copy() (bind_vec_state.h): deep copy of the device state. */
+int mqa_sim_copy(const mqa_sim* sim, mqa_sim** out);
+int mqa_sim_n_qubits(const mqa_sim* sim, int32_t* n_qubits, int32_t* dtype);
+/* get_qs() (cpu_vector_core_policy.cpp:99-118): device -> host, dim = 2^n interleaved pairs. */
+int mqa_sim_get_qs(const mqa_sim* sim, void* host_out, uint64_t dim);
+/* set_qs() (cpu_vector_core_policy.cpp:120-130): host -> device; the length must equal 2^n
+ * ("state size not match").  No normalisation here: mqsim.py:537-544 normalises in Python. */
+int mqa_sim_set_qs(mqa_sim* sim, const void* host_in, uint64_t dim);
+/* apply_circuit(circ, pr) (vector_state.tpp:948-960); params holds the circuit's parameters. */
+int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ, const double* params, int32_t n_params);
+/* apply_hamiltonian(ham) (vector_state.tpp:962-974): state <- H state. */
+int mqa_sim_apply_hamiltonian(mqa_sim* sim, const mqa_ham* ham);
+/* get_expectation(ham, circ, pr) (vector_state.tpp:711-726): <0|U^dag H U|0> on a copy of the
+ * state; the simulator's own state is left unchanged.  circ may be NULL (no circuit). */
+int mqa_sim_expectation(const mqa_sim* sim, const mqa_ham* ham, const mqa_circuit* circ, const double* params,
+                        int32_t n_params, double out[2]);
+/* get_expectation(ham, circ_right, circ_left, sim_left, pr) (vector_state.tpp:728-768):
+ * <sim_left|Ul^dag H Ur|sim>.  left may be NULL (== sim); either circuit may be NULL. */
+int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ham* ham, const mqa_circuit* circ_right,
+                           const mqa_circuit* circ_left, const double* params, int32_t n_params, double out[2]);
+/* get_expectation_with_grad_multi_multi (vector_state.tpp:1020-1028 -> gradient.h:100-121):
+ * hermitian adjoint gradient for a batch of encoder rows sharing one ansatz vector.
+ * Param index p < n_enc reads enc[b*n_enc + p]; p >= n_enc reads ans[p - n_enc].
+ * f_out: [batch][n_hams][2] (complex <psi|H|psi>); g_out: [batch][n_hams][n_enc+n_ans]
+ * (real; the reference returns 2*Re with a zero imaginary part, gradient.h:124-133). */
+int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ,
+                       const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
+                       double* g_out);
+/* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
+ * state cast to dtype. */
+int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out);
+/* blas.inner_product(a, b) (bind_vec_state.h:86-90, blas.h): <a|b>. */
+int mqa_inner_product(const mqa_sim* bra, const mqa_sim* ket, double out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MQAMD_H_ */

@@ -1,0 +1,14 @@
+"""mindquantum_amd: MindQuantum's state-vector simulator path, native on MI355X (gfx950).
+
+``Simulator('mqvector_amd', n)`` keeps MindQuantum's Simulator API (apply_circuit, get_qs,
+get_expectation, get_expectation_with_grad); every gate, expectation and adjoint gradient runs
+in the HIP library ``libmqamd.so`` (C ABI in include/mqamd.h).
+"""
+from . import dtype
+from .core import (CNOT, H, I, ISWAP, RX, RY, RZ, SWAP, SX, UN, Circuit, GlobalPhase, Hamiltonian, ParameterResolver,
+                   PhaseShift, QubitOperator, Rxx, Ryy, Rzz, S, T, UnivMathGate, X, Y, Z)
+from .core import gates
+from .dtype import complex64, complex128
+from .simulator import Simulator, inner_product
+
+__version__ = "0.1.0"

@@ -1,0 +1,222 @@
+"""Circuit: an ordered list of gates with parameter bookkeeping.
+
+Host-side mirror of mindquantum/core/circuit/circuit.py for the state-vector path: gate
+insertion (+=, +, append), the parameter name views (params_name, encoder_params_name,
+ansatz_params_name, all_encoder, all_ansatz), as_encoder / as_ansatz / no_grad, hermitian()
+(reversed order, each gate hermitian-ed, circuit.py:831-845) and the gate helper methods.
+"""
+import copy
+
+from . import gates as G
+from .parameterresolver import ParameterResolver
+
+
+class _NameList(list):
+    """Ordered parameter names; ``.keys()`` mirrors the reference's ordered-dict views."""
+
+    def keys(self):
+        return list(self)
+
+
+class Circuit(list):
+    """Ordered list of gates."""
+
+    def __init__(self, gates=None):
+        super().__init__()
+        if gates is None:
+            return
+        if isinstance(gates, G.BasicGate):
+            gates = [gates]
+        for g in gates:
+            self.append(g)
+
+    # ---- construction
+    def append(self, gate):
+        if isinstance(gate, Circuit):
+            for g in gate:
+                super().append(g)
+            return self
+        if not isinstance(gate, G.BasicGate):
+            raise TypeError(f"Circuit only accepts gates, but get {type(gate)}")
+        if not gate.obj_qubits:
+            raise ValueError(f"Gate {gate.name} has not been applied on any qubit; use .on()")
+        super().append(gate)
+        return self
+
+    def extend(self, gates):
+        for g in gates:
+            self.append(g)
+        return self
+
+    def insert(self, index, gate):
+        if not isinstance(gate, G.BasicGate):
+            raise TypeError("Circuit only accepts gates")
+        super().insert(index, gate)
+
+    def __iadd__(self, other):
+        if isinstance(other, (Circuit, list)):
+            for g in other:
+                self.append(g)
+        else:
+            self.append(other)
+        return self
+
+    def __add__(self, other):
+        out = Circuit(list(self))
+        out += other
+        return out
+
+    def __radd__(self, other):
+        out = Circuit(other if isinstance(other, (list, Circuit)) else [other])
+        out += self
+        return out
+
+    def __getitem__(self, item):
+        res = super().__getitem__(item)
+        if isinstance(item, slice):
+            return Circuit(res)
+        return res
+
+    def __copy__(self):
+        return Circuit(copy.deepcopy(list(self)))
+
+    # ---- properties
+    @property
+    def n_qubits(self):
+        qs = [q for g in self for q in g.obj_qubits + g.ctrl_qubits]
+        return max(qs) + 1 if qs else 0
+
+    def _names(self, pred):
+        out = _NameList()
+        for g in self:
+            pr = getattr(g, "coeff", None)
+            if isinstance(pr, ParameterResolver):
+                for k in pr.keys():
+                    if pred(pr, k) and k not in out:
+                        out.append(k)
+        return out
+
+    @property
+    def params_name(self):
+        return self._names(lambda pr, k: True)
+
+    @property
+    def encoder_params_name(self):
+        return self._names(lambda pr, k: k in pr.encoder_parameters)
+
+    @property
+    def ansatz_params_name(self):
+        return self._names(lambda pr, k: k not in pr.encoder_parameters)
+
+    @property
+    def all_encoder(self):
+        return self.encoder_params_name
+
+    @property
+    def all_ansatz(self):
+        return self.ansatz_params_name
+
+    @property
+    def parameterized(self):
+        return any(getattr(g, "parameterized", False) for g in self)
+
+    @property
+    def has_measure_gate(self):
+        return False
+
+    @property
+    def is_noise_circuit(self):
+        return False
+
+    # ---- parameter flags
+    def _each_pr(self, fn):
+        for g in self:
+            pr = getattr(g, "coeff", None)
+            if isinstance(pr, ParameterResolver):
+                fn(pr)
+
+    def as_encoder(self, inplace=True):
+        circ = self if inplace else copy.deepcopy(self)
+        circ._each_pr(lambda pr: pr.as_encoder())
+        return circ
+
+    def as_ansatz(self, inplace=True):
+        circ = self if inplace else copy.deepcopy(self)
+        circ._each_pr(lambda pr: pr.as_ansatz())
+        return circ
+
+    def no_grad(self):
+        self._each_pr(lambda pr: pr.no_grad())
+        return self
+
+    def requires_grad(self):
+        self._each_pr(lambda pr: pr.requires_grad())
+        return self
+
+    def hermitian(self):
+        """Reversed circuit with every gate hermitian-ed (circuit.py:831-845)."""
+        return Circuit([g.hermitian() for g in reversed(self)])
+
+    def get_cpp_obj(self, hermitian=False):  # pragma: no cover - reference naming only
+        raise NotImplementedError("the HIP backend compiles circuits with mindquantum_amd.ir")
+
+    # ---- helpers (circuit.py gate shortcuts)
+    def _add(self, gate, obj, ctrl):
+        self.append(gate.on(obj, ctrl))
+        return self
+
+    def x(self, obj, ctrl=None):
+        return self._add(G.X, obj, ctrl)
+
+    def y(self, obj, ctrl=None):
+        return self._add(G.Y, obj, ctrl)
+
+    def z(self, obj, ctrl=None):
+        return self._add(G.Z, obj, ctrl)
+
+    def h(self, obj, ctrl=None):
+        return self._add(G.H, obj, ctrl)
+
+    def s(self, obj, ctrl=None):
+        return self._add(G.S, obj, ctrl)
+
+    def t(self, obj, ctrl=None):
+        return self._add(G.T, obj, ctrl)
+
+    def swap(self, obj, ctrl=None):
+        return self._add(G.SWAP, obj, ctrl)
+
+    def rx(self, para, obj, ctrl=None):
+        return self._add(G.RX(para), obj, ctrl)
+
+    def ry(self, para, obj, ctrl=None):
+        return self._add(G.RY(para), obj, ctrl)
+
+    def rz(self, para, obj, ctrl=None):
+        return self._add(G.RZ(para), obj, ctrl)
+
+    def rxx(self, para, obj, ctrl=None):
+        return self._add(G.Rxx(para), obj, ctrl)
+
+    def ryy(self, para, obj, ctrl=None):
+        return self._add(G.Ryy(para), obj, ctrl)
+
+    def rzz(self, para, obj, ctrl=None):
+        return self._add(G.Rzz(para), obj, ctrl)
+
+    def phase_shift(self, para, obj, ctrl=None):
+        return self._add(G.PhaseShift(para), obj, ctrl)
+
+    def __repr__(self):
+        return "Circuit(" + ", ".join(repr(g) for g in self) + ")"
+
+
+def UN(gate, maps_obj, maps_ctrl=None):  # pylint: disable=invalid-name
+    """Apply ``gate`` on every qubit of ``maps_obj`` (core/circuit/module_circuit.py UN)."""
+    if isinstance(maps_obj, int):
+        maps_obj = list(range(maps_obj))
+    circ = Circuit()
+    for i, q in enumerate(maps_obj):
+        ctrl = None if maps_ctrl is None else maps_ctrl[i]
+        circ += gate.on(q, ctrl)
+    return circ

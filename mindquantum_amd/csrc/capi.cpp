@@ -1,0 +1,991 @@
+// capi.cpp - the C ABI (include/mqamd.h): simulator handles, device workspaces, cached
+// tile-pass plans and the orchestration of forward sweeps, Hamiltonian application,
+// expectation reductions and the batched adjoint gradient.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mqamd.h"
+#include "launch.h"
+#include "program.h"
+#include "schedule.h"
+
+namespace mqa {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        throw Error(MQA_ERR_OOM, std::string(what) + ": out of device memory");
+    }
+    throw Error(MQA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return MQA_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return MQA_ERR_ARG;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return MQA_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MQA_ERR_RUNTIME;
+    }
+}
+
+// ---------------------------------------------------------------- options
+struct Options {
+    std::atomic<int64_t> tile_c64{12}, tile_c128{11}, adj_tile_c64{12}, adj_tile_c128{11};
+    std::atomic<int64_t> group_width{4}, lowq_c64{5}, lowq_c128{4}, max_ops{0};
+    std::atomic<int64_t> budget{0};  // bytes of HBM a batched call may use (0: 85% of free memory)
+    std::atomic<int64_t> profile{0};  // 1: time kernels with HIP events
+};
+Options g_opt;
+
+struct OptEntry {
+    const char* key;
+    std::atomic<int64_t>* v;
+    int64_t def;
+};
+std::vector<OptEntry> opt_table() {
+    return {{"tile_qubits_c64", &g_opt.tile_c64, 12},     {"tile_qubits_c128", &g_opt.tile_c128, 11},
+            {"adj_tile_qubits_c64", &g_opt.adj_tile_c64, 12}, {"adj_tile_qubits_c128", &g_opt.adj_tile_c128, 11},
+            {"group_width", &g_opt.group_width, 4},       {"lowq_c64", &g_opt.lowq_c64, 5},
+            {"lowq_c128", &g_opt.lowq_c128, 4},           {"max_ops_per_pass", &g_opt.max_ops, 0},
+            {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0}};
+}
+
+SchedOptions sched_options(int dtype, bool adjoint) {
+    SchedOptions o;
+    o.tile_qubits = int(dtype == MQA_C64 ? (adjoint ? g_opt.adj_tile_c64 : g_opt.tile_c64)
+                                         : (adjoint ? g_opt.adj_tile_c128 : g_opt.tile_c128));
+    o.low_run = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
+    o.group_width = int(g_opt.group_width);
+    o.max_ops_per_pass = int(g_opt.max_ops);
+    return o;
+}
+
+// ---------------------------------------------------------------- device memory
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        release();
+        if (n == 0) return;
+        hip_check(hipMalloc(&p, n), "hipMalloc");
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+    b.ensure(n);
+    if (!v.empty()) hip_check(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+int current_device() {
+    int d = 0;
+    hip_check(hipGetDevice(&d), "hipGetDevice");
+    return d;
+}
+
+// ---------------------------------------------------------------- device copies of plans
+struct DevPlan {
+    Plan plan;
+    DevBuf passes, groups, ops, pool, tap_ids;
+    bool full = false;
+};
+
+// ---------------------------------------------------------------- kernel timing
+struct KernelStats {
+    int64_t launches = 0;
+    double ms = 0;
+    double bytes = 0;
+};
+std::mutex g_prof_mu;
+std::map<std::string, KernelStats> g_prof;
+
+// Event pairs recorded on one stream; resolved after the stream is synchronised.
+struct Timer {
+    struct Pending {
+        hipEvent_t a, b;
+        std::string name;
+        double bytes;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Pending> pending;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    void resolve() {
+        if (pending.empty()) return;
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        for (auto& p : pending) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                auto& st = g_prof[p.name];
+                st.launches++;
+                st.ms += ms;
+                st.bytes += p.bytes;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~Timer() {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+}  // namespace mqa
+
+// ---------------------------------------------------------------- circuit
+struct mqa_circuit {
+    std::vector<mqa::CGate> gates;
+    int n_params = 0;
+    int n_slots = 0;
+    int max_qubit = -1;
+    // angle table inputs (per slot)
+    std::vector<double> slot_const;
+    std::vector<int32_t> slot_term_begin;
+    std::vector<int32_t> slot_term_param;
+    std::vector<double> slot_term_coeff;
+    std::vector<int8_t> slot_half;
+    // jacobian CSR by parameter: (tap id, coeff)
+    std::vector<int32_t> jac_begin, jac_tap;
+    std::vector<double> jac_coeff;
+    int n_tap_ids = 0;
+
+    std::mutex mu;
+    std::map<std::tuple<int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
+    std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev_slots;  // per device: 6 buffers
+
+    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev) {
+        auto o = mqa::sched_options(dtype, adjoint);
+        int dev = upload_dev ? mqa::current_device() : -1;
+        auto key = std::make_tuple(n_eff, dtype, int(adjoint), o.tile_qubits, o.low_run, o.group_width,
+                                   o.max_ops_per_pass, dev);
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = plans.find(key);
+        if (it != plans.end()) return it->second.get();
+        auto dp = std::make_unique<mqa::DevPlan>();
+        dp->plan = mqa::build_plan(gates, n_eff, adjoint, o);
+        for (auto& op : dp->plan.dev_ops)
+            if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4) dp->full = true;
+        if (upload_dev) {
+            mqa::upload(dp->passes, dp->plan.dev_passes, s);
+            mqa::upload(dp->groups, dp->plan.dev_groups, s);
+            mqa::upload(dp->ops, dp->plan.dev_ops, s);
+            mqa::upload(dp->tap_ids, dp->plan.dev_tap_slots, s);
+            if (dtype == MQA_C64) {
+                std::vector<float> pf(dp->plan.pool.begin(), dp->plan.pool.end());
+                mqa::upload(dp->pool, pf, s);
+            } else {
+                mqa::upload(dp->pool, dp->plan.pool, s);
+            }
+            mqa::hip_check(hipStreamSynchronize(s), "plan upload");
+        }
+        auto* raw = dp.get();
+        plans[key] = std::move(dp);
+        return raw;
+    }
+
+    // device copies of the angle-table inputs and the jacobian (index order fixed below)
+    mqa::DevBuf* slots_on_device(hipStream_t s) {
+        int dev = mqa::current_device();
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = dev_slots.find(dev);
+        if (it != dev_slots.end()) return it->second.get();
+        std::unique_ptr<mqa::DevBuf[]> b(new mqa::DevBuf[8]);
+        mqa::upload(b[0], slot_const, s);
+        mqa::upload(b[1], slot_term_begin, s);
+        mqa::upload(b[2], slot_term_param, s);
+        mqa::upload(b[3], slot_term_coeff, s);
+        mqa::upload(b[4], slot_half, s);
+        mqa::upload(b[5], jac_begin, s);
+        mqa::upload(b[6], jac_tap, s);
+        mqa::upload(b[7], jac_coeff, s);
+        mqa::hip_check(hipStreamSynchronize(s), "slot upload");
+        auto* raw = b.get();
+        dev_slots[dev] = std::move(b);
+        return raw;
+    }
+};
+
+// ---------------------------------------------------------------- Hamiltonian
+struct mqa_ham {
+    std::vector<mqa_pauli_term> terms;
+    std::vector<mqa::DevHamGroup> groups;
+    std::vector<mqa::DevHamTerm> dterms;
+    uint64_t qubit_mask = 0;
+    std::mutex mu;
+    std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev;
+
+    mqa::DevBuf* on_device(hipStream_t s) {
+        int d = mqa::current_device();
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = dev.find(d);
+        if (it != dev.end()) return it->second.get();
+        std::unique_ptr<mqa::DevBuf[]> b(new mqa::DevBuf[2]);
+        mqa::upload(b[0], groups, s);
+        mqa::upload(b[1], dterms, s);
+        mqa::hip_check(hipStreamSynchronize(s), "ham upload");
+        auto* raw = b.get();
+        dev[d] = std::move(b);
+        return raw;
+    }
+};
+
+// ---------------------------------------------------------------- simulator
+struct mqa_sim {
+    int n = 0;
+    int n_eff = 0;
+    int dtype = MQA_C64;
+    uint32_t seed = 0;
+    uint64_t dim = 0, dim_eff = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mqa::DevBuf state;
+    // workspaces (grow-only)
+    mqa::DevBuf psi, lam, psi_keep, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
+    mqa::Timer timer;
+    size_t amp_bytes() const { return dtype == MQA_C64 ? 8 : 16; }
+    size_t state_bytes() const { return dim_eff * amp_bytes(); }
+};
+
+namespace mqa {
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        hip_check(hipGetDevice(&prev), "hipGetDevice");
+        if (prev != d) hip_check(hipSetDevice(d), "hipSetDevice");
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void require(bool ok, const std::string& msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+int n_eff_of(int n) { return std::max(n, kMinTileQubits); }
+
+// Launch `fn` on the simulator's stream, bracketed by HIP events when profiling is on.
+template <typename F>
+void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
+    if (!g_opt.profile) {
+        fn();
+        return;
+    }
+    hipEvent_t a = sim->timer.get(), b = sim->timer.get();
+    hip_check(hipEventRecord(a, sim->stream), "hipEventRecord");
+    fn();
+    hip_check(hipEventRecord(b, sim->stream), "hipEventRecord");
+    sim->timer.pending.push_back({a, b, name, bytes});
+}
+
+void sync(mqa_sim* sim, const char* what) {
+    hip_check(hipStreamSynchronize(sim->stream), what);
+    sim->timer.resolve();
+}
+
+// Run every pass of a plan on `batch` states.
+void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, const double* angles, int n_slots,
+              double* partials, int n_tap_ids) {
+    const Plan& pl = dp->plan;
+    TilePassArgs a{};
+    a.psi = psi;
+    a.lam = lam;
+    a.K = pl.K;
+    a.stride = sim->dim_eff;
+    a.n_tiles = uint64_t(1) << (pl.n_eff - pl.K);
+    a.passes = dp->passes.as<DevPass>();
+    a.groups = dp->groups.as<DevGroup>();
+    a.ops = dp->ops.as<DevOp>();
+    a.pool = dp->pool.p;
+    a.angles = angles;
+    a.n_slots = n_slots;
+    a.n_tap_ids = n_tap_ids;
+    a.tap_ids = dp->tap_ids.as<int32_t>();
+    a.partials = partials;
+    const double bytes = double(batch) * double(sim->state_bytes()) * (pl.adjoint ? 4.0 : 2.0);
+    for (size_t i = 0; i < pl.passes.size(); i++) {
+        a.pass_index = int(i);
+        timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
+            hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C, std::max(pl.max_taps, 1),
+                                       sim->stream),
+                      "tile pass");
+        });
+    }
+}
+
+// (cos, sin) table for `batch` rows: params p < n_enc from enc rows, the rest from ans.
+const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_host, uint64_t batch, int n_enc,
+                           const double* ans_host, int n_ans) {
+    if (circ->n_slots == 0) return nullptr;
+    DevBuf* sl = circ->slots_on_device(sim->stream);
+    hipStream_t s = sim->stream;
+    sim->enc.ensure(std::max<size_t>(batch * n_enc, 1) * sizeof(double));
+    sim->ans.ensure(std::max<size_t>(n_ans, 1) * sizeof(double));
+    if (n_enc > 0)
+        hip_check(hipMemcpyAsync(sim->enc.p, enc_host, batch * n_enc * sizeof(double), hipMemcpyHostToDevice, s),
+                  "enc upload");
+    if (n_ans > 0)
+        hip_check(hipMemcpyAsync(sim->ans.p, ans_host, n_ans * sizeof(double), hipMemcpyHostToDevice, s), "ans upload");
+    sim->angles.ensure(batch * circ->n_slots * 2 * sizeof(double));
+    AngleArgs aa{};
+    aa.enc = sim->enc.as<double>();
+    aa.ans = sim->ans.as<double>();
+    aa.n_enc = n_enc;
+    aa.n_slots = circ->n_slots;
+    aa.batch = batch;
+    aa.const_val = sl[0].as<double>();
+    aa.term_begin = sl[1].as<int32_t>();
+    aa.term_param = sl[2].as<int32_t>();
+    aa.term_coeff = sl[3].as<double>();
+    aa.half = sl[4].as<int8_t>();
+    aa.out = sim->angles.as<double>();
+    hip_check(launch_angles(aa, s), "angles");
+    return sim->angles.as<double>();
+}
+
+// <bra|H|ket> or H ket over a batch; returns per-sample sums in sim->fsum (device, [batch][2]).
+void ham_apply(mqa_sim* sim, mqa_ham* ham, const void* ket, const void* bra, void* out, uint64_t batch) {
+    DevBuf* hd = ham->on_device(sim->stream);
+    HamArgs h{};
+    h.ket = ket;
+    h.ket_stride = sim->dim_eff;
+    h.bra = bra;
+    h.bra_stride = sim->dim_eff;
+    h.out = out;
+    h.out_stride = sim->dim_eff;
+    h.dim = sim->dim_eff;
+    h.amps_per_block = uint32_t(std::min<uint64_t>(sim->dim_eff, 2048));
+    h.n_blocks = uint32_t((sim->dim_eff + h.amps_per_block - 1) / h.amps_per_block);
+    h.groups = hd[0].as<DevHamGroup>();
+    h.n_groups = int(ham->groups.size());
+    h.terms = hd[1].as<DevHamTerm>();
+    sim->fpart.ensure(batch * h.n_blocks * 2 * sizeof(double));
+    sim->fsum.ensure(batch * 2 * sizeof(double));
+    h.partial = bra ? sim->fpart.as<double>() : nullptr;
+    const double passes = 1.0 + (out ? 1.0 : 0.0) + ((bra && bra != ket) ? 1.0 : 0.0);
+    timed(sim, "hamiltonian", double(batch) * double(sim->state_bytes()) * passes, [&] {
+        hip_check(launch_ham(sim->dtype, out != nullptr, h, batch, sim->stream), "hamiltonian");
+    });
+    if (bra)
+        hip_check(launch_reduce_rows(sim->fpart.as<double>(), h.n_blocks, sim->fsum.as<double>(), batch, sim->stream),
+                  "reduce");
+}
+
+void vdot(mqa_sim* sim, const void* a, const void* b, double out[2]) {
+    VdotArgs v{};
+    v.a = a;
+    v.a_stride = sim->dim_eff;
+    v.b = b;
+    v.b_stride = sim->dim_eff;
+    v.dim = sim->dim_eff;
+    v.amps_per_block = uint32_t(std::min<uint64_t>(sim->dim_eff, 4096));
+    v.n_blocks = uint32_t((sim->dim_eff + v.amps_per_block - 1) / v.amps_per_block);
+    sim->fpart.ensure(v.n_blocks * 2 * sizeof(double));
+    sim->fsum.ensure(2 * sizeof(double));
+    v.partial = sim->fpart.as<double>();
+    hip_check(launch_vdot(sim->dtype, v, 1, sim->stream), "vdot");
+    hip_check(launch_reduce_rows(sim->fpart.as<double>(), v.n_blocks, sim->fsum.as<double>(), 1, sim->stream), "reduce");
+    hip_check(hipMemcpyAsync(out, sim->fsum.p, 2 * sizeof(double), hipMemcpyDeviceToHost, sim->stream), "d2h");
+    hip_check(hipStreamSynchronize(sim->stream), "sync");
+}
+
+void check_circuit(const mqa_sim* sim, const mqa_circuit* c, int n_params) {
+    require(c != nullptr, "circuit is NULL");
+    require(c->max_qubit < sim->n, "Simulator has " + std::to_string(sim->n) + " qubits, but circuit has " +
+                                       std::to_string(c->max_qubit + 1) + " qubits.");
+    require(n_params >= c->n_params, "circuit needs " + std::to_string(c->n_params) + " parameters, got " +
+                                         std::to_string(n_params));
+}
+
+void check_ham(const mqa_sim* sim, const mqa_ham* h) {
+    require(h != nullptr, "hamiltonian is NULL");
+    require((h->qubit_mask >> sim->n) == 0 || sim->n >= 64, "hamiltonian acts on more qubits than the simulator has");
+}
+
+size_t hbm_budget() {
+    int64_t b = g_opt.budget;
+    if (const char* env = std::getenv("MQAMD_HBM_BUDGET_BYTES")) b = std::atoll(env);
+    if (b > 0) return size_t(b);
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    return size_t(double(free_b) * 0.85);
+}
+
+}  // namespace
+}  // namespace mqa
+
+using namespace mqa;
+
+extern "C" {
+
+const char* mqa_last_error(void) { return g_last_error.c_str(); }
+
+const char* mqa_version(void) { return "mqamd 0.1.0 (gfx950)"; }
+
+int mqa_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int mqa_set_option(const char* key, int64_t value) {
+    return guarded([&] {
+        require(key != nullptr, "option key is NULL");
+        for (auto& e : opt_table()) {
+            if (std::strcmp(e.key, key) == 0) {
+                *e.v = value < 0 ? e.def : value;
+                return;
+            }
+        }
+        throw std::invalid_argument(std::string("unknown option ") + key);
+    });
+}
+
+int mqa_get_option(const char* key, int64_t* value) {
+    return guarded([&] {
+        require(key != nullptr && value != nullptr, "NULL argument");
+        for (auto& e : opt_table()) {
+            if (std::strcmp(e.key, key) == 0) {
+                *value = *e.v;
+                return;
+            }
+        }
+        throw std::invalid_argument(std::string("unknown option ") + key);
+    });
+}
+
+int mqa_profile_get(const char* kernel, int64_t* launches, double* total_ms, double* bytes) {
+    return guarded([&] {
+        require(kernel != nullptr, "kernel name is NULL");
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        KernelStats st;
+        auto it = g_prof.find(kernel);
+        if (it != g_prof.end()) st = it->second;
+        if (launches) *launches = st.launches;
+        if (total_ms) *total_ms = st.ms;
+        if (bytes) *bytes = st.bytes;
+    });
+}
+
+int mqa_profile_reset(void) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        g_prof.clear();
+    });
+}
+
+int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* term_param, const double* term_coeff,
+                       int32_t n_terms, const double* mats, int64_t n_mat_entries, int32_t n_params,
+                       const uint8_t* param_requires_grad, mqa_circuit** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_gates >= 0 && (n_gates == 0 || gates != nullptr), "bad gate array");
+        require(n_params >= 0, "bad parameter count");
+        auto c = std::make_unique<mqa_circuit>();
+        c->n_params = n_params;
+        std::vector<std::vector<std::pair<int, double>>> by_param(n_params);
+        for (int i = 0; i < n_gates; i++) {
+            const mqa_gate& s = gates[i];
+            CGate g;
+            g.kind = s.kind;
+            require(s.n_obj >= 1 && s.n_obj <= MQA_MAX_OBJ, "gate " + std::to_string(i) + ": bad target count");
+            for (int k = 0; k < s.n_obj; k++) {
+                require(s.obj[k] >= 0 && s.obj[k] < 64, "gate " + std::to_string(i) + ": bad qubit");
+                g.objs.push_back(s.obj[k]);
+                c->max_qubit = std::max(c->max_qubit, s.obj[k]);
+            }
+            g.ctrl = s.ctrl_mask;
+            for (int q = 0; q < 64; q++)
+                if ((s.ctrl_mask >> q) & 1) c->max_qubit = std::max(c->max_qubit, q);
+            require(!(g.targets() & g.ctrl), "gate " + std::to_string(i) + ": a qubit is both target and control");
+            bool param_kind = s.kind >= MQA_GATE_RX && s.kind <= MQA_GATE_RZZ;
+            bool fixed_kind = (s.kind >= MQA_GATE_I && s.kind <= MQA_GATE_MATRIX);
+            if (!param_kind && !fixed_kind)
+                throw Error(MQA_ERR_UNSUPPORTED, "Apply of gate " + std::to_string(s.kind) + " not implement.");
+            int want = 1;
+            if (s.kind == MQA_GATE_SWAP || s.kind == MQA_GATE_ISWAP || s.kind == MQA_GATE_ISWAPDAG ||
+                s.kind == MQA_GATE_RXX || s.kind == MQA_GATE_RYY || s.kind == MQA_GATE_RZZ)
+                want = 2;
+            if (s.kind != MQA_GATE_MATRIX)
+                require(s.n_obj == want, "gate " + std::to_string(i) + ": wrong number of target qubits");
+            if (s.kind == MQA_GATE_MATRIX) {
+                int64_t m = int64_t(1) << s.n_obj;
+                require(mats != nullptr && s.mat_begin >= 0 && s.mat_begin + m * m <= n_mat_entries,
+                        "gate " + std::to_string(i) + ": matrix out of range");
+                for (int64_t e = 0; e < m * m; e++)
+                    g.mat.push_back({mats[2 * (s.mat_begin + e)], mats[2 * (s.mat_begin + e) + 1]});
+            }
+            if (param_kind) {
+                g.slot = c->n_slots++;
+                g.const_val = s.const_val;
+                require(s.n_term >= 0 && s.term_begin >= 0 && s.term_begin + s.n_term <= n_terms,
+                        "gate " + std::to_string(i) + ": parameter terms out of range");
+                c->slot_const.push_back(s.const_val);
+                c->slot_term_begin.push_back(int32_t(c->slot_term_param.size()));
+                c->slot_half.push_back((s.kind == MQA_GATE_PS || s.kind == MQA_GATE_GP) ? 0 : 1);
+                for (int t = 0; t < s.n_term; t++) {
+                    int p = term_param[s.term_begin + t];
+                    require(p >= 0 && p < n_params, "gate " + std::to_string(i) + ": parameter index out of range");
+                    g.terms.push_back({p, term_coeff[s.term_begin + t]});
+                    c->slot_term_param.push_back(p);
+                    c->slot_term_coeff.push_back(term_coeff[s.term_begin + t]);
+                }
+                g.grad = s.requires_grad != 0 && s.n_term > 0;
+                if (g.grad) {
+                    int tap = c->n_tap_ids++;
+                    for (auto& t : g.terms) by_param[t.first].push_back({tap, t.second});
+                }
+            }
+            c->gates.push_back(std::move(g));
+        }
+        c->slot_term_begin.push_back(int32_t(c->slot_term_param.size()));
+        c->jac_begin.push_back(0);
+        for (int p = 0; p < n_params; p++) {
+            if (param_requires_grad && !param_requires_grad[p]) by_param[p].clear();
+            for (auto& e : by_param[p]) {
+                c->jac_tap.push_back(e.first);
+                c->jac_coeff.push_back(e.second);
+            }
+            c->jac_begin.push_back(int32_t(c->jac_tap.size()));
+        }
+        *out = c.release();
+    });
+}
+
+int mqa_circuit_destroy(mqa_circuit* circ) {
+    return guarded([&] { delete circ; });
+}
+
+int mqa_circuit_schedule(const mqa_circuit* circ_c, int32_t n_qubits, int32_t dtype, int32_t adjoint, int64_t* out,
+                         int64_t cap, int64_t* len) {
+    return guarded([&] {
+        require(circ_c != nullptr && len != nullptr, "NULL argument");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
+        require(n_qubits > circ_c->max_qubit && n_qubits <= 40, "bad qubit count");
+        auto* circ = const_cast<mqa_circuit*>(circ_c);
+        DevPlan* dp = circ->get_plan(n_eff_of(n_qubits), dtype, adjoint != 0, nullptr, false);
+        std::vector<int64_t> w;
+        w.push_back(int64_t(dp->plan.passes.size()));
+        for (auto& ps : dp->plan.passes) {
+            uint64_t L = 0;
+            for (int q : ps.local_qubits) L |= uint64_t(1) << q;
+            w.push_back(int64_t(L));
+            int64_t cnt = 0;
+            for (auto& g : ps.groups) cnt += int64_t(g.ops.size());
+            w.push_back(cnt);
+            for (auto& g : ps.groups)
+                for (auto& op : g.ops) w.push_back(op.gate);
+        }
+        *len = int64_t(w.size());
+        if (out && cap >= int64_t(w.size())) std::memcpy(out, w.data(), w.size() * sizeof(int64_t));
+    });
+}
+
+int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(n_terms >= 0 && (n_terms == 0 || terms != nullptr), "bad term array");
+        auto h = std::make_unique<mqa_ham>();
+        std::map<uint64_t, std::vector<int>> by_flip;
+        for (int t = 0; t < n_terms; t++) {
+            const auto& tm = terms[t];
+            require((tm.x_mask & tm.y_mask) == 0 && (tm.x_mask & tm.z_mask) == 0 && (tm.y_mask & tm.z_mask) == 0,
+                    "pauli masks overlap");
+            h->terms.push_back(tm);
+            h->qubit_mask |= tm.x_mask | tm.y_mask | tm.z_mask;
+            by_flip[tm.x_mask | tm.y_mask].push_back(t);
+        }
+        for (auto& [flip, ids] : by_flip) {
+            DevHamGroup g{};
+            g.flip = flip;
+            g.term_begin = int32_t(h->dterms.size());
+            g.n_terms = int32_t(ids.size());
+            for (int t : ids) {
+                const auto& tm = terms[t];
+                // c * i^{|y|}: P|src> = i^{|y|} (-1)^{popcount(src & (y|z))} |src ^ (x|y)>
+                int ny = __builtin_popcountll(tm.y_mask) & 3;
+                static const double pr[4] = {1, 0, -1, 0}, pi[4] = {0, 1, 0, -1};
+                DevHamTerm d{};
+                d.yz = tm.y_mask | tm.z_mask;
+                d.coef_re = tm.coeff * pr[ny];
+                d.coef_im = tm.coeff * pi[ny];
+                h->dterms.push_back(d);
+            }
+            h->groups.push_back(g);
+        }
+        if (h->groups.empty()) {  // zero operator: one empty group keeps the kernels uniform
+            DevHamGroup g{};
+            g.flip = 0;
+            g.term_begin = 0;
+            g.n_terms = 0;
+            h->groups.push_back(g);
+        }
+        *out = h.release();
+    });
+}
+
+int mqa_ham_destroy(mqa_ham* ham) {
+    return guarded([&] { delete ham; });
+}
+
+int mqa_sim_create(int32_t n_qubits, int32_t dtype, uint32_t seed, mqa_sim** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "dtype must be MQA_C64 or MQA_C128");
+        require(n_qubits >= 1 && n_qubits <= 36, "Dimension too large.");
+        if (mqa_device_count() <= 0) throw Error(MQA_ERR_HIP, "no HIP device is visible");
+        auto s = std::make_unique<mqa_sim>();
+        s->n = n_qubits;
+        s->n_eff = n_eff_of(n_qubits);
+        s->dtype = dtype;
+        s->seed = seed;
+        s->dim = uint64_t(1) << n_qubits;
+        s->dim_eff = uint64_t(1) << s->n_eff;
+        s->device = current_device();
+        hip_check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        s->state.ensure(s->state_bytes());
+        hip_check(launch_fill(dtype, s->state.p, nullptr, s->dim_eff, s->dim_eff, 0, 1, s->stream), "init");
+        hip_check(hipStreamSynchronize(s->stream), "sync");
+        *out = s.release();
+    });
+}
+
+int mqa_sim_destroy(mqa_sim* sim) {
+    return guarded([&] {
+        if (!sim) return;
+        {
+            DeviceGuard dg(sim->device);
+            if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+            if (sim->stream) (void)hipStreamDestroy(sim->stream);
+            sim->stream = nullptr;
+        }
+        delete sim;
+    });
+}
+
+int mqa_sim_reset(mqa_sim* sim) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        DeviceGuard dg(sim->device);
+        hip_check(launch_fill(sim->dtype, sim->state.p, nullptr, sim->dim_eff, sim->dim_eff, 0, 1, sim->stream), "reset");
+        hip_check(hipStreamSynchronize(sim->stream), "sync");
+    });
+}
+
+int mqa_sim_copy(const mqa_sim* sim, mqa_sim** out) {
+    return guarded([&] {
+        require(sim != nullptr && out != nullptr, "NULL argument");
+        DeviceGuard dg(sim->device);
+        mqa_sim* c = nullptr;
+        int st = mqa_sim_create(sim->n, sim->dtype, sim->seed, &c);
+        if (st != MQA_OK) throw Error(st, g_last_error);
+        hip_check(hipMemcpyAsync(c->state.p, sim->state.p, sim->state_bytes(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        *out = c;
+    });
+}
+
+int mqa_sim_n_qubits(const mqa_sim* sim, int32_t* n_qubits, int32_t* dtype) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        if (n_qubits) *n_qubits = sim->n;
+        if (dtype) *dtype = sim->dtype;
+    });
+}
+
+int mqa_sim_get_qs(const mqa_sim* sim, void* host_out, uint64_t dim) {
+    return guarded([&] {
+        require(sim != nullptr && host_out != nullptr, "NULL argument");
+        require(dim == sim->dim, "state size not match");
+        DeviceGuard dg(sim->device);
+        hip_check(hipMemcpyAsync(host_out, sim->state.p, sim->dim * sim->amp_bytes(), hipMemcpyDeviceToHost, sim->stream),
+                  "get_qs");
+        hip_check(hipStreamSynchronize(sim->stream), "sync");
+    });
+}
+
+int mqa_sim_set_qs(mqa_sim* sim, const void* host_in, uint64_t dim) {
+    return guarded([&] {
+        require(sim != nullptr && host_in != nullptr, "NULL argument");
+        require(dim == sim->dim, "state size not match");
+        DeviceGuard dg(sim->device);
+        if (sim->dim_eff != sim->dim)
+            hip_check(hipMemsetAsync(sim->state.p, 0, sim->state_bytes(), sim->stream), "set_qs clear");
+        hip_check(hipMemcpyAsync(sim->state.p, host_in, sim->dim * sim->amp_bytes(), hipMemcpyHostToDevice, sim->stream),
+                  "set_qs");
+        hip_check(hipStreamSynchronize(sim->stream), "sync");
+    });
+}
+
+int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double* params, int32_t n_params) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        check_circuit(sim, circ_c, n_params);
+        auto* circ = const_cast<mqa_circuit*>(circ_c);
+        if (circ->gates.empty()) return;
+        DeviceGuard dg(sim->device);
+        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
+        const double* ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
+        run_plan(sim, dp, sim->state.p, nullptr, 1, ang, circ->n_slots, nullptr, 0);
+        sync(sim, "apply_circuit");
+    });
+}
+
+int mqa_sim_apply_hamiltonian(mqa_sim* sim, const mqa_ham* ham_c) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        check_ham(sim, ham_c);
+        auto* ham = const_cast<mqa_ham*>(ham_c);
+        DeviceGuard dg(sim->device);
+        sim->scratch_state.ensure(sim->state_bytes());
+        ham_apply(sim, ham, sim->state.p, nullptr, sim->scratch_state.p, 1);
+        hip_check(hipMemcpyAsync(sim->state.p, sim->scratch_state.p, sim->state_bytes(), hipMemcpyDeviceToDevice,
+                                 sim->stream),
+                  "copy");
+        sync(sim, "apply_hamiltonian");
+    });
+}
+
+int mqa_sim_expectation_lr(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_ham* ham_c,
+                           const mqa_circuit* circ_right, const mqa_circuit* circ_left, const double* params,
+                           int32_t n_params, double out[2]) {
+    return guarded([&] {
+        require(sim_c != nullptr && out != nullptr, "NULL argument");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        auto* left = const_cast<mqa_sim*>(left_c);
+        check_ham(sim, ham_c);
+        if (left) {
+            require(left->n == sim->n && left->dtype == sim->dtype, "simulator_left does not match this simulator");
+        }
+        if (circ_right) check_circuit(sim, circ_right, n_params);
+        if (circ_left) check_circuit(sim, circ_left, n_params);
+        auto* ham = const_cast<mqa_ham*>(ham_c);
+        DeviceGuard dg(sim->device);
+        const size_t sb = sim->state_bytes();
+        // ket = Ur |sim>, bra = Ul |left> in two workspace states
+        sim->psi.ensure(sb);
+        sim->lam.ensure(sb);
+        hip_check(hipMemcpyAsync(sim->psi.p, sim->state.p, sb, hipMemcpyDeviceToDevice, sim->stream), "copy");
+        if (circ_right && !circ_right->gates.empty()) {
+            auto* cr = const_cast<mqa_circuit*>(circ_right);
+            DevPlan* dp = cr->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
+            const double* ang = build_angles(sim, cr, nullptr, 1, 0, params, n_params);
+            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, cr->n_slots, nullptr, 0);
+        }
+        const void* bra = sim->psi.p;
+        bool two = (left != nullptr && left != sim) || (circ_left != nullptr && circ_left != circ_right);
+        if (two) {
+            const mqa_sim* src = left ? left : sim;
+            hip_check(hipMemcpyAsync(sim->lam.p, src->state.p, sb, hipMemcpyDeviceToDevice, sim->stream), "copy");
+            if (left && left->stream != sim->stream) hip_check(hipStreamSynchronize(left->stream), "sync");
+            const mqa_circuit* cl = circ_left ? circ_left : circ_right;
+            if (cl && !cl->gates.empty()) {
+                auto* clm = const_cast<mqa_circuit*>(cl);
+                DevPlan* dp = clm->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
+                const double* ang = build_angles(sim, clm, nullptr, 1, 0, params, n_params);
+                run_plan(sim, dp, sim->lam.p, nullptr, 1, ang, clm->n_slots, nullptr, 0);
+            }
+            bra = sim->lam.p;
+        }
+        ham_apply(sim, ham, sim->psi.p, bra, nullptr, 1);
+        hip_check(hipMemcpyAsync(out, sim->fsum.p, 2 * sizeof(double), hipMemcpyDeviceToHost, sim->stream), "d2h");
+        sync(sim, "expectation");
+    });
+}
+
+int mqa_sim_expectation(const mqa_sim* sim, const mqa_ham* ham, const mqa_circuit* circ, const double* params,
+                        int32_t n_params, double out[2]) {
+    return mqa_sim_expectation_lr(sim, nullptr, ham, circ, nullptr, params, n_params, out);
+}
+
+int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ_c,
+                       const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
+                       double* g_out) {
+    return guarded([&] {
+        require(sim_c != nullptr && circ_c != nullptr, "NULL argument");
+        require(n_hams >= 1 && hams != nullptr, "need at least one hamiltonian");
+        require(batch >= 1, "batch must be positive");
+        require(n_enc >= 0 && n_ans >= 0, "bad parameter counts");
+        require(n_enc == 0 || enc != nullptr, "enc is NULL");
+        require(n_ans == 0 || ans != nullptr, "ans is NULL");
+        require(f_out != nullptr && g_out != nullptr, "output is NULL");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        auto* circ = const_cast<mqa_circuit*>(circ_c);
+        const int n_params = n_enc + n_ans;
+        check_circuit(sim, circ, n_params);
+        for (int h = 0; h < n_hams; h++) check_ham(sim, hams[h]);
+        DeviceGuard dg(sim->device);
+        hipStream_t s = sim->stream;
+
+        DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
+        DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true);
+        DevBuf* sl = circ->slots_on_device(s);
+        const int n_tap_ids = circ->n_tap_ids;
+        const uint64_t n_tiles = uint64_t(1) << (adj->plan.n_eff - adj->plan.K);
+        const size_t sb = sim->state_bytes();
+
+        // samples per chunk from the HBM budget: psi + lam (+ a kept copy for several hamiltonians)
+        const size_t per_sample = sb * (n_hams > 1 ? 3 : 2) + size_t(n_tap_ids) * n_tiles * 16 +
+                                  size_t(circ->n_slots) * 16 + size_t(n_tap_ids + n_params) * 16 + 4096;
+        const size_t budget = hbm_budget();
+        int64_t chunk = int64_t(std::max<size_t>(budget / per_sample, 1));
+        chunk = std::min<int64_t>(chunk, batch);
+        chunk = std::min<int64_t>(chunk, 65535);
+
+        sim->psi.ensure(sb * chunk);
+        sim->lam.ensure(sb * chunk);
+        if (n_hams > 1) sim->psi_keep.ensure(sb * chunk);
+        sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 16, 16));
+        sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 16, 16));
+        sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
+        std::vector<double> g_host(size_t(chunk) * n_params), f_host(size_t(chunk) * 2);
+
+        for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+            const int64_t nb = std::min<int64_t>(chunk, batch - b0);
+            const double* ang = build_angles(sim, circ, n_enc ? enc + b0 * n_enc : nullptr, nb, n_enc, ans, n_ans);
+            hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff, nb, s),
+                      "fill");
+            run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+            if (n_hams > 1)
+                hip_check(hipMemcpyAsync(sim->psi_keep.p, sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s), "keep");
+            for (int h = 0; h < n_hams; h++) {
+                if (h > 0)
+                    hip_check(hipMemcpyAsync(sim->psi.p, sim->psi_keep.p, sb * nb, hipMemcpyDeviceToDevice, s), "restore");
+                auto* ham = const_cast<mqa_ham*>(hams[h]);
+                // lam = H psi, f = <psi|H psi>
+                ham_apply(sim, ham, sim->psi.p, sim->psi.p, sim->lam.p, nb);
+                hip_check(hipMemcpyAsync(f_host.data(), sim->fsum.p, nb * 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+                          "d2h f");
+                if (n_tap_ids > 0) {
+                    run_plan(sim, adj, sim->psi.p, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(),
+                             n_tap_ids);
+                    hip_check(launch_reduce_rows(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
+                                                 uint64_t(nb) * n_tap_ids, s),
+                              "reduce taps");
+                }
+                if (n_params > 0) {
+                    if (n_tap_ids > 0) {
+                        hip_check(launch_jacobian(sim->taps.as<double>(), n_tap_ids, sl[5].as<int32_t>(),
+                                                  sl[6].as<int32_t>(), sl[7].as<double>(), n_params,
+                                                  sim->grads.as<double>(), nb, s),
+                                  "jacobian");
+                        hip_check(hipMemcpyAsync(g_host.data(), sim->grads.p, size_t(nb) * n_params * sizeof(double),
+                                                 hipMemcpyDeviceToHost, s),
+                                  "d2h g");
+                    } else {
+                        std::fill(g_host.begin(), g_host.end(), 0.0);
+                    }
+                }
+                sync(sim, "grad batch");
+                for (int64_t b = 0; b < nb; b++) {
+                    double* fo = f_out + ((b0 + b) * n_hams + h) * 2;
+                    fo[0] = f_host[2 * b];
+                    fo[1] = f_host[2 * b + 1];
+                    double* go = g_out + ((b0 + b) * n_hams + h) * size_t(n_params);
+                    for (int p = 0; p < n_params; p++) go[p] = g_host[size_t(b) * n_params + p];
+                }
+            }
+        }
+    });
+}
+
+int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out) {
+    return guarded([&] {
+        require(sim != nullptr && out != nullptr, "NULL argument");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
+        DeviceGuard dg(sim->device);
+        mqa_sim* c = nullptr;
+        int st = mqa_sim_create(sim->n, dtype, sim->seed, &c);
+        if (st != MQA_OK) throw Error(st, g_last_error);
+        if (dtype == sim->dtype) {
+            hip_check(hipMemcpyAsync(c->state.p, sim->state.p, sim->state_bytes(), hipMemcpyDeviceToDevice, c->stream),
+                      "copy");
+        } else {
+            hip_check(hipStreamSynchronize(sim->stream), "sync");
+            hip_check(launch_cast(sim->dtype, sim->state.p, c->state.p, sim->dim_eff * 2, c->stream), "cast");
+        }
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        *out = c;
+    });
+}
+
+int mqa_inner_product(const mqa_sim* bra, const mqa_sim* ket, double out[2]) {
+    return guarded([&] {
+        require(bra != nullptr && ket != nullptr && out != nullptr, "NULL argument");
+        require(bra->n == ket->n && bra->dtype == ket->dtype, "states do not match");
+        auto* b = const_cast<mqa_sim*>(bra);
+        DeviceGuard dg(b->device);
+        if (ket->stream != b->stream) hip_check(hipStreamSynchronize(ket->stream), "sync");
+        vdot(b, b->state.p, ket->state.p, out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// launch.h - kernel argument blocks and launchers (kernels.hip) used by the C ABI (capi.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "program.h"
+
+namespace mqa {
+
+// dtype: 0 = complex64 (float2 amplitudes), 1 = complex128 (double2 amplitudes).
+
+struct TilePassArgs {
+    void* psi;          // [batch][stride] amplitudes
+    void* lam;          // co-state (adjoint passes only)
+    int K;              // tile qubits
+    int pass_index;
+    uint64_t stride;    // amplitudes between consecutive samples
+    uint64_t n_tiles;   // tiles per sample = 2^(n - K)
+    const DevPass* passes;
+    const DevGroup* groups;
+    const DevOp* ops;
+    const void* pool;      // matrix pool in the state's real type (float or double)
+    const double* angles;  // [batch][n_slots][2] (cos, sin) (NULL without parameterised ops)
+    int n_slots;
+    int n_tap_ids;
+    const int32_t* tap_ids;
+    double* partials;      // [batch][n_tap_ids][n_tiles] (re, im)
+};
+
+struct HamArgs {
+    const void* ket;
+    uint64_t ket_stride;
+    const void* bra;  // NULL: no reduction
+    uint64_t bra_stride;
+    void* out;        // H ket (WRITE kernels)
+    uint64_t out_stride;
+    uint64_t dim;
+    uint32_t amps_per_block;
+    uint32_t n_blocks;
+    const DevHamGroup* groups;
+    int n_groups;
+    const DevHamTerm* terms;
+    double* partial;  // [batch][n_blocks] (re, im) or NULL
+};
+
+struct VdotArgs {
+    const void* a;
+    uint64_t a_stride;
+    const void* b;
+    uint64_t b_stride;
+    uint64_t dim;
+    uint32_t amps_per_block;
+    uint32_t n_blocks;
+    double* partial;  // [batch][n_blocks] (re, im)
+};
+
+struct AngleArgs {
+    const double* enc;         // [batch][n_enc]
+    const double* ans;         // [n_ans]
+    int n_enc;
+    int n_slots;
+    uint64_t batch;
+    const double* const_val;   // [n_slots]
+    const int32_t* term_begin; // [n_slots + 1]
+    const int32_t* term_param;
+    const double* term_coeff;
+    const int8_t* half;        // [n_slots]
+    double* out;               // [batch][n_slots][2]
+};
+
+size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps);
+// full: the pass holds dense 4x4 / 8x8 / 16x16 blocks (a heavier instantiation)
+hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
+                            int max_taps, hipStream_t s);
+hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s);
+hipError_t launch_vdot(int dtype, const VdotArgs& a, uint64_t batch, hipStream_t s);
+hipError_t launch_fill(int dtype, void* dst, const void* src, uint64_t dim, uint64_t stride, uint64_t src_dim,
+                       uint64_t batch, hipStream_t s);
+hipError_t launch_cast(int src_dtype, const void* src, void* dst, uint64_t n_reals, hipStream_t s);
+hipError_t launch_angles(const AngleArgs& a, hipStream_t s);
+hipError_t launch_reduce_rows(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
+hipError_t launch_jacobian(const double* taps, int n_tap_ids, const int32_t* jac_begin, const int32_t* jac_tap,
+                           const double* jac_coeff, int n_params, double* g, uint64_t batch, hipStream_t s);
+
+}  // namespace mqa

@@ -1,0 +1,92 @@
+// program.h - the tile-pass program format shared by the host scheduler and the gfx950 kernels.
+//
+// A circuit is executed as a sequence of *passes*.  A pass streams every amplitude of the
+// state(s) through LDS exactly once: the amplitude index space is split into tiles of 2^K
+// amplitudes whose K "local" qubits are chosen per pass (always including the lowest qubits,
+// so that global loads are runs of contiguous amplitudes).  Inside a tile, gates whose target
+// qubits are all local are applied in *groups*: a group names 4 local positions; every thread
+// gathers the 16 amplitudes spanned by those positions into registers, applies all of the
+// group's ops there, and writes them back.  One pass therefore costs one HBM read and one
+// HBM write of the state no matter how many gates it holds.
+//
+// The same program drives the adjoint back-sweep: there every op applies G^dagger to both the
+// forward state and the co-state, and parameterised ops first emit a derivative tap.
+#pragma once
+#include <stdint.h>
+
+namespace mqa {
+
+constexpr int kGroupWidth = 4;                   // register positions per group
+constexpr int kAmpsPerThread = 1 << kGroupWidth;  // 16 amplitudes per thread per state
+constexpr int kMaxTapsPerPass = 96;               // LDS tap slots per pass
+constexpr int kMinTileQubits = 6;                 // smallest tile: 4 threads
+constexpr int kMaxTileQubits = 13;                // largest tile: 512 threads
+
+// Device op kinds.
+enum OpKind : int32_t {
+    OPK_X = 1,   // swap the pair (X, CNOT, Toffoli...)
+    OPK_M1 = 5,  // dense 2x2 from the pool (Y, Z, H, S, T, SX, 1-qubit matrices)
+    OPK_SWAP = 6,
+    OPK_M2 = 7,  // dense 4x4 from the pool (ISWAP, 2-qubit matrices)
+    OPK_M3 = 8,  // dense 8x8 over group positions 0..2 (3-qubit matrices)
+    OPK_M4 = 9,  // dense 16x16 over group positions 0..3 (4-qubit matrices)
+    // parameterised: the angle comes from the per-sample angle table
+    OPK_RX = 16,
+    OPK_RY = 17,
+    OPK_RZ = 18,
+    OPK_PS = 19,
+    OPK_GP = 20,
+    OPK_RXX = 21,
+    OPK_RYY = 22,
+    OPK_RZZ = 23,
+};
+
+inline bool opk_is_param(int k) { return k >= OPK_RX; }
+inline bool opk_is_two(int k) {
+    return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ;
+}
+
+struct DevOp {             // 32 bytes
+    int32_t kind;          // OpKind
+    int32_t slot;          // angle slot (param kinds), -1 otherwise
+    int32_t tap;           // tap index inside the pass (adjoint), -1: none
+    int32_t mat;           // offset in doubles into the matrix pool (fixed kinds)
+    uint8_t q0, q1;        // register positions (0..3) of matrix bit 0 / bit 1
+    uint8_t pad0, pad1;
+    uint32_t ctrl_local;   // control qubits that are local, as a mask over tile positions
+    uint64_t ctrl_global;  // control qubits that are not local, as a global index mask
+};
+
+struct DevGroup {          // 16 bytes
+    uint8_t pos[4];        // local positions; register index bit i <=> pos[i]
+    uint8_t spos[4];       // the same positions sorted ascending
+    int32_t op_begin;
+    int32_t n_ops;
+};
+
+struct DevPass {           // 48 bytes
+    uint64_t nonlocal_mask;  // global qubits that index the tiles
+    uint64_t local_hi_mask;  // local qubits above the contiguous low run
+    int32_t group_begin;
+    int32_t n_groups;
+    int32_t tap_begin;       // first entry of this pass in the tap -> slot table
+    int32_t n_taps;
+    int32_t low_run;         // number of contiguous low qubits (C)
+    int32_t pad;
+};
+
+// Pauli-term Hamiltonian grouped by flip mask (x|y): out[j] = sum_g sum_t coef_t *
+// (-1)^{popcount(src & yz_t)} * psi[src], src = j ^ flip_g, coef_t = c_t * i^{|y_t|}.
+struct DevHamGroup {
+    uint64_t flip;
+    int32_t term_begin;
+    int32_t n_terms;
+};
+struct DevHamTerm {
+    uint64_t yz;
+    double coef_re;
+    double coef_im;
+    double pad;
+};
+
+}  // namespace mqa

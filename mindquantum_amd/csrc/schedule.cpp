@@ -1,0 +1,298 @@
+// schedule.cpp - cache-blocked tile-pass scheduler.
+//
+// The reference applies one gate per full sweep of the state (vector_state.tpp:948-960,
+// one OpenMP loop per gate).  On MI355X every sweep is an HBM round trip, so this scheduler
+// packs as many gates as the dependency order allows into each pass: a pass picks K local
+// qubits, and every not-yet-applied gate whose targets are local and which does not depend
+// on a deferred gate is applied inside the tiles.  Controls never need to be local.
+#include "schedule.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "../../include/mqamd.h"
+
+namespace mqa {
+
+uint64_t CGate::targets() const {
+    uint64_t m = 0;
+    for (int q : objs) m |= uint64_t(1) << q;
+    return m;
+}
+uint64_t CGate::qubits() const { return targets() | ctrl; }
+
+static int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+std::vector<cplx> fixed_matrix(const CGate& g) {
+    const double r = M_SQRT1_2;
+    const cplx I(0, 1);
+    switch (g.kind) {
+        case MQA_GATE_X: return {0, 1, 1, 0};
+        case MQA_GATE_Y: return {0, -I, I, 0};
+        case MQA_GATE_Z: return {1, 0, 0, -1};
+        case MQA_GATE_H: return {r, r, r, -r};
+        case MQA_GATE_S: return {1, 0, 0, I};
+        case MQA_GATE_SDAG: return {1, 0, 0, -I};
+        case MQA_GATE_T: return {1, 0, 0, cplx(r, r)};
+        case MQA_GATE_TDAG: return {1, 0, 0, cplx(r, -r)};
+        case MQA_GATE_SX: return {cplx(0.5, 0.5), cplx(0.5, -0.5), cplx(0.5, -0.5), cplx(0.5, 0.5)};
+        case MQA_GATE_SXDAG: return {cplx(0.5, -0.5), cplx(0.5, 0.5), cplx(0.5, 0.5), cplx(0.5, -0.5)};
+        case MQA_GATE_SWAP: return {1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1};
+        case MQA_GATE_ISWAP: return {1, 0, 0, 0, 0, 0, I, 0, 0, I, 0, 0, 0, 0, 0, 1};
+        case MQA_GATE_ISWAPDAG: return {1, 0, 0, 0, 0, 0, -I, 0, 0, -I, 0, 0, 0, 0, 0, 1};
+        case MQA_GATE_MATRIX: return g.mat;
+        default: throw std::invalid_argument("gate kind " + std::to_string(g.kind) + " has no fixed matrix");
+    }
+}
+
+static std::vector<cplx> dagger(const std::vector<cplx>& m) {
+    size_t d = size_t(std::lround(std::sqrt(double(m.size()))));
+    std::vector<cplx> h(m.size());
+    for (size_t r = 0; r < d; r++)
+        for (size_t c = 0; c < d; c++) h[r * d + c] = std::conj(m[c * d + r]);
+    return h;
+}
+
+// Device op kind of a gate, plus its pool matrix (already daggered for the back-sweep).
+static int lower_kind(const CGate& g, bool adjoint, std::vector<cplx>* mat) {
+    mat->clear();
+    switch (g.kind) {
+        case MQA_GATE_X: return OPK_X;
+        case MQA_GATE_SWAP: return OPK_SWAP;
+        case MQA_GATE_Y:
+        case MQA_GATE_Z:
+        case MQA_GATE_S:
+        case MQA_GATE_SDAG:
+        case MQA_GATE_T:
+        case MQA_GATE_TDAG:
+        case MQA_GATE_H:
+        case MQA_GATE_SX:
+        case MQA_GATE_SXDAG:
+        case MQA_GATE_ISWAP:
+        case MQA_GATE_ISWAPDAG:
+        case MQA_GATE_MATRIX: {
+            auto m = fixed_matrix(g);
+            if (adjoint) m = dagger(m);
+            *mat = m;
+            switch (g.objs.size()) {
+                case 1: return OPK_M1;
+                case 2: return OPK_M2;
+                case 3: return OPK_M3;
+                case 4: return OPK_M4;
+                default: throw std::invalid_argument("matrix gate acts on too many qubits");
+            }
+        }
+        case MQA_GATE_RX: return OPK_RX;
+        case MQA_GATE_RY: return OPK_RY;
+        case MQA_GATE_RZ: return OPK_RZ;
+        case MQA_GATE_PS: return OPK_PS;
+        case MQA_GATE_GP: return OPK_GP;
+        case MQA_GATE_RXX: return OPK_RXX;
+        case MQA_GATE_RYY: return OPK_RYY;
+        case MQA_GATE_RZZ: return OPK_RZZ;
+        default: throw std::invalid_argument("Apply of gate " + std::to_string(g.kind) + " not implement.");
+    }
+}
+
+namespace {
+
+struct PassBuilder {
+    const std::vector<CGate>& gates;
+    bool adjoint;
+    int W;
+    std::vector<int> tap_id_of_slot;  // angle slot -> global tap id (-1: no tap)
+
+    // Split the pass's gates (dependency-ordered) into register groups of <= W target positions.
+    void groups(HostPass* pass, const std::vector<int>& chosen, const std::vector<int>& pos_of, uint64_t nonlocal,
+                std::vector<double>* pool) {
+        std::vector<int> todo = chosen;
+        while (!todo.empty()) {
+            HostGroup grp;
+            std::vector<int> gpos;  // positions in register order
+            std::vector<int> keep;
+            uint64_t blocked = 0;
+            bool closed = false;
+            for (int gi : todo) {
+                const CGate& g = gates[gi];
+                uint64_t q = g.qubits();
+                if (closed || (q & blocked)) {
+                    blocked |= q;
+                    keep.push_back(gi);
+                    continue;
+                }
+                std::vector<cplx> m;
+                int kind = lower_kind(g, adjoint, &m);
+                if (kind == OPK_M3 || kind == OPK_M4) {
+                    if (!grp.ops.empty()) {  // dense multi-qubit blocks take a group of their own
+                        blocked |= q;
+                        keep.push_back(gi);
+                        continue;
+                    }
+                    for (int t : g.objs) gpos.push_back(pos_of[t]);
+                    closed = true;
+                } else {
+                    std::vector<int> merged = gpos;
+                    for (int t : g.objs)
+                        if (std::find(merged.begin(), merged.end(), pos_of[t]) == merged.end()) merged.push_back(pos_of[t]);
+                    if (int(merged.size()) > W && !grp.ops.empty()) {
+                        blocked |= q;
+                        keep.push_back(gi);
+                        continue;
+                    }
+                    gpos = merged;
+                }
+                HostOp op;
+                op.gate = gi;
+                DevOp& d = op.dev;
+                d.kind = kind;
+                d.slot = opk_is_param(kind) ? g.slot : -1;
+                d.tap = -1;
+                if (adjoint && opk_is_param(kind) && g.grad) {
+                    d.tap = int(pass->tap_slots.size());
+                    pass->tap_slots.push_back(tap_id_of_slot[g.slot]);
+                }
+                d.mat = -1;
+                if (!m.empty()) {
+                    d.mat = int(pool->size());
+                    for (auto& c : m) {
+                        pool->push_back(c.real());
+                        pool->push_back(c.imag());
+                    }
+                }
+                auto reg = [&](int t) {
+                    return int(std::find(gpos.begin(), gpos.end(), pos_of[t]) - gpos.begin());
+                };
+                d.q0 = uint8_t(reg(g.objs[0]));
+                d.q1 = uint8_t(g.objs.size() > 1 ? reg(g.objs[1]) : 0);
+                d.ctrl_global = g.ctrl & nonlocal;
+                d.ctrl_local = 0;
+                for (int c = 0; c < 64; c++)
+                    if (((g.ctrl & ~nonlocal) >> c) & 1) d.ctrl_local |= uint32_t(1) << pos_of[c];
+                grp.ops.push_back(op);
+            }
+            // pad the group to 4 distinct positions
+            int K = int(pass->local_qubits.size());
+            for (int p = 0; int(gpos.size()) < kGroupWidth && p < K; p++)
+                if (std::find(gpos.begin(), gpos.end(), p) == gpos.end()) gpos.push_back(p);
+            for (int i = 0; i < kGroupWidth; i++) grp.pos[i] = gpos[i];
+            pass->groups.push_back(std::move(grp));
+            todo = keep;
+        }
+    }
+};
+
+}  // namespace
+
+Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const SchedOptions& opts) {
+    Plan plan;
+    plan.n_eff = n_eff;
+    plan.adjoint = adjoint;
+    plan.opts = opts;
+    if (n_eff < kMinTileQubits) throw std::invalid_argument("state too small for the tile engine");
+    const uint64_t all = (n_eff >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n_eff) - 1);
+    int K = std::min(std::max(opts.tile_qubits, kMinTileQubits), std::min(n_eff, kMaxTileQubits));
+    int C = std::min(std::max(opts.low_run, 0), K - kGroupWidth);
+    const bool whole = (n_eff <= K);
+    if (whole) C = K;
+    const int W = std::min(std::max(opts.group_width, 1), kGroupWidth);
+    plan.K = K;
+    plan.C = C;
+
+    std::vector<int> remaining;
+    for (size_t i = 0; i < gates.size(); i++) {
+        const CGate& g = gates[i];
+        if (g.kind == MQA_GATE_I) continue;
+        if (g.objs.empty() || int(g.objs.size()) > MQA_MAX_OBJ) throw std::invalid_argument("bad target count");
+        if ((g.qubits() & ~all) != 0) throw std::invalid_argument("gate acts on a qubit outside the state");
+        if (g.targets() & g.ctrl) throw std::invalid_argument("a qubit cannot be both target and control");
+        if (popc(g.targets()) != int(g.objs.size())) throw std::invalid_argument("repeated target qubit");
+        remaining.push_back(int(i));
+    }
+    if (adjoint) std::reverse(remaining.begin(), remaining.end());
+
+    PassBuilder pb{gates, adjoint, W, {}};
+    int n_slots = 0;
+    for (auto& g : gates) n_slots = std::max(n_slots, g.slot + 1);
+    pb.tap_id_of_slot.assign(n_slots, -1);
+    for (auto& g : gates)
+        if (g.slot >= 0 && g.grad) pb.tap_id_of_slot[g.slot] = plan.n_tap_ids++;
+    const uint64_t lowmask = (C >= 64) ? all : ((uint64_t(1) << C) - 1);
+    while (!remaining.empty()) {
+        uint64_t L = whole ? all : lowmask;
+        uint64_t blocked = 0;
+        std::vector<int> chosen, rest;
+        int taps = 0;
+        for (int gi : remaining) {
+            const CGate& g = gates[gi];
+            uint64_t q = g.qubits();
+            bool needs_tap = adjoint && g.grad && g.slot >= 0;
+            bool full = (opts.max_ops_per_pass > 0 && int(chosen.size()) >= opts.max_ops_per_pass) ||
+                        (needs_tap && taps >= kMaxTapsPerPass);
+            if (full || (q & blocked)) {
+                blocked |= q;
+                rest.push_back(gi);
+                continue;
+            }
+            uint64_t nl = L | g.targets();
+            if (popc(nl) <= K) {
+                L = nl;
+                chosen.push_back(gi);
+                taps += needs_tap ? 1 : 0;
+            } else {
+                blocked |= q;
+                rest.push_back(gi);
+            }
+        }
+        if (chosen.empty()) throw std::runtime_error("scheduler made no progress");
+        for (int q = 0; popc(L) < K && q < n_eff; q++) L |= uint64_t(1) << q;
+        HostPass pass;
+        std::vector<int> pos_of(64, -1);
+        for (int q = 0; q < n_eff; q++)
+            if ((L >> q) & 1) {
+                pos_of[q] = int(pass.local_qubits.size());
+                pass.local_qubits.push_back(q);
+            }
+        pb.groups(&pass, chosen, pos_of, all & ~L, &plan.pool);
+        plan.passes.push_back(std::move(pass));
+        remaining = rest;
+    }
+
+    // flatten
+    for (auto& pass : plan.passes) {
+        DevPass dp{};
+        uint64_t L = 0;
+        for (int q : pass.local_qubits) L |= uint64_t(1) << q;
+        dp.nonlocal_mask = all & ~L;
+        dp.local_hi_mask = L & ~lowmask;
+        dp.low_run = C;
+        dp.group_begin = int(plan.dev_groups.size());
+        dp.n_groups = int(pass.groups.size());
+        dp.tap_begin = int(plan.dev_tap_slots.size());
+        dp.n_taps = int(pass.tap_slots.size());
+        plan.max_taps = std::max(plan.max_taps, dp.n_taps);
+        for (int s : pass.tap_slots) plan.dev_tap_slots.push_back(s);
+        for (auto& grp : pass.groups) {
+            DevGroup dg{};
+            int sp[4];
+            for (int i = 0; i < 4; i++) {
+                dg.pos[i] = uint8_t(grp.pos[i]);
+                sp[i] = grp.pos[i];
+            }
+            std::sort(sp, sp + 4);
+            for (int i = 0; i < 4; i++) dg.spos[i] = uint8_t(sp[i]);
+            dg.op_begin = int(plan.dev_ops.size());
+            dg.n_ops = int(grp.ops.size());
+            for (auto& op : grp.ops) plan.dev_ops.push_back(op.dev);
+            plan.dev_groups.push_back(dg);
+        }
+        plan.dev_passes.push_back(dp);
+    }
+    if (plan.pool.empty()) plan.pool.push_back(0.0);
+    if (plan.dev_ops.empty()) plan.dev_ops.push_back(DevOp{});
+    if (plan.dev_groups.empty()) plan.dev_groups.push_back(DevGroup{});
+    if (plan.dev_tap_slots.empty()) plan.dev_tap_slots.push_back(0);
+    return plan;
+}
+
+}  // namespace mqa

@@ -1,0 +1,77 @@
+// schedule.h - host side: compiled circuits and the tile-pass scheduler (no HIP calls).
+#pragma once
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "program.h"
+
+namespace mqa {
+
+using cplx = std::complex<double>;
+
+// One validated gate of a circuit.
+struct CGate {
+    int kind = 0;                              // MQA_GATE_*
+    std::vector<int> objs;                     // targets, obj[i] <=> matrix bit i
+    uint64_t ctrl = 0;                         // control mask
+    int slot = -1;                             // angle slot for parameterised kinds
+    bool grad = false;                         // emits a derivative tap in the back-sweep
+    double const_val = 0;                      // angle = const + sum coeff * param
+    std::vector<std::pair<int, double>> terms;  // (param index, coefficient)
+    std::vector<cplx> mat;                     // MATRIX kind, row-major
+    uint64_t qubits() const;                   // targets | controls
+    uint64_t targets() const;
+};
+
+struct SchedOptions {
+    int tile_qubits = 12;      // K: amplitudes per tile = 2^K
+    int low_run = 5;           // C: lowest qubits always local (contiguous runs of 2^C amplitudes)
+    int group_width = 4;       // max target positions per register group (1..4)
+    int max_ops_per_pass = 0;  // 0: unlimited; 1 gives the unfused one-gate-per-pass schedule
+    bool operator==(const SchedOptions& o) const {
+        return tile_qubits == o.tile_qubits && low_run == o.low_run && group_width == o.group_width &&
+               max_ops_per_pass == o.max_ops_per_pass;
+    }
+};
+
+struct HostOp {
+    int gate = -1;  // index into the circuit
+    DevOp dev{};
+};
+struct HostGroup {
+    int pos[4] = {0, 1, 2, 3};
+    std::vector<HostOp> ops;
+};
+struct HostPass {
+    std::vector<int> local_qubits;  // sorted ascending, size K
+    std::vector<HostGroup> groups;
+    std::vector<int> tap_slots;     // tap index inside the pass -> global tap id
+};
+
+// A complete schedule, flattened into the device arrays.
+struct Plan {
+    int n_eff = 0;   // qubits of the (padded) state the plan runs on
+    int K = 0;       // tile qubits
+    int C = 0;       // contiguous low run
+    bool adjoint = false;
+    SchedOptions opts;
+    std::vector<HostPass> passes;
+    std::vector<DevPass> dev_passes;
+    std::vector<DevGroup> dev_groups;
+    std::vector<DevOp> dev_ops;
+    std::vector<int32_t> dev_tap_slots;
+    std::vector<double> pool;  // matrix pool (interleaved complex)
+    int max_taps = 0;
+    int n_tap_ids = 0;  // gates with a derivative tap; tap ids follow circuit order
+};
+
+// Build the schedule of `gates` (adjoint: the reverse order, each op applying G^dagger).
+// n_eff: qubits of the state.  Throws std::invalid_argument on unsupported gates.
+Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const SchedOptions& opts);
+
+// Matrix of a fixed (non-parameterised) gate kind; 2^m x 2^m row-major.
+std::vector<cplx> fixed_matrix(const CGate& g);
+
+}  // namespace mqa

@@ -1,0 +1,121 @@
+"""Lowering of a Circuit to the gate IR of include/mqamd.h.
+
+A circuit is compiled against an ordered parameter-name list (encoder names first, then
+ansatz names, gradient.h:149-158): every parameterised gate gets ``angle = const +
+sum coeff * params[index]`` terms, fixed gates their kind or a dense matrix.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .core import gates as G
+from .core.parameterresolver import ParameterResolver
+
+
+def lower(circ, names):
+    """Return (gates, term_param, term_coeff, mats, requires_grad_per_param) numpy arrays."""
+    index = {n: i for i, n in enumerate(names)}
+    gates = np.zeros(len(circ), dtype=_native.GATE_DTYPE)
+    t_param, t_coeff, mats = [], [], []
+    no_grad = set()
+    for i, g in enumerate(circ):
+        rec = gates[i]
+        rec["n_obj"] = len(g.obj_qubits)
+        for k, q in enumerate(g.obj_qubits):
+            rec["obj"][k] = q
+        rec["ctrl_mask"] = sum(1 << c for c in g.ctrl_qubits)
+        if isinstance(g, G.ParameterGate):
+            rec["kind"] = _native.GATE_KIND[g.ir_kind]
+            pr = g.coeff
+            rec["const_val"] = float(np.real(pr.const))
+            rec["term_begin"] = len(t_param)
+            for name, c in pr.items():
+                if name not in index:
+                    raise ValueError(f"Parameter {name} is not in the parameter list")
+                t_param.append(index[name])
+                t_coeff.append(float(np.real(c)))
+            rec["n_term"] = len(pr)
+            no_grad.update(pr.no_grad_parameters)
+            rec["requires_grad"] = 1 if pr.requires_grad_parameters else 0
+        elif isinstance(g, G.NoneParameterGate):
+            kind = g.ir_kind
+            rec["kind"] = _native.GATE_KIND[kind]
+            if kind == "MATRIX":
+                m = g.matrix()
+                rec["mat_begin"] = len(mats)
+                mats.extend(np.asarray(m, dtype=np.complex128).ravel())
+        else:
+            raise ValueError(f"Gate {g.name} is not supported by the HIP backend")
+    mats_arr = np.zeros(2 * max(len(mats), 1), dtype=np.float64)
+    if mats:
+        m = np.asarray(mats, dtype=np.complex128)
+        mats_arr[0::2] = m.real
+        mats_arr[1::2] = m.imag
+    rg = np.array([0 if n in no_grad else 1 for n in names], dtype=np.uint8)
+    return (gates, np.asarray(t_param, dtype=np.int32), np.asarray(t_coeff, dtype=np.float64), mats_arr,
+            len(mats), rg)
+
+
+class NativeCircuit:
+    """C ABI circuit handle for (circuit, parameter order)."""
+
+    def __init__(self, circ, names):
+        self.names = list(names)
+        self.arrays = lower(circ, self.names)
+        gates, tp, tc, mats, n_mats, rg = self.arrays
+        h = ctypes.c_void_p()
+        rgp = _native.ptr(rg) if len(rg) else None
+        _native.check(_native.lib().mqa_circuit_create(
+            _native.ptr(gates), len(gates), _native.ptr(tp) if len(tp) else None,
+            _native.ptr(tc) if len(tc) else None, len(tp), _native.ptr(mats), n_mats, len(self.names), rgp,
+            ctypes.byref(h)))
+        self.handle = h
+
+    def schedule(self, n_qubits, dtype_code, adjoint=False):
+        """Tile-pass schedule: list of (local qubit mask, [gate indices])."""
+        n = ctypes.c_int64(0)
+        _native.check(_native.lib().mqa_circuit_schedule(self.handle, n_qubits, dtype_code, int(adjoint), None, 0,
+                                                          ctypes.byref(n)))
+        buf = np.zeros(n.value, dtype=np.int64)
+        _native.check(_native.lib().mqa_circuit_schedule(self.handle, n_qubits, dtype_code, int(adjoint),
+                                                          _native.ptr(buf), n.value, ctypes.byref(n)))
+        out, pos = [], 1
+        for _ in range(int(buf[0])):
+            mask, cnt = int(buf[pos]), int(buf[pos + 1])
+            out.append((mask, [int(x) for x in buf[pos + 2: pos + 2 + cnt]]))
+            pos += 2 + cnt
+        return out
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _native._lib is not None:
+            _native._lib.mqa_circuit_destroy(h)
+            self.handle = None
+
+
+def param_names(circ):
+    """(encoder names, ansatz names) in order of appearance."""
+    return list(circ.encoder_params_name), list(circ.ansatz_params_name)
+
+
+def resolve(circ, pr):
+    """Parameter vector of a circuit for ``pr`` (dict, ParameterResolver, array or None)."""
+    names = list(circ.params_name)
+    if not names:
+        return names, np.zeros(0, dtype=np.float64)
+    if pr is None:
+        raise ValueError("Parameterized circuit needs a parameter resolver.")
+    if isinstance(pr, ParameterResolver):
+        vals = dict(pr.items())
+    elif isinstance(pr, dict):
+        vals = pr
+    else:
+        arr = np.asarray(pr, dtype=np.float64).ravel()
+        if arr.size != len(names):
+            raise ValueError(f"Need {len(names)} parameters, but get {arr.size}.")
+        return names, arr
+    missing = [n for n in names if n not in vals]
+    if missing:
+        raise ValueError(f"Parameters {missing} are not given.")
+    return names, np.array([float(np.real(vals[n])) for n in names], dtype=np.float64)

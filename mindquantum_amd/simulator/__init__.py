@@ -1,0 +1,6 @@
+"""Simulator API (mindquantum.simulator) backed by the MI355X state-vector engine."""
+from .backend import MQVectorAMD, inner_product
+from .simulator import SUPPORTED_SIMULATOR, Simulator
+from .utils import GradOpsWrapper
+
+__all__ = ["Simulator", "MQVectorAMD", "inner_product", "GradOpsWrapper", "SUPPORTED_SIMULATOR"]

@@ -1,0 +1,288 @@
+"""The MI355X state-vector backend: Python host side of the C ABI.
+
+Mirrors mindquantum/simulator/mqsim.py (MQSim): argument validation, the ``version``
+logic of get_expectation_with_grad (encoder / ansatz / both), and the output slicing
+``f = res[:, :, 0]``, ``g = res[:, :, 1:]``.  All numerical work runs in libmqamd.so; there
+is no CPU code path.
+"""
+import ctypes
+import numbers
+
+import numpy as np
+
+from .. import _native
+from .. import dtype as mqdtype
+from ..core.circuit import Circuit
+from ..core.gates import BasicGate
+from ..core.operators import Hamiltonian
+from ..ir import NativeCircuit, resolve
+from .utils import GradOpsWrapper, _thread_balance
+
+
+def _check_encoder(data, n):
+    """type_value_check.py:24-34"""
+    if not isinstance(data, np.ndarray):
+        raise ValueError(f"encoder parameters need numpy array, but get {type(data)}")
+    if data.ndim != 2:
+        raise ValueError("encoder data requires a two dimension numpy array")
+    if data.shape[1] != n:
+        raise ValueError(f"encoder parameters size do not match with encoder parameters name, need {n} but get "
+                         f"{data.shape[1]}.")
+
+
+def _check_ansatz(data, n):
+    """type_value_check.py:37-48"""
+    if not isinstance(data, np.ndarray):
+        raise ValueError(f"ansatz parameters need numpy array, but get {type(data)}")
+    if data.ndim != 1:
+        raise ValueError("ansatz data requires a one dimension numpy array")
+    if data.shape[0] != n:
+        raise ValueError(f"ansatz parameters size do not match with ansatz parameters name, need {n} but get "
+                         f"{data.shape[0]}")
+
+
+class MQVectorAMD:
+    """State-vector simulator on one MI355X (gfx950) GPU."""
+
+    name = "mqvector_amd"
+
+    def __init__(self, name, n_qubits, seed=None, dtype=None):
+        if not isinstance(n_qubits, (int, np.integer)) or n_qubits < 1:
+            raise ValueError(f"n_qubits should be a positive int, but get {n_qubits}")
+        self.name = name
+        self.n_qubits = int(n_qubits)
+        self.seed = int(np.random.randint(1, 2**23)) if seed is None else int(seed)
+        self.dtype = mqdtype.to_mq_type(dtype)
+        self._code = mqdtype.native_code(self.dtype)
+        self._circ_cache = {}
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().mqa_sim_create(self.n_qubits, self._code, self.seed & 0xFFFFFFFF, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def _wrap(cls, name, handle, n_qubits, seed, dtype):
+        obj = cls.__new__(cls)
+        obj.name = name
+        obj.n_qubits = n_qubits
+        obj.seed = seed
+        obj.dtype = mqdtype.to_mq_type(dtype)
+        obj._code = mqdtype.native_code(obj.dtype)
+        obj._circ_cache = {}
+        obj._h = handle
+        return obj
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _native._lib is not None:
+            _native._lib.mqa_sim_destroy(h)
+            self._h = None
+
+    def __getstate__(self):  # mqsim.py:84-98: the state itself is not pickled
+        return {"name": self.name, "n_qubits": self.n_qubits, "seed": self.seed, "dtype": self.dtype}
+
+    def __setstate__(self, st):
+        self.__init__(st["name"], st["n_qubits"], st["seed"], st["dtype"])
+
+    # ---- circuits
+    def _native_circuit(self, circ, names):
+        key = (id(circ), len(circ), tuple(names))
+        hit = self._circ_cache.get(key)
+        if hit is not None and hit[0] is circ:
+            return hit[1]
+        nc = NativeCircuit(circ, names)
+        if len(self._circ_cache) > 64:
+            self._circ_cache.clear()
+        self._circ_cache[key] = (circ, nc)
+        return nc
+
+    def _check_circ(self, circ):
+        if not isinstance(circ, Circuit):
+            raise TypeError(f"circuit requires a Circuit, but get {type(circ)}")
+        if circ.n_qubits > self.n_qubits:
+            raise ValueError(f"Circuit has {circ.n_qubits} qubits, which is more than simulator qubits "
+                             f"{self.n_qubits}.")
+
+    def _check_ham(self, ham, i=0):
+        if not isinstance(ham, Hamiltonian):
+            raise TypeError(f"hams's element requires a Hamiltonian, but get {type(ham)}")
+        if not mqdtype.is_same_precision(self.dtype, ham.dtype):
+            raise TypeError(
+                f"Data type of {self.name} simulator is {mqdtype.precision_str(self.dtype)} ({self.dtype.__name__}), "
+                f"but {i}th hamiltonian is {mqdtype.precision_str(ham.dtype)} ({ham.dtype.__name__}).")
+        if ham.n_qubits > self.n_qubits:
+            raise ValueError(f"Hamiltonian qubits is {ham.n_qubits}, not match with simulator qubits number "
+                             f"{self.n_qubits}")
+
+    def apply_gate(self, gate, pr=None, diff=False):
+        if not isinstance(gate, BasicGate):
+            raise TypeError(f"gate requires a quantum gate, but get {type(gate)}")
+        if diff:
+            raise ValueError("the differential form of a gate is not applied on states by this backend")
+        return self.apply_circuit(Circuit([gate]), pr)
+
+    def apply_circuit(self, circuit, pr=None):
+        self._check_circ(circuit)
+        if len(circuit) == 0:
+            return None
+        names, vals = resolve(circuit, pr)
+        nc = self._native_circuit(circuit, names)
+        _native.check(_native.lib().mqa_sim_apply_circuit(self._h, nc.handle, _native.ptr(vals), len(vals)))
+        return None
+
+    def apply_hamiltonian(self, hamiltonian):
+        self._check_ham(hamiltonian)
+        _native.check(_native.lib().mqa_sim_apply_hamiltonian(self._h, hamiltonian.native()))
+
+    # ---- state access
+    def get_qs(self, ket=False):
+        out = np.empty(1 << self.n_qubits, dtype=self.dtype)
+        _native.check(_native.lib().mqa_sim_get_qs(self._h, _native.ptr(out), out.size))
+        if ket:
+            from ..utils import ket_string
+
+            return "\n".join(ket_string(out))
+        return out
+
+    def set_qs(self, quantum_state):
+        qs = np.asarray(quantum_state)
+        if qs.ndim != 1 or qs.size != (1 << self.n_qubits):
+            raise ValueError(f"Quantum state should be a 1-D array of length {1 << self.n_qubits}")
+        norm = np.sqrt(np.sum(np.abs(qs) ** 2))
+        if norm < 1e-12:
+            raise ValueError("Norm of quantum state can not be zero.")
+        qs = np.ascontiguousarray(qs / norm, dtype=self.dtype)  # mqsim.py:537-544
+        _native.check(_native.lib().mqa_sim_set_qs(self._h, _native.ptr(qs), qs.size))
+
+    def reset(self):
+        _native.check(_native.lib().mqa_sim_reset(self._h))
+
+    def copy(self):
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().mqa_sim_copy(self._h, ctypes.byref(h)))
+        return MQVectorAMD._wrap(self.name, h, self.n_qubits, self.seed, self.dtype)
+
+    def astype(self, dtype, seed=None):
+        code = mqdtype.native_code(dtype)
+        h = ctypes.c_void_p()
+        _native.check(_native.lib().mqa_sim_astype(self._h, code, ctypes.byref(h)))
+        return MQVectorAMD._wrap(self.name, h, self.n_qubits, self.seed if seed is None else seed, dtype)
+
+    # ---- expectations
+    def get_expectation(self, hamiltonian, circ_right=None, circ_left=None, simulator_left=None, pr=None):
+        self._check_ham(hamiltonian)
+        if circ_right is None:
+            circ_right = Circuit()
+        self._check_circ(circ_right)
+        if circ_left is not None:
+            self._check_circ(circ_left)
+        if simulator_left is not None:
+            if not isinstance(simulator_left, MQVectorAMD):
+                raise TypeError("simulator_left should be a simulator of the same backend")
+            if simulator_left.n_qubits != self.n_qubits or simulator_left.dtype != self.dtype:
+                raise ValueError("simulator_left should have the same n_qubits and dtype as this simulator")
+        names = list(circ_right.params_name)
+        if circ_left is not None:
+            names += [n for n in circ_left.params_name if n not in names]
+        vals = np.zeros(len(names), dtype=np.float64)
+        if names:
+            if pr is None:
+                raise ValueError("Parameterized circuit needs a parameter resolver.")
+            mapping = dict(pr.items()) if hasattr(pr, "items") else dict(zip(names, np.asarray(pr).ravel()))
+            vals = np.array([float(np.real(mapping[n])) for n in names], dtype=np.float64)
+        cr = self._native_circuit(circ_right, names).handle if len(circ_right) else None
+        cl = self._native_circuit(circ_left, names).handle if circ_left is not None and len(circ_left) else None
+        left = simulator_left._h if simulator_left is not None else None
+        out = np.zeros(2, dtype=np.float64)
+        if circ_left is not None and cl is None:
+            # an empty left circuit still makes the bra the (left) initial state
+            cl_empty = self._native_circuit(Circuit(), names)
+            cl = cl_empty.handle
+        _native.check(_native.lib().mqa_sim_expectation_lr(self._h, left, hamiltonian.native(), cr, cl,
+                                                           _native.ptr(vals), len(vals), _native.ptr(out)))
+        return complex(out[0], out[1])
+
+    def get_expectation_with_grad(self, hams, circ_right, circ_left=None, simulator_left=None,
+                                  parallel_worker=None, pr_shift=False):
+        if isinstance(hams, Hamiltonian):
+            hams = [hams]
+        elif not isinstance(hams, list):
+            raise TypeError(f"hams requires a Hamiltonian or a list of Hamiltonian, but get {type(hams)}")
+        for i, h in enumerate(hams):
+            self._check_ham(h, i)
+        self._check_circ(circ_right)
+        if circ_left is not None or simulator_left is not None:
+            raise NotImplementedError("non-hermitian (circ_left / simulator_left) gradients are not on this path yet")
+        if pr_shift:
+            raise NotImplementedError("parameter-shift gradients are not on this path; the adjoint is used")
+        if circ_right.has_measure_gate:
+            raise ValueError("circuit for variational algorithm cannot have measure gate")
+        if parallel_worker is not None and not isinstance(parallel_worker, (int, np.integer)):
+            raise TypeError(f"parallel_worker requires a int, but get {type(parallel_worker)}")
+        ansatz_params_name = list(circ_right.all_ansatz.keys())
+        encoder_params_name = list(circ_right.all_encoder.keys())
+        if set(ansatz_params_name) & set(encoder_params_name):
+            raise RuntimeError("Parameter cannot be both encoder and ansatz parameter.")
+        version = "both"
+        if not ansatz_params_name:
+            version = "encoder"
+        if not encoder_params_name:
+            version = "ansatz"
+        names = encoder_params_name + ansatz_params_name
+        nc = self._native_circuit(circ_right, names)
+        ham_handles = (ctypes.c_void_p * len(hams))(*[h.native().value for h in hams])
+        n_enc, n_ans = len(encoder_params_name), len(ansatz_params_name)
+
+        def grad_ops(*inputs_):
+            inputs = [np.array(x) if isinstance(x, list) else x for x in inputs_]
+            if version == "both" and len(inputs) != 2:
+                raise ValueError("Need two inputs!")
+            if version in ("encoder", "ansatz") and len(inputs) != 1:
+                raise ValueError("Need one input!")
+            if version in ("both", "encoder"):
+                if inputs[0].ndim == 1:
+                    inputs[0] = np.array([inputs[0]])
+                _check_encoder(inputs[0], n_enc)
+                enc = np.ascontiguousarray(inputs[0], dtype=np.float64)
+                ans = (np.ascontiguousarray(inputs[1], dtype=np.float64) if version == "both"
+                       else np.zeros(0, dtype=np.float64))
+                if version == "both":
+                    _check_ansatz(inputs[1], n_ans)
+            else:
+                _check_ansatz(inputs[0], n_ans)
+                enc = np.zeros((1, 0), dtype=np.float64)
+                ans = np.ascontiguousarray(inputs[0], dtype=np.float64)
+            batch = enc.shape[0]
+            _thread_balance(batch, len(hams), parallel_worker)  # validates like the reference
+            f = np.zeros((batch, len(hams), 2), dtype=np.float64)
+            g = np.zeros((batch, len(hams), n_enc + n_ans), dtype=np.float64)
+            _native.check(_native.lib().mqa_sim_grad_batch(
+                self._h, ham_handles, len(hams), nc.handle, _native.ptr(enc) if n_enc else None, batch, n_enc,
+                _native.ptr(ans) if n_ans else None, n_ans, _native.ptr(f), _native.ptr(g)))
+            fc = (f[..., 0] + 1j * f[..., 1]).astype(self.dtype)
+            gc = g.astype(self.dtype)
+            if version == "both":
+                return fc, gc[:, :, :n_enc], gc[:, :, n_enc:]
+            return fc, gc
+
+        wrapper = GradOpsWrapper(grad_ops, hams, circ_right, circ_left, encoder_params_name, ansatz_params_name,
+                                 parallel_worker, self)
+        s = f"{self.n_qubits} qubit" + ("" if self.n_qubits == 1 else "s") + f" {self.name} VQA Operator"
+        wrapper.set_str(s)
+        return wrapper
+
+
+def inner_product(bra, ket):
+    """<bra|ket> of two simulators of the same backend (mq_blas.py / blas.h InnerProduct)."""
+    a = bra.backend if hasattr(bra, "backend") else bra
+    b = ket.backend if hasattr(ket, "backend") else ket
+    if a.n_qubits != b.n_qubits:
+        raise ValueError("two states must have the same qubit number")
+    if a.dtype != b.dtype:
+        raise TypeError("two states must have the same dtype")
+    out = np.zeros(2, dtype=np.float64)
+    _native.check(_native.lib().mqa_inner_product(a._h, b._h, _native.ptr(out)))
+    return complex(out[0], out[1])
+
+
+def is_number(x):
+    return isinstance(x, numbers.Number)

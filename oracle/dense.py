@@ -1,0 +1,134 @@
+"""TEST INFRASTRUCTURE ONLY: dense numpy simulation for small registers (n <= 10).
+
+Independent of both the HIP library and the C++ oracle: every gate is embedded into a full
+2^n x 2^n matrix from its closed-form numpy matrix (mindquantum_amd.core.gates, which follow
+mindquantum/core/gates/basicgate.py), Hamiltonians are built from Kronecker products of Pauli
+matrices, and gradients use the direct product formula
+
+    dE/dtheta_j = sum_k c_kj * 2 Re <psi| U^dag H U_{>k} dG_k U_{<k} |psi>
+
+(no adjoint sweep), so a sign, ordering or jacobian mistake in either simulator shows up here.
+"""
+import numpy as np
+
+_PAULI = {
+    "I": np.eye(2, dtype=np.complex128),
+    "X": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+
+def embed(m, objs, ctrls, n):
+    """Full matrix of a gate with target matrix ``m`` (bit b of its index <=> objs[b])."""
+    dim = 1 << n
+    full = np.zeros((dim, dim), dtype=np.complex128)
+    cmask = sum(1 << c for c in ctrls)
+    omask = sum(1 << q for q in objs)
+    k = len(objs)
+    for i in range(dim):
+        if (i & cmask) != cmask:
+            full[i, i] = 1
+            continue
+        base = i & ~omask
+        col = sum(((i >> q) & 1) << b for b, q in enumerate(objs))
+        for row in range(1 << k):
+            j = base | sum(((row >> b) & 1) << q for b, q in enumerate(objs))
+            full[j, i] = m[row, col]
+    return full
+
+
+def embed_diff(dm, objs, ctrls, n):
+    """Derivative of a controlled gate: zero outside the control subspace."""
+    dim = 1 << n
+    full = np.zeros((dim, dim), dtype=np.complex128)
+    cmask = sum(1 << c for c in ctrls)
+    omask = sum(1 << q for q in objs)
+    k = len(objs)
+    for i in range(dim):
+        if (i & cmask) != cmask:
+            continue
+        base = i & ~omask
+        col = sum(((i >> q) & 1) << b for b, q in enumerate(objs))
+        for row in range(1 << k):
+            j = base | sum(((row >> b) & 1) << q for b, q in enumerate(objs))
+            full[j, i] = dm[row, col]
+    return full
+
+
+def gate_matrix(g, values):
+    from mindquantum_amd.core.gates import ParameterGate
+
+    if isinstance(g, ParameterGate):
+        return g.matrix(values)
+    return g.matrix()
+
+
+def unitary(circ, values, n):
+    u = np.eye(1 << n, dtype=np.complex128)
+    for g in circ:
+        u = embed(gate_matrix(g, values), g.obj_qubits, g.ctrl_qubits, n) @ u
+    return u
+
+
+def state(circ, values, n, init=None):
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    if init is None:
+        psi[0] = 1
+    else:
+        psi[:] = init
+    return unitary(circ, values, n) @ psi
+
+
+def hamiltonian_matrix(qubit_operator, n):
+    """Sum of coeff * (kron of Pauli matrices), qubit 0 as the least significant factor."""
+    dim = 1 << n
+    h = np.zeros((dim, dim), dtype=np.complex128)
+    for key, coeff in qubit_operator.terms.items():
+        ops = dict((q, p) for q, p in key)
+        m = np.ones((1, 1), dtype=np.complex128)
+        for q in range(n - 1, -1, -1):
+            m = np.kron(m, _PAULI[ops.get(q, "I")])
+        h += coeff * m
+    return h
+
+
+def expectation(circ, values, n, qubit_operator, init=None):
+    psi = state(circ, values, n, init)
+    return np.vdot(psi, hamiltonian_matrix(qubit_operator, n) @ psi)
+
+
+def gradient(circ, values, names, n, qubit_operator, init=None):
+    """(f, dE/dparam for every name in ``names``) by the direct product formula."""
+    from mindquantum_amd.core.gates import ParameterGate
+
+    dim = 1 << n
+    psi0 = np.zeros(dim, dtype=np.complex128)
+    if init is None:
+        psi0[0] = 1
+    else:
+        psi0[:] = init
+    hm = hamiltonian_matrix(qubit_operator, n)
+    mats = [embed(gate_matrix(g, values), g.obj_qubits, g.ctrl_qubits, n) for g in circ]
+    # prefix states and suffix unitaries
+    prefix = [psi0]
+    for m in mats:
+        prefix.append(m @ prefix[-1])
+    psi = prefix[-1]
+    f = np.vdot(psi, hm @ psi)
+    suffix = [np.eye(dim, dtype=np.complex128)]
+    for m in reversed(mats):
+        suffix.append(suffix[-1] @ m)
+    suffix = suffix[::-1]  # suffix[k] = U_N ... U_k (product of gates k..N-1), suffix[N] = I
+    bra = (hm @ psi).conj()
+    grad = np.zeros(len(names))
+    idx = {nm: i for i, nm in enumerate(names)}
+    for k, g in enumerate(circ):
+        if not isinstance(g, ParameterGate) or g.coeff.is_const():
+            continue
+        dm = embed_diff(g.diff_matrix(values), g.obj_qubits, g.ctrl_qubits, n)
+        val = bra @ (suffix[k + 1] @ (dm @ prefix[k]))
+        for name, c in g.coeff.items():
+            if name in idx and name in g.coeff.requires_grad_parameters:
+                grad[idx[name]] += 2 * np.real(val) * np.real(c)
+    return f, grad
