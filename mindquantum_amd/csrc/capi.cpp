@@ -233,8 +233,9 @@ struct mqa_circuit {
             mqa::upload(dp->groups, dp->plan.dev_groups, s);
             mqa::upload(dp->ops, dp->plan.dev_ops, s);
             mqa::upload(dp->tap_ids, dp->plan.dev_tap_slots, s);
+            std::vector<float> pf;  // kept alive until the copy has completed
             if (dtype == MQA_C64) {
-                std::vector<float> pf(dp->plan.pool.begin(), dp->plan.pool.end());
+                pf.assign(dp->plan.pool.begin(), dp->plan.pool.end());
                 mqa::upload(dp->pool, pf, s);
             } else {
                 mqa::upload(dp->pool, dp->plan.pool, s);

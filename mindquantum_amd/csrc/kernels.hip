@@ -461,12 +461,12 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
     const DevPass pass = args.passes[args.pass_index];
     const int C = pass.low_run;
     const int H = K - C;
-    const uint32_t tile_amps = 1u << K;
 
-    V* t_psi = reinterpret_cast<V*>(smem);
-    V* t_lam = t_psi + (ADJ ? tile_amps : 0);
-    uint64_t* offtab = reinterpret_cast<uint64_t*>(t_lam + (ADJ ? tile_amps : 0));
-    double* lds_taps = reinterpret_cast<double*>(offtab + (1u << H));
+    const LdsLayout lay = lds_layout(K, C, ADJ, int(sizeof(V)), 0);
+    V* t_psi = reinterpret_cast<V*>(smem + lay.psi);
+    V* t_lam = reinterpret_cast<V*>(smem + lay.lam);  // used by the adjoint pass only
+    uint64_t* offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
+    double* lds_taps = reinterpret_cast<double*>(smem + lay.taps);
 
     const uint64_t sample = blockIdx.y;
     const uint64_t tile = blockIdx.x;
@@ -743,10 +743,7 @@ __global__ void __launch_bounds__(256) k_jacobian(const double* __restrict__ tap
 static constexpr size_t kLdsLimit = 160 * 1024;
 
 size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps) {
-    const size_t amp = (dtype == 0) ? 8 : 16;
-    const int nt = 1 << (K - kGroupWidth);
-    const int n_waves = (nt + 63) / 64;
-    return (size_t(1) << K) * amp * (adj ? 2 : 1) + (size_t(1) << (K - C)) * 8 + size_t(max_taps) * n_waves * 16;
+    return lds_layout(K, C, adj, dtype == 0 ? 8 : 16, max_taps).total;
 }
 
 template <typename T, bool ADJ, bool FULL>

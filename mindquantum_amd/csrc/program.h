@@ -46,6 +46,27 @@ inline bool opk_is_two(int k) {
     return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ;
 }
 
+// Byte layout of a tile pass's dynamic LDS: [psi tile][lambda tile (adjoint)][offset table]
+// [tap partials].  Shared by the kernel and the host launcher so the two cannot disagree.
+struct LdsLayout {
+    uint32_t psi, lam, offtab, taps, total;
+};
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline LdsLayout lds_layout(int K, int C, bool adj, int amp_bytes, int max_taps) {
+    LdsLayout l;
+    const uint32_t tile = (uint32_t(1) << K) * uint32_t(amp_bytes);
+    const uint32_t n_threads = uint32_t(1) << (K - kGroupWidth);
+    const uint32_t n_waves = (n_threads + 63) / 64;
+    l.psi = 0;
+    l.lam = tile;
+    l.offtab = tile * (adj ? 2u : 1u);
+    l.taps = (l.offtab + (uint32_t(1) << (K - C)) * 8u + 15u) & ~15u;
+    l.total = l.taps + uint32_t(max_taps) * n_waves * 16u;
+    return l;
+}
+
 struct DevOp {             // 32 bytes
     int32_t kind;          // OpKind
     int32_t slot;          // angle slot (param kinds), -1 otherwise
