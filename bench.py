@@ -152,12 +152,14 @@ def cpu_baseline(n, layers, seed, budget_s):
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
     results = {}
     # (ii) batch_threads = 1, OpenMP over all cores inside each sample
+    log(f"cpu baseline: 1 sample, OpenMP over {cores} threads")
     cpu.set_num_threads(cores)
     t0 = time.perf_counter()
     cpu.grad_batch(n, np.complex64, arrays, terms, enc[:1], ans, batch_threads=1)
     t1 = time.perf_counter() - t0
     results["omp_per_sample"] = 1 / t1
     # (i) batch_threads = cores, one OpenMP thread per sample
+    log(f"cpu baseline: {cores} samples, one thread each")
     cpu.set_num_threads(1)
     m = max(cores, 1)
     t0 = time.perf_counter()
