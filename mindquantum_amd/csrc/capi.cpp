@@ -90,6 +90,8 @@ SchedOptions sched_options(int dtype, bool adjoint) {
     o.low_run = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
     o.group_width = int(g_opt.group_width);
     o.max_ops_per_pass = int(g_opt.max_ops);
+    o.taps_per_group = dtype == MQA_C64 ? 8 : 4;  // register budget of the adjoint pass
+    o.amp_bytes = dtype == MQA_C64 ? 8 : 16;
     return o;
 }
 
@@ -213,21 +215,23 @@ struct mqa_circuit {
     int n_tap_ids = 0;
 
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
     std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev_slots;  // per device: 6 buffers
 
     mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev) {
         auto o = mqa::sched_options(dtype, adjoint);
         int dev = upload_dev ? mqa::current_device() : -1;
         auto key = std::make_tuple(n_eff, dtype, int(adjoint), o.tile_qubits, o.low_run, o.group_width,
-                                   o.max_ops_per_pass, dev);
+                                   o.max_ops_per_pass, o.taps_per_group, dev);
         std::lock_guard<std::mutex> lk(mu);
         auto it = plans.find(key);
         if (it != plans.end()) return it->second.get();
         auto dp = std::make_unique<mqa::DevPlan>();
         dp->plan = mqa::build_plan(gates, n_eff, adjoint, o);
+        // dense blocks or controls that stay per-amplitude predicates need the FULL instantiation
         for (auto& op : dp->plan.dev_ops)
-            if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4) dp->full = true;
+            if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0)
+                dp->full = true;
         if (upload_dev) {
             mqa::upload(dp->passes, dp->plan.dev_passes, s);
             mqa::upload(dp->groups, dp->plan.dev_groups, s);

@@ -54,11 +54,7 @@ __device__ __forceinline__ uint64_t pdep64(uint64_t src, uint64_t mask) {
 // groups whose positions include low qubits do not hit one bank with every lane.
 template <typename T>
 __device__ __forceinline__ uint32_t swz(uint32_t u) {
-    if constexpr (sizeof(T) == 4) {
-        return u ^ (((u >> 5) ^ (u >> 10)) & 31u);
-    } else {
-        return u ^ (((u >> 4) ^ (u >> 8) ^ (u >> 12)) & 15u);
-    }
+    return lds_swizzle(u, int(sizeof(T) * 2));
 }
 
 template <typename T>
@@ -66,6 +62,40 @@ __device__ __forceinline__ T wave_sum(T v, int width) {
     for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+
+// Wave-wide sum of a full 64-lane wave: DPP within rows of 16 (VALU, no LDS round trip),
+// then two cross-row exchanges.  Every lane ends with the total.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum64(T v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Per-thread derivative-tap accumulators of one register group.  The scheduler caps the taps
+// of a group at 8 (complex64) / 4 (complex128) so that the accumulators stay in registers.
+template <typename T>
+constexpr int kTapSlots = sizeof(T) == 4 ? 8 : 4;
+template <typename T>
+struct TapAcc {
+    T r[kTapSlots<T>];
+    T i[kTapSlots<T>];
+};
 
 // Register state of one thread inside a group: 16 amplitudes of psi (and of lambda).
 template <typename T>
@@ -118,48 +148,72 @@ __device__ __forceinline__ void mat2(const T* __restrict__ m, T& ar, T& ai, T& b
     ar = xr, ai = xi, br = yr, bi = yi;
 }
 
-// conj(x) * y accumulated
+// conj(x) * y accumulated as four FMAs (a dependent chain costs nothing extra: a wave64
+// VALU op occupies its SIMD for four cycles, which covers the FMA latency)
 template <typename T>
 __device__ __forceinline__ void cdot_acc(T xr, T xi, T yr, T yi, T& sr, T& si) {
-    sr += xr * yr + xi * yi;
-    si += xr * yi - xi * yr;
+    sr = fma(xr, yr, sr);
+    sr = fma(xi, yi, sr);
+    si = fma(xr, yi, si);
+    si = fma(-xi, yr, si);
 }
 
 // One-target op on a register set.  c, s: cos / sin of the half angle (RX, RY, RZ) or of the
-// angle (PS, GP); m: dense 2x2 (M1) in the matrix pool.
-template <typename T, int KIND, int Q>
+// angle (PS, GP); m: dense 2x2 (M1) in the matrix pool.  CTRL: some control qubits are local;
+// their predicate is evaluated per pair and applied with selects (no divergent branches).
+template <typename T, int KIND, int Q, bool CTRL>
 __device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32_t cl, T c, T s,
                                            const T* __restrict__ m) {
     for_pairs<Q>([&]<int V, int W>() {
-        if (cl && !ctrl_ok<V>(g, cl)) return;
-        T& ar = a.r[V];
-        T& ai = a.i[V];
-        T& br = a.r[W];
-        T& bi = a.i[W];
+        T ar = a.r[V], ai = a.i[V], br = a.r[W], bi = a.i[W];
+        T xr, xi, yr, yi;
         if constexpr (KIND == OPK_X) {
-            T tr = ar, ti = ai;
-            ar = br, ai = bi, br = tr, bi = ti;
+            xr = br, xi = bi, yr = ar, yi = ai;
         } else if constexpr (KIND == OPK_RX) {  // [[c, -is], [-is, c]]
-            T xr = c * ar + s * bi, xi = c * ai - s * br;
-            T yr = c * br + s * ai, yi = c * bi - s * ar;
-            ar = xr, ai = xi, br = yr, bi = yi;
+            xr = c * ar + s * bi, xi = c * ai - s * br;
+            yr = c * br + s * ai, yi = c * bi - s * ar;
         } else if constexpr (KIND == OPK_RY) {  // [[c, -s], [s, c]]
-            T xr = c * ar - s * br, xi = c * ai - s * bi;
-            T yr = s * ar + c * br, yi = s * ai + c * bi;
-            ar = xr, ai = xi, br = yr, bi = yi;
+            xr = c * ar - s * br, xi = c * ai - s * bi;
+            yr = s * ar + c * br, yi = s * ai + c * bi;
         } else if constexpr (KIND == OPK_RZ) {  // diag(c - is, c + is)
-            T xr = c * ar + s * ai, xi = c * ai - s * ar;
-            T yr = c * br - s * bi, yi = c * bi + s * br;
-            ar = xr, ai = xi, br = yr, bi = yi;
+            xr = c * ar + s * ai, xi = c * ai - s * ar;
+            yr = c * br - s * bi, yi = c * bi + s * br;
         } else if constexpr (KIND == OPK_PS) {  // diag(1, c + is)
-            T yr = c * br - s * bi, yi = c * bi + s * br;
-            br = yr, bi = yi;
+            xr = ar, xi = ai;
+            yr = c * br - s * bi, yi = c * bi + s * br;
         } else if constexpr (KIND == OPK_GP) {  // (c - is) I
-            T xr = c * ar + s * ai, xi = c * ai - s * ar;
-            T yr = c * br + s * bi, yi = c * bi - s * br;
-            ar = xr, ai = xi, br = yr, bi = yi;
+            xr = c * ar + s * ai, xi = c * ai - s * ar;
+            yr = c * br + s * bi, yi = c * bi - s * br;
         } else {  // dense 2x2
-            mat2(m, ar, ai, br, bi);
+            xr = ar, xi = ai, yr = br, yi = bi;
+            mat2(m, xr, xi, yr, yi);
+        }
+        if constexpr (CTRL) {
+            const bool ok = ctrl_ok<V>(g, cl);
+            xr = ok ? xr : ar, xi = ok ? xi : ai, yr = ok ? yr : br, yi = ok ? yi : bi;
+        }
+        a.r[V] = xr, a.i[V] = xi, a.r[W] = yr, a.i[W] = yi;
+    });
+}
+
+// Controlled X / Z whose (first) control is register position QC of the group: the control
+// bit is known per register at compile time.  cl: further local controls (predicated).
+template <typename T, bool ISX, int QT, int QC, bool CTRL>
+__device__ __forceinline__ void controlled_xz(Regs<T>& a, const GroupCtx& g, uint32_t cl) {
+    for_pairs<QT>([&]<int V, int W>() {
+        if constexpr (((V >> QC) & 1) != 0) {
+            T ar = a.r[V], ai = a.i[V], br = a.r[W], bi = a.i[W];
+            T xr, xi, yr, yi;
+            if constexpr (ISX) {
+                xr = br, xi = bi, yr = ar, yi = ai;
+            } else {
+                xr = ar, xi = ai, yr = -br, yi = -bi;
+            }
+            if constexpr (CTRL) {
+                const bool ok = ctrl_ok<V>(g, cl);
+                xr = ok ? xr : ar, xi = ok ? xi : ai, yr = ok ? yr : br, yi = ok ? yi : bi;
+            }
+            a.r[V] = xr, a.i[V] = xi, a.r[W] = yr, a.i[W] = yi;
         }
     });
 }
@@ -168,35 +222,43 @@ __device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32
 // accumulates the raw sum s such that tap = factor * s (factor applied by the caller).
 //   RX: s = <l|X|p>, RY: s = <l|Y|p>/i, RZ: s = <l|Z|p> (tap = -i/2 * <l|P|p>)
 //   PS: s = sum conj(l_b) p_b (tap = i s);  GP: s = <l|p> (tap = -i s)
-template <typename T, int KIND, int Q>
+template <typename T, int KIND, int Q, bool CTRL>
 __device__ __forceinline__ void one_target_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl,
                                                T& sr, T& si) {
     for_pairs<Q>([&]<int V, int W>() {
-        if (cl && !ctrl_ok<V>(g, cl)) return;
+        T ur = 0, ui = 0;
+        T& xr = CTRL ? ur : sr;
+        T& xi = CTRL ? ui : si;
         if constexpr (KIND == OPK_RX) {
-            cdot_acc(l.r[V], l.i[V], p.r[W], p.i[W], sr, si);
-            cdot_acc(l.r[W], l.i[W], p.r[V], p.i[V], sr, si);
+            cdot_acc(l.r[V], l.i[V], p.r[W], p.i[W], xr, xi);
+            cdot_acc(l.r[W], l.i[W], p.r[V], p.i[V], xr, xi);
         } else if constexpr (KIND == OPK_RY) {  // <l|Y|p> = i (conj(l_b) p_a - conj(l_a) p_b)
-            cdot_acc(l.r[W], l.i[W], p.r[V], p.i[V], sr, si);
-            cdot_acc(l.r[V], l.i[V], -p.r[W], -p.i[W], sr, si);
+            cdot_acc(l.r[W], l.i[W], p.r[V], p.i[V], xr, xi);
+            cdot_acc(l.r[V], l.i[V], -p.r[W], -p.i[W], xr, xi);
         } else if constexpr (KIND == OPK_RZ) {
-            cdot_acc(l.r[V], l.i[V], p.r[V], p.i[V], sr, si);
-            cdot_acc(l.r[W], l.i[W], -p.r[W], -p.i[W], sr, si);
+            cdot_acc(l.r[V], l.i[V], p.r[V], p.i[V], xr, xi);
+            cdot_acc(l.r[W], l.i[W], -p.r[W], -p.i[W], xr, xi);
         } else if constexpr (KIND == OPK_PS) {
-            cdot_acc(l.r[W], l.i[W], p.r[W], p.i[W], sr, si);
+            cdot_acc(l.r[W], l.i[W], p.r[W], p.i[W], xr, xi);
         } else {  // GP
-            cdot_acc(l.r[V], l.i[V], p.r[V], p.i[V], sr, si);
-            cdot_acc(l.r[W], l.i[W], p.r[W], p.i[W], sr, si);
+            cdot_acc(l.r[V], l.i[V], p.r[V], p.i[V], xr, xi);
+            cdot_acc(l.r[W], l.i[W], p.r[W], p.i[W], xr, xi);
+        }
+        if constexpr (CTRL) {
+            const bool ok = ctrl_ok<V>(g, cl);
+            sr += ok ? ur : T(0), si += ok ? ui : T(0);
         }
     });
 }
 
 // Two-target ops; quad order (00, 01, 10, 11) with bit 0 <=> Q0 (matrix bit 0).
-template <typename T, int KIND, int Q0, int Q1>
+template <typename T, int KIND, int Q0, int Q1, bool CTRL>
 __device__ __forceinline__ void two_target(Regs<T>& a, const GroupCtx& g, uint32_t cl, T c, T s,
                                            const T* __restrict__ m) {
     for_quads<Q0, Q1>([&]<int V0, int V1, int V2_, int V3>() {
-        if (cl && !ctrl_ok<V0>(g, cl)) return;
+        if constexpr (CTRL) {
+            if (!ctrl_ok<V0>(g, cl)) return;
+        }
         if constexpr (KIND == OPK_SWAP) {
             T tr = a.r[V1], ti = a.i[V1];
             a.r[V1] = a.r[V2_], a.i[V1] = a.i[V2_];
@@ -240,11 +302,13 @@ __device__ __forceinline__ void two_target(Regs<T>& a, const GroupCtx& g, uint32
 }
 
 // raw sum s with tap = -i/2 * s, s = <l|PP|p> for PP = XX, YY, ZZ
-template <typename T, int KIND, int Q0, int Q1>
+template <typename T, int KIND, int Q0, int Q1, bool CTRL>
 __device__ __forceinline__ void two_target_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl,
                                                T& sr, T& si) {
     for_quads<Q0, Q1>([&]<int V0, int V1, int V2_, int V3>() {
-        if (cl && !ctrl_ok<V0>(g, cl)) return;
+        if constexpr (CTRL) {
+            if (!ctrl_ok<V0>(g, cl)) return;
+        }
         if constexpr (KIND == OPK_RZZ) {
             cdot_acc(l.r[V0], l.i[V0], p.r[V0], p.i[V0], sr, si);
             cdot_acc(l.r[V1], l.i[V1], -p.r[V1], -p.i[V1], sr, si);
@@ -289,12 +353,6 @@ __device__ __forceinline__ void dense_from_lds(Regs<T>& a, const GroupCtx& g, ui
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void store_tap(double* lds_taps, int tap, int wave, int n_waves, double tr, double ti) {
-    lds_taps[2 * (tap * n_waves + wave)] = tr;
-    lds_taps[2 * (tap * n_waves + wave) + 1] = ti;
-}
-
 #define MQA_SW1(CALL)                 \
     switch (op.q0) {                  \
         case 0: CALL(0); break;       \
@@ -323,9 +381,13 @@ __device__ __forceinline__ void store_tap(double* lds_taps, int tap, int wave, i
 template <typename T, bool FULL>
 __device__ __forceinline__ void apply_op(int kind, const DevOp& op, Regs<T>& a, const GroupCtx& g, uint32_t cl, T c,
                                          T s, const T* __restrict__ m) {
-#define C1(KD) [&]<int Q>() { one_target<T, KD, Q>(a, g, cl, c, s, m); }
-#define ONE(KD, Q) one_target<T, KD, Q>(a, g, cl, c, s, m)
-#define TWO(KD, Q0, Q1) two_target<T, KD, Q0, Q1>(a, g, cl, c, s, m)
+    // lean passes never carry local control predicates (the host picks FULL when they do)
+#define ONE(KD, Q)                                                                       \
+    ((FULL && cl) ? one_target<T, KD, Q, FULL>(a, g, cl, c, s, m) : one_target<T, KD, Q, false>(a, g, cl, c, s, m))
+#define TWO(KD, Q0, Q1)                                                                        \
+    ((FULL && cl) ? two_target<T, KD, Q0, Q1, FULL>(a, g, cl, c, s, m) : two_target<T, KD, Q0, Q1, false>(a, g, cl, c, s, m))
+#define CXZ(ISX, Q0, Q1)                                                                     \
+    ((FULL && cl) ? controlled_xz<T, ISX, Q0, Q1, FULL>(a, g, cl) : controlled_xz<T, ISX, Q0, Q1, false>(a, g, cl))
 #define X1(Q) ONE(OPK_X, Q)
 #define RX1(Q) ONE(OPK_RX, Q)
 #define RY1(Q) ONE(OPK_RY, Q)
@@ -338,6 +400,8 @@ __device__ __forceinline__ void apply_op(int kind, const DevOp& op, Regs<T>& a, 
 #define XX2(Q0, Q1) TWO(OPK_RXX, Q0, Q1)
 #define YY2(Q0, Q1) TWO(OPK_RYY, Q0, Q1)
 #define M22(Q0, Q1) TWO(OPK_M2, Q0, Q1)
+#define CX2(Q0, Q1) CXZ(true, Q0, Q1)
+#define CZ2(Q0, Q1) CXZ(false, Q0, Q1)
     switch (kind) {
         case OPK_X: MQA_SW1(X1) break;
         case OPK_RX: MQA_SW1(RX1) break;
@@ -346,6 +410,8 @@ __device__ __forceinline__ void apply_op(int kind, const DevOp& op, Regs<T>& a, 
         case OPK_PS: MQA_SW1(PS1) break;
         case OPK_GP: MQA_SW1(GP1) break;
         case OPK_M1: MQA_SW1(M11) break;
+        case OPK_CX: MQA_SW2(CX2) break;
+        case OPK_CZ: MQA_SW2(CZ2) break;
         case OPK_SWAP: MQA_SW2(SW2) break;
         case OPK_RZZ: MQA_SW2(ZZ2) break;
         case OPK_RXX: MQA_SW2(XX2) break;
@@ -355,9 +421,9 @@ __device__ __forceinline__ void apply_op(int kind, const DevOp& op, Regs<T>& a, 
             break;
         default: break;
     }
-#undef C1
 #undef ONE
 #undef TWO
+#undef CXZ
 #undef X1
 #undef RX1
 #undef RY1
@@ -370,20 +436,16 @@ __device__ __forceinline__ void apply_op(int kind, const DevOp& op, Regs<T>& a, 
 #undef XX2
 #undef YY2
 #undef M22
+#undef CX2
+#undef CZ2
 }
 
-// One op of a group.  ADJ: emit the derivative tap, then apply G^dagger to psi and lambda.
+// One op of a group.  ADJ: accumulate the derivative tap, then apply G^dagger to psi and lambda.
 template <typename T, bool ADJ, bool FULL>
 __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g, uint64_t base,
                                        const T* __restrict__ pool, const double* __restrict__ ang_row,
-                                       double* __restrict__ lds_taps, int wave, int n_waves, int wave_width) {
-    const bool lane0 = (threadIdx.x & 63) == 0;
-    if ((base & op.ctrl_global) != op.ctrl_global) {  // a non-local control is off for this tile
-        if constexpr (ADJ) {
-            if (op.tap >= 0 && lane0) store_tap<T>(lds_taps, op.tap, wave, n_waves, 0.0, 0.0);
-        }
-        return;
-    }
+                                       TapAcc<T>& acc) {
+    if ((base & op.ctrl_global) != op.ctrl_global) return;  // a non-local control is off: no-op, zero tap
     const uint32_t cl = op.ctrl_local;
     const int kind = op.kind;
     T c = 0, s = 0;
@@ -397,8 +459,12 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
     if constexpr (ADJ) {
         if (op.tap >= 0) {
             T sr = 0, si = 0;
-#define T1(KD, Q) one_target_tap<T, KD, Q>(p, l, g, cl, sr, si)
-#define T2(KD, Q0, Q1) two_target_tap<T, KD, Q0, Q1>(p, l, g, cl, sr, si)
+#define T1(KD, Q)                                                              \
+    ((FULL && cl) ? one_target_tap<T, KD, Q, FULL>(p, l, g, cl, sr, si)          \
+                  : one_target_tap<T, KD, Q, false>(p, l, g, cl, sr, si))
+#define T2(KD, Q0, Q1)                                                         \
+    ((FULL && cl) ? two_target_tap<T, KD, Q0, Q1, FULL>(p, l, g, cl, sr, si)     \
+                  : two_target_tap<T, KD, Q0, Q1, false>(p, l, g, cl, sr, si))
 #define RXT(Q) T1(OPK_RX, Q)
 #define RYT(Q) T1(OPK_RY, Q)
 #define RZT(Q) T1(OPK_RZ, Q)
@@ -427,21 +493,23 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
 #undef XXT
 #undef YYT
 #undef ZZT
-            sr = wave_sum(sr, wave_width);
-            si = wave_sum(si, wave_width);
-            if (lane0) {
-                double tr, ti;
-                if (kind == OPK_PS) {  // i s
-                    tr = -double(si), ti = double(sr);
-                } else if (kind == OPK_GP) {  // -i s
-                    tr = double(si), ti = -double(sr);
-                } else if (kind == OPK_RY) {  // -i/2 * (i s)
-                    tr = 0.5 * double(sr), ti = 0.5 * double(si);
-                } else {  // -i/2 * s
-                    tr = 0.5 * double(si), ti = -0.5 * double(sr);
-                }
-                store_tap<T>(lds_taps, op.tap, wave, n_waves, tr, ti);
+            // tap = factor * s
+            T tr, ti;
+            if (kind == OPK_PS) {  // i s
+                tr = -si, ti = sr;
+            } else if (kind == OPK_GP) {  // -i s
+                tr = si, ti = -sr;
+            } else if (kind == OPK_RY) {  // -i/2 * (i s)
+                tr = T(0.5) * sr, ti = T(0.5) * si;
+            } else {  // -i/2 * s
+                tr = T(0.5) * si, ti = T(-0.5) * sr;
             }
+            // the group's taps arrive in order: shift them through the register file so that
+            // every access uses a constant index (no dynamically indexed, scratch-backed array)
+#pragma unroll
+            for (int j = kTapSlots<T> - 1; j > 0; j--) acc.r[j] = acc.r[j - 1], acc.i[j] = acc.i[j - 1];
+            acc.r[0] = tr;
+            acc.i[0] = ti;
         }
     }
     apply_op<T, FULL>(kind, op, p, g, cl, c, s, m);
@@ -512,14 +580,25 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
 #pragma unroll
         for (int k = 0; k < 4; k++) g.pm[k] = 1u << grp.pos[k];
 
+        // LDS slot of register v = swz(ub) ^ swz(offset of v): a per-thread base XOR a
+        // wave-uniform offset precomputed by the host
+        const uint32_t a0 = swz<T>(ub);
         Regs<T> p, l;
-        [&]<int... Vs>(std::integer_sequence<int, Vs...>) {
-            ((p.r[Vs] = t_psi[swz<T>(ub | pv<Vs>(g))].x, p.i[Vs] = t_psi[swz<T>(ub | pv<Vs>(g))].y), ...);
+#pragma unroll
+        for (int v = 0; v < kAmpsPerThread; v++) {
+            const V x = t_psi[a0 ^ grp.off[v]];
+            p.r[v] = x.x, p.i[v] = x.y;
             if constexpr (ADJ) {
-                ((l.r[Vs] = t_lam[swz<T>(ub | pv<Vs>(g))].x, l.i[Vs] = t_lam[swz<T>(ub | pv<Vs>(g))].y), ...);
+                const V y = t_lam[a0 ^ grp.off[v]];
+                l.r[v] = y.x, l.i[v] = y.y;
             }
-        }(std::make_integer_sequence<int, kAmpsPerThread>{});
+        }
 
+        TapAcc<T> acc;
+        if constexpr (ADJ) {
+#pragma unroll
+            for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0), acc.i[j] = T(0);
+        }
         const DevOp op0 = args.ops[grp.op_begin];
         if (FULL && (op0.kind == OPK_M3 || op0.kind == OPK_M4)) {  // a dense block alone in its group
             if ((base & op0.ctrl_global) == op0.ctrl_global) {
@@ -535,17 +614,45 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
         } else {
             for (int oi = 0; oi < grp.n_ops; oi++) {
                 const DevOp op = args.ops[grp.op_begin + oi];
-                run_op<T, ADJ, FULL>(op, p, l, g, base, static_cast<const T*>(args.pool), ang_row, lds_taps, wave, n_waves,
-                               wave_width);
+                run_op<T, ADJ, FULL>(op, p, l, g, base, static_cast<const T*>(args.pool), ang_row, acc);
+            }
+        }
+        if constexpr (ADJ) {
+            if (grp.n_taps > 0) {  // one batched cross-lane reduction per group
+                const bool lane0 = (tid & 63) == 0;
+                if (wave_width == 64) {
+#pragma unroll
+                    for (int j = 0; j < kTapSlots<T>; j++) {
+                        if (j < 4 || grp.n_taps > 4) {
+                            acc.r[j] = wave_sum64(acc.r[j]);
+                            acc.i[j] = wave_sum64(acc.i[j]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kTapSlots<T>; j++) {
+                        acc.r[j] = wave_sum(acc.r[j], wave_width);
+                        acc.i[j] = wave_sum(acc.i[j], wave_width);
+                    }
+                }
+                if (lane0) {
+#pragma unroll
+                    for (int j = 0; j < kTapSlots<T>; j++) {
+                        if (j < grp.n_taps) {  // acc[j] holds the group's (n_taps - 1 - j)-th tap
+                            const int t = grp.tap_begin + grp.n_taps - 1 - j;
+                            lds_taps[2 * (t * n_waves + wave)] = double(acc.r[j]);
+                            lds_taps[2 * (t * n_waves + wave) + 1] = double(acc.i[j]);
+                        }
+                    }
+                }
             }
         }
 
-        [&]<int... Vs>(std::integer_sequence<int, Vs...>) {
-            ((t_psi[swz<T>(ub | pv<Vs>(g))] = V{p.r[Vs], p.i[Vs]}), ...);
-            if constexpr (ADJ) {
-                ((t_lam[swz<T>(ub | pv<Vs>(g))] = V{l.r[Vs], l.i[Vs]}), ...);
-            }
-        }(std::make_integer_sequence<int, kAmpsPerThread>{});
+#pragma unroll
+        for (int v = 0; v < kAmpsPerThread; v++) {
+            t_psi[a0 ^ grp.off[v]] = V{p.r[v], p.i[v]};
+            if constexpr (ADJ) t_lam[a0 ^ grp.off[v]] = V{l.r[v], l.i[v]};
+        }
         __syncthreads();
     }
 

@@ -19,6 +19,7 @@ namespace mqa {
 constexpr int kGroupWidth = 4;                   // register positions per group
 constexpr int kAmpsPerThread = 1 << kGroupWidth;  // 16 amplitudes per thread per state
 constexpr int kMaxTapsPerPass = 96;               // LDS tap slots per pass
+constexpr int kMaxTapsPerGroup = 8;               // register tap accumulators per thread
 constexpr int kMinTileQubits = 6;                 // smallest tile: 4 threads
 constexpr int kMaxTileQubits = 13;                // largest tile: 512 threads
 
@@ -30,6 +31,8 @@ enum OpKind : int32_t {
     OPK_M2 = 7,  // dense 4x4 from the pool (ISWAP, 2-qubit matrices)
     OPK_M3 = 8,  // dense 8x8 over group positions 0..2 (3-qubit matrices)
     OPK_M4 = 9,  // dense 16x16 over group positions 0..3 (4-qubit matrices)
+    OPK_CX = 10, // X on q0 controlled by register position q1 (CNOT), plus any other controls
+    OPK_CZ = 11, // Z on q0 controlled by register position q1 (CZ), plus any other controls
     // parameterised: the angle comes from the per-sample angle table
     OPK_RX = 16,
     OPK_RY = 17,
@@ -43,7 +46,7 @@ enum OpKind : int32_t {
 
 inline bool opk_is_param(int k) { return k >= OPK_RX; }
 inline bool opk_is_two(int k) {
-    return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ;
+    return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ || k == OPK_CX || k == OPK_CZ;
 }
 
 // Byte layout of a tile pass's dynamic LDS: [psi tile][lambda tile (adjoint)][offset table]
@@ -78,11 +81,27 @@ struct DevOp {             // 32 bytes
     uint64_t ctrl_global;  // control qubits that are not local, as a global index mask
 };
 
-struct DevGroup {          // 16 bytes
+// XOR swizzle of a tile index into its LDS slot.  It folds higher index bits into the low bits
+// so that register groups whose positions include low qubits do not send every lane of a wave
+// to one bank.  It is linear over GF(2): swz(a ^ b) == swz(a) ^ swz(b), which lets the kernel
+// combine a per-thread base with per-group, wave-uniform register offsets.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t lds_swizzle(uint32_t u, int amp_bytes) {
+    if (amp_bytes == 8) return u ^ (((u >> 5) ^ (u >> 10)) & 31u);
+    return u ^ (((u >> 4) ^ (u >> 8) ^ (u >> 12)) & 15u);
+}
+
+struct DevGroup {          // 64 bytes
     uint8_t pos[4];        // local positions; register index bit i <=> pos[i]
     uint8_t spos[4];       // the same positions sorted ascending
     int32_t op_begin;
     int32_t n_ops;
+    int32_t tap_begin;     // pass-level index of the group's first tap (taps are consecutive)
+    int32_t n_taps;        // <= kMaxTapsPerGroup
+    int32_t pad[2];
+    uint16_t off[16];      // lds_swizzle(register offset of register v) for v = 0..15
 };
 
 struct DevPass {           // 48 bytes

@@ -101,6 +101,7 @@ struct PassBuilder {
     const std::vector<CGate>& gates;
     bool adjoint;
     int W;
+    int taps_per_group;
     std::vector<int> tap_id_of_slot;  // angle slot -> global tap id (-1: no tap)
 
     // Split the pass's gates (dependency-ordered) into register groups of <= W target positions.
@@ -109,6 +110,7 @@ struct PassBuilder {
         std::vector<int> todo = chosen;
         while (!todo.empty()) {
             HostGroup grp;
+            grp.tap_begin = int(pass->tap_slots.size());
             std::vector<int> gpos;  // positions in register order
             std::vector<int> keep;
             uint64_t blocked = 0;
@@ -116,13 +118,15 @@ struct PassBuilder {
             for (int gi : todo) {
                 const CGate& g = gates[gi];
                 uint64_t q = g.qubits();
-                if (closed || (q & blocked)) {
+                const bool taps_full = adjoint && g.grad && g.slot >= 0 && grp.n_taps >= taps_per_group;
+                if (closed || (q & blocked) || taps_full) {
                     blocked |= q;
                     keep.push_back(gi);
                     continue;
                 }
                 std::vector<cplx> m;
                 int kind = lower_kind(g, adjoint, &m);
+                int reg_ctrl = -1;  // control qubit bound to a register position (CX / CZ)
                 if (kind == OPK_M3 || kind == OPK_M4) {
                     if (!grp.ops.empty()) {  // dense multi-qubit blocks take a group of their own
                         blocked |= q;
@@ -135,10 +139,28 @@ struct PassBuilder {
                     std::vector<int> merged = gpos;
                     for (int t : g.objs)
                         if (std::find(merged.begin(), merged.end(), pos_of[t]) == merged.end()) merged.push_back(pos_of[t]);
+                    // CNOT / CZ: one local control is bound to a register position, so that the
+                    // control bit is a compile-time property of each register (no predicate)
+                    const uint64_t lctrl = g.ctrl & ~nonlocal;
+                    int want_ctrl = -1;
+                    if ((g.kind == MQA_GATE_X || g.kind == MQA_GATE_Z) && lctrl) {
+                        for (int c = 0; c < 64 && want_ctrl < 0; c++)
+                            if (((lctrl >> c) & 1) && std::find(merged.begin(), merged.end(), pos_of[c]) != merged.end())
+                                want_ctrl = c;
+                        for (int c = 0; c < 64 && want_ctrl < 0; c++)
+                            if ((lctrl >> c) & 1) want_ctrl = c;
+                        if (std::find(merged.begin(), merged.end(), pos_of[want_ctrl]) == merged.end())
+                            merged.push_back(pos_of[want_ctrl]);
+                    }
                     if (int(merged.size()) > W && !grp.ops.empty()) {
                         blocked |= q;
                         keep.push_back(gi);
                         continue;
+                    }
+                    if (want_ctrl >= 0) {
+                        reg_ctrl = want_ctrl;
+                        kind = (g.kind == MQA_GATE_X) ? OPK_CX : OPK_CZ;
+                        m.clear();
                     }
                     gpos = merged;
                 }
@@ -151,6 +173,7 @@ struct PassBuilder {
                 if (adjoint && opk_is_param(kind) && g.grad) {
                     d.tap = int(pass->tap_slots.size());
                     pass->tap_slots.push_back(tap_id_of_slot[g.slot]);
+                    grp.n_taps++;
                 }
                 d.mat = -1;
                 if (!m.empty()) {
@@ -164,11 +187,12 @@ struct PassBuilder {
                     return int(std::find(gpos.begin(), gpos.end(), pos_of[t]) - gpos.begin());
                 };
                 d.q0 = uint8_t(reg(g.objs[0]));
-                d.q1 = uint8_t(g.objs.size() > 1 ? reg(g.objs[1]) : 0);
+                d.q1 = uint8_t(reg_ctrl >= 0 ? reg(reg_ctrl) : (g.objs.size() > 1 ? reg(g.objs[1]) : 0));
                 d.ctrl_global = g.ctrl & nonlocal;
                 d.ctrl_local = 0;
+                const uint64_t rest = g.ctrl & ~nonlocal & ~(reg_ctrl >= 0 ? (uint64_t(1) << reg_ctrl) : uint64_t(0));
                 for (int c = 0; c < 64; c++)
-                    if (((g.ctrl & ~nonlocal) >> c) & 1) d.ctrl_local |= uint32_t(1) << pos_of[c];
+                    if ((rest >> c) & 1) d.ctrl_local |= uint32_t(1) << pos_of[c];
                 grp.ops.push_back(op);
             }
             // pad the group to 4 distinct positions
@@ -211,7 +235,7 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     }
     if (adjoint) std::reverse(remaining.begin(), remaining.end());
 
-    PassBuilder pb{gates, adjoint, W, {}};
+    PassBuilder pb{gates, adjoint, W, std::min(std::max(opts.taps_per_group, 1), kMaxTapsPerGroup), {}};
     int n_slots = 0;
     for (auto& g : gates) n_slots = std::max(n_slots, g.slot + 1);
     pb.tap_id_of_slot.assign(n_slots, -1);
@@ -283,6 +307,14 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
             for (int i = 0; i < 4; i++) dg.spos[i] = uint8_t(sp[i]);
             dg.op_begin = int(plan.dev_ops.size());
             dg.n_ops = int(grp.ops.size());
+            dg.tap_begin = grp.tap_begin;
+            dg.n_taps = grp.n_taps;
+            for (int v = 0; v < 16; v++) {
+                uint32_t u = 0;
+                for (int i = 0; i < 4; i++)
+                    if ((v >> i) & 1) u |= uint32_t(1) << grp.pos[i];
+                dg.off[v] = uint16_t(lds_swizzle(u, opts.amp_bytes));
+            }
             for (auto& op : grp.ops) plan.dev_ops.push_back(op.dev);
             plan.dev_groups.push_back(dg);
         }

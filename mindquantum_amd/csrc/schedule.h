@@ -30,9 +30,12 @@ struct SchedOptions {
     int low_run = 5;           // C: lowest qubits always local (contiguous runs of 2^C amplitudes)
     int group_width = 4;       // max target positions per register group (1..4)
     int max_ops_per_pass = 0;  // 0: unlimited; 1 gives the unfused one-gate-per-pass schedule
+    int taps_per_group = 8;    // derivative taps a register group may hold (<= kMaxTapsPerGroup)
+    int amp_bytes = 8;         // 8 (complex64) or 16 (complex128): selects the LDS swizzle
     bool operator==(const SchedOptions& o) const {
         return tile_qubits == o.tile_qubits && low_run == o.low_run && group_width == o.group_width &&
-               max_ops_per_pass == o.max_ops_per_pass;
+               max_ops_per_pass == o.max_ops_per_pass && taps_per_group == o.taps_per_group &&
+               amp_bytes == o.amp_bytes;
     }
 };
 
@@ -43,6 +46,8 @@ struct HostOp {
 struct HostGroup {
     int pos[4] = {0, 1, 2, 3};
     std::vector<HostOp> ops;
+    int tap_begin = 0;
+    int n_taps = 0;
 };
 struct HostPass {
     std::vector<int> local_qubits;  // sorted ascending, size K

@@ -54,3 +54,43 @@ def test_lds_regions_disjoint():
 def test_default_tiles_fit_one_cu(amp, adj, K, C):
     tile = (1 << K) * amp * (2 if adj else 1)
     assert tile + (1 << (K - C)) * 8 + 96 * ((1 << (K - 4)) // 64 or 1) * 16 <= 160 * 1024
+
+
+SWZ = r"""
+#include <cstdio>
+#include <vector>
+#include "program.h"
+int main() {
+    int bad = 0;
+    for (int amp : {8, 16}) {
+        for (int K = mqa::kMinTileQubits; K <= mqa::kMaxTileQubits; K++) {
+            const unsigned n = 1u << K;
+            std::vector<int> seen(n, 0);
+            for (unsigned u = 0; u < n; u++) {
+                unsigned s = mqa::lds_swizzle(u, amp);
+                if (s >= n) bad++; else seen[s]++;
+            }
+            for (unsigned u = 0; u < n; u++) if (seen[u] != 1) bad++;
+            for (unsigned a = 0; a < n; a += 37)
+                for (unsigned b = 0; b < n; b += 53)
+                    if (mqa::lds_swizzle(a ^ b, amp) != (mqa::lds_swizzle(a, amp) ^ mqa::lds_swizzle(b, amp))) bad++;
+        }
+    }
+    std::printf("%d\n", bad);
+    return 0;
+}
+"""
+
+
+def test_lds_swizzle_is_a_linear_bijection():
+    """The kernel forms LDS slots as swz(base) ^ swz(offset); that needs XOR-linearity, and the
+    slots must be a permutation of the tile for every tile size."""
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "swz.cpp")
+        exe = os.path.join(d, "swz")
+        with open(src, "w") as f:
+            f.write(SWZ.replace("#include <cstdio>", "#include <cstdio>\n#include <initializer_list>"))
+        subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "mindquantum_amd", "csrc"), src, "-o", exe],
+                       check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    assert int(out.strip()) == 0
