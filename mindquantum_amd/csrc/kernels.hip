@@ -518,6 +518,71 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
 #undef MQA_SW1
 #undef MQA_SW2
 
+// ---------------------------------------------------------------- tile I/O
+// HBM <-> LDS transfer of one tile.  Every access of a thread is issued before the first is
+// waited on (full unroll) so that enough bytes are in flight per CU to stream at HBM rate.
+// complex64 moves two adjacent amplitudes (16 B) per access: for u even, u and u + 1 are
+// adjacent in HBM (the low run C >= 1 is contiguous) and share the 16-byte LDS slot at
+// swz(u) & ~1, since swz(u ^ 1) == swz(u) ^ 1; when the swizzle sets bit 0 of swz(u) the
+// two amplitudes sit in that slot in swapped order.
+__device__ __forceinline__ float4 swap_halves_if(float4 v, bool odd) {
+    return odd ? make_float4(v.z, v.w, v.x, v.y) : v;
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* __restrict__ g,
+                                          const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT) {
+    const uint32_t cmask = (1u << C) - 1u;
+    if constexpr (sizeof(T) == 4) {
+        float4 buf[kAmpsPerThread / 2];
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t u = 2u * (tid + NT * j);
+            buf[j] = *reinterpret_cast<const float4*>(g + ((u & cmask) | offtab[u >> C]));
+        }
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t a = swz<T>(2u * (tid + NT * j));
+            *reinterpret_cast<float4*>(lds + (a & ~1u)) = swap_halves_if(buf[j], a & 1u);
+        }
+    } else {
+        // complex128: two batches of 8 x 16 B, the same bytes in flight as complex64
+#pragma unroll
+        for (int h = 0; h < kAmpsPerThread; h += kAmpsPerThread / 2) {
+            V2<T> buf[kAmpsPerThread / 2];
+#pragma unroll
+            for (int j = 0; j < kAmpsPerThread / 2; j++) {
+                const uint32_t u = tid + NT * (h + j);
+                buf[j] = g[(u & cmask) | offtab[u >> C]];
+            }
+#pragma unroll
+            for (int j = 0; j < kAmpsPerThread / 2; j++) lds[swz<T>(tid + NT * (h + j))] = buf[j];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_store(const V2<T>* __restrict__ lds, V2<T>* __restrict__ g,
+                                           const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT) {
+    const uint32_t cmask = (1u << C) - 1u;
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t u = 2u * (tid + NT * j);
+            const uint32_t a = swz<T>(u);
+            *reinterpret_cast<float4*>(g + ((u & cmask) | offtab[u >> C])) =
+                swap_halves_if(*reinterpret_cast<const float4*>(lds + (a & ~1u)), a & 1u);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread; j++) {
+            const uint32_t u = tid + NT * j;
+            g[(u & cmask) | offtab[u >> C]] = lds[swz<T>(u)];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- the tile pass
 template <typename T, bool ADJ, bool FULL>
 __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
@@ -544,22 +609,8 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
 
     V* g_psi = static_cast<V*>(args.psi) + sample * args.stride + base;
     V* g_lam = ADJ ? (static_cast<V*>(args.lam) + sample * args.stride + base) : nullptr;
-    const uint32_t cmask = (1u << C) - 1u;
-    // HBM -> LDS; partially unrolled to bound the registers held by loads in flight
-#pragma unroll 4
-    for (int j = 0; j < kAmpsPerThread; j++) {
-        uint32_t u = tid + NT * j;
-        uint64_t off = (u & cmask) | offtab[u >> C];
-        t_psi[swz<T>(u)] = g_psi[off];
-    }
-    if constexpr (ADJ) {
-#pragma unroll 4
-        for (int j = 0; j < kAmpsPerThread; j++) {
-            uint32_t u = tid + NT * j;
-            uint64_t off = (u & cmask) | offtab[u >> C];
-            t_lam[swz<T>(u)] = g_lam[off];
-        }
-    }
+    tile_load<T>(t_psi, g_psi, offtab, C, tid, NT);
+    if constexpr (ADJ) tile_load<T>(t_lam, g_lam, offtab, C, tid, NT);
     __syncthreads();
 
     const int wave = int(tid >> 6);
@@ -656,13 +707,12 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
         __syncthreads();
     }
 
-#pragma unroll 4
-    for (int j = 0; j < kAmpsPerThread; j++) {
-        uint32_t u = tid + NT * j;
-        uint64_t off = (u & cmask) | offtab[u >> C];
-        g_psi[off] = t_psi[swz<T>(u)];
-        if constexpr (ADJ) g_lam[off] = t_lam[swz<T>(u)];
-    }
+    // launder tid so that the address arithmetic of the load phase is recomputed here rather
+    // than kept live in registers through the whole group loop
+    uint32_t tid_st = tid;
+    asm volatile("" : "+v"(tid_st));
+    tile_store<T>(t_psi, g_psi, offtab, C, tid_st, NT);
+    if constexpr (ADJ) tile_store<T>(t_lam, g_lam, offtab, C, tid_st, NT);
 
     if constexpr (ADJ) {
         // one partial per (sample, tap id, tile); summed over tiles by k_reduce_rows

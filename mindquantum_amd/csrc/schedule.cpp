@@ -216,7 +216,8 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     if (n_eff < kMinTileQubits) throw std::invalid_argument("state too small for the tile engine");
     const uint64_t all = (n_eff >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n_eff) - 1);
     int K = std::min(std::max(opts.tile_qubits, kMinTileQubits), std::min(n_eff, kMaxTileQubits));
-    int C = std::min(std::max(opts.low_run, 0), K - kGroupWidth);
+    // at least one contiguous low qubit: the kernels move complex64 amplitudes in pairs
+    int C = std::min(std::max(opts.low_run, 1), K - kGroupWidth);
     const bool whole = (n_eff <= K);
     if (whole) C = K;
     const int W = std::min(std::max(opts.group_width, 1), kGroupWidth);

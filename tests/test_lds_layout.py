@@ -74,6 +74,18 @@ int main() {
             for (unsigned a = 0; a < n; a += 37)
                 for (unsigned b = 0; b < n; b += 53)
                     if (mqa::lds_swizzle(a ^ b, amp) != (mqa::lds_swizzle(a, amp) ^ mqa::lds_swizzle(b, amp))) bad++;
+            if (amp == 8) {
+                // complex64 tile I/O moves amplitudes u, u + 1 (u even) through the 16-byte slot at
+                // swz(u) & ~1, halves swapped when swz(u) is odd; every amplitude must land on swz(u)
+                std::vector<unsigned> lds(n, ~0u);
+                for (unsigned u = 0; u < n; u += 2) {
+                    if (mqa::lds_swizzle(u | 1, amp) != (mqa::lds_swizzle(u, amp) ^ 1u)) bad++;
+                    const unsigned a = mqa::lds_swizzle(u, amp), slot = a & ~1u;
+                    lds[slot] = (a & 1u) ? u + 1 : u;
+                    lds[slot + 1] = (a & 1u) ? u : u + 1;
+                }
+                for (unsigned u = 0; u < n; u++) if (lds[mqa::lds_swizzle(u, amp)] != u) bad++;
+            }
         }
     }
     std::printf("%d\n", bad);
@@ -84,7 +96,8 @@ int main() {
 
 def test_lds_swizzle_is_a_linear_bijection():
     """The kernel forms LDS slots as swz(base) ^ swz(offset); that needs XOR-linearity, and the
-    slots must be a permutation of the tile for every tile size."""
+    slots must be a permutation of the tile for every tile size.  The complex64 tile I/O moves
+    amplitude pairs through 16-byte slots, which needs swz(u | 1) == swz(u) ^ 1."""
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "swz.cpp")
         exe = os.path.join(d, "swz")
