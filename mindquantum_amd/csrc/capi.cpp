@@ -67,6 +67,7 @@ struct Options {
     std::atomic<int64_t> group_width{4}, lowq_c64{5}, lowq_c128{4}, max_ops{0};
     std::atomic<int64_t> budget{0};  // bytes of HBM a batched call may use (0: 85% of free memory)
     std::atomic<int64_t> profile{0};  // 1: time kernels with HIP events
+    std::atomic<int64_t> ham_tile_c64{14}, ham_tile_c128{13}, ham_engine{1};  // ham_engine 0: gather kernel only
 };
 Options g_opt;
 
@@ -80,7 +81,9 @@ std::vector<OptEntry> opt_table() {
             {"adj_tile_qubits_c64", &g_opt.adj_tile_c64, 12}, {"adj_tile_qubits_c128", &g_opt.adj_tile_c128, 11},
             {"group_width", &g_opt.group_width, 4},       {"lowq_c64", &g_opt.lowq_c64, 5},
             {"lowq_c128", &g_opt.lowq_c128, 4},           {"max_ops_per_pass", &g_opt.max_ops, 0},
-            {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0}};
+            {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0},
+            {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
+            {"ham_tile_engine", &g_opt.ham_engine, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -282,6 +285,33 @@ struct mqa_ham {
     std::mutex mu;
     std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev;
 
+    // tile-engine plans keyed by (device, n_eff, K, C); ok == false: some flip fits no tile
+    struct TilePlan {
+        bool ok = false;
+        mqa::HamTilePlan plan;
+        mqa::DevBuf passes, groups, terms;
+    };
+    std::map<std::tuple<int, int, int, int>, std::unique_ptr<TilePlan>> tile_plans;
+
+    TilePlan* tile_plan(int n_eff, int K, int C, hipStream_t s) {
+        const int d = mqa::current_device();
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_tuple(d, n_eff, K, C);
+        auto it = tile_plans.find(key);
+        if (it != tile_plans.end()) return it->second.get();
+        auto tp = std::make_unique<TilePlan>();
+        tp->ok = mqa::build_ham_plan(groups, dterms, n_eff, K, C, tp->plan);
+        if (tp->ok) {
+            mqa::upload(tp->passes, tp->plan.passes, s);
+            mqa::upload(tp->groups, tp->plan.groups, s);
+            mqa::upload(tp->terms, tp->plan.terms, s);
+            mqa::hip_check(hipStreamSynchronize(s), "ham plan upload");
+        }
+        auto* raw = tp.get();
+        tile_plans[key] = std::move(tp);
+        return raw;
+    }
+
     mqa::DevBuf* on_device(hipStream_t s) {
         int d = mqa::current_device();
         std::lock_guard<std::mutex> lk(mu);
@@ -413,8 +443,49 @@ const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_ho
     return sim->angles.as<double>();
 }
 
+// out = H ket over a batch with the tile engine; with `expect`, per-sample <ket|H|ket> in
+// sim->fsum.  Returns false (nothing launched) when the Hamiltonian has no tile plan.
+bool ham_apply_tiles(mqa_sim* sim, mqa_ham* ham, const void* ket, void* out, bool expect, uint64_t batch) {
+    if (!g_opt.ham_engine) return false;
+    // complex128 tiles above 13 qubits exceed the LDS of a CU
+    const int K = int(sim->dtype == MQA_C64 ? g_opt.ham_tile_c64.load() : std::min<int64_t>(g_opt.ham_tile_c128, 13));
+    const int C = int(sim->dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
+    mqa_ham::TilePlan* tp = ham->tile_plan(sim->n_eff, K, C, sim->stream);
+    if (!tp->ok) return false;
+    const HamTilePlan& pl = tp->plan;
+    const uint64_t n_tiles = uint64_t(1) << (sim->n_eff - pl.K);
+    HamTileArgs h{};
+    h.ket = ket;
+    h.out = out;
+    h.stride = sim->dim_eff;
+    h.K = pl.K;
+    h.passes = tp->passes.as<DevHamPass>();
+    h.groups = tp->groups.as<DevHamTileGroup>();
+    h.terms = tp->terms.as<DevHamTileTerm>();
+    if (expect) {
+        sim->fpart.ensure(batch * n_tiles * 2 * sizeof(double));
+        sim->fsum.ensure(batch * 2 * sizeof(double));
+    }
+    const int np = int(pl.passes.size());
+    for (int p = 0; p < np; p++) {
+        h.pass_index = p;
+        h.partial = (expect && p == np - 1) ? sim->fpart.as<double>() : nullptr;
+        const double bytes = double(batch) * double(sim->state_bytes()) * (p == 0 ? 2.0 : 3.0);
+        timed(sim, "hamiltonian", bytes, [&] {
+            hip_check(launch_ham_tile(sim->dtype, p == 0, p == np - 1, h, pl.C, n_tiles, batch, sim->stream),
+                      "hamiltonian tile pass");
+        });
+    }
+    if (expect)
+        hip_check(launch_reduce_rows(sim->fpart.as<double>(), uint32_t(n_tiles), sim->fsum.as<double>(), batch,
+                                     sim->stream),
+                  "reduce");
+    return true;
+}
+
 // <bra|H|ket> or H ket over a batch; returns per-sample sums in sim->fsum (device, [batch][2]).
 void ham_apply(mqa_sim* sim, mqa_ham* ham, const void* ket, const void* bra, void* out, uint64_t batch) {
+    if (out && (bra == nullptr || bra == ket) && ham_apply_tiles(sim, ham, ket, out, bra != nullptr, batch)) return;
     DevBuf* hd = ham->on_device(sim->stream);
     HamArgs h{};
     h.ket = ket;

@@ -739,7 +739,8 @@ __device__ __forceinline__ void kahan_add(T x, T& sum, T& comp) {
     sum = t;
 }
 
-// Block-wide double sum of (a, b) written to out[0..1] by thread 0.
+// Block-wide double sum of (a, b) written to out[0..1] by thread 0 (up to 16 waves).
+constexpr size_t kBlockSumLds = 2 * 16 * sizeof(double);
 __device__ __forceinline__ void block_sum2(double a, double b, double* out) {
     __shared__ double red[2 * 16];
     a = wave_sum(a, 64);
@@ -800,6 +801,185 @@ __global__ void __launch_bounds__(256) k_ham(HamArgs args) {
     }
     if (args.partial) {
         block_sum2(double(fr), double(fi), args.partial + 2 * (sample * gridDim.x + blockIdx.x));
+    }
+}
+
+// ---------------------------------------------------------------- Hamiltonian tile pass
+// out (+)= sum_g w_g(u) * ket[u ^ flip_g] over the flip groups of one pass, for every amplitude u
+// of an LDS-staged ket tile (reference: cpu_vector_core_policy.cpp:132-161 ApplyTerms, which
+// sweeps the whole state once per term).  The term sign (-1)^{popcount(src & yz)} factors over
+// the global index bits: bits outside the tile (one parity per tile), the thread-index bits of the
+// tile index and the register bits, so most terms cost a few operations per thread, not per
+// amplitude.  Each thread owns 16 amplitudes, u(J) = 2 tid | (J & 1) | (J >> 1) << (K - 3),
+// read and written in adjacent pairs.  The last pass also reduces <ket|out> per tile.
+template <typename T>
+__device__ __forceinline__ T flip_sign(T x, uint32_t neg);
+template <>
+__device__ __forceinline__ float flip_sign<float>(float x, uint32_t neg) {
+    return __uint_as_float(__float_as_uint(x) ^ (neg << 31));
+}
+template <>
+__device__ __forceinline__ double flip_sign<double>(double x, uint32_t neg) {
+    return __longlong_as_double(__double_as_longlong(x) ^ (static_cast<long long>(neg) << 63));
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_load(const V2<T>* p, T& r0, T& i0, T& r1, T& i1) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(p);
+        r0 = v.x, i0 = v.y, r1 = v.z, i1 = v.w;
+    } else {
+        const double2 a = p[0], b = p[1];
+        r0 = a.x, i0 = a.y, r1 = b.x, i1 = b.y;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void pair_store(V2<T>* p, T r0, T i0, T r1, T i1) {
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(r0, i0, r1, i1);
+    } else {
+        p[0] = double2{r0, i0};
+        p[1] = double2{r1, i1};
+    }
+}
+
+template <typename T>
+constexpr int ham_tile_threads() {
+    return sizeof(T) == 4 ? 1024 : 512;
+}
+
+template <typename T, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile(HamTileArgs args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using V = V2<T>;
+    constexpr int NA = kAmpsPerThread;
+    const int K = args.K;
+    const uint32_t NT = blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    const DevHamPass pass = args.passes[args.pass_index];
+    const int C = pass.low_run;
+    const int H = K - C;
+    const LdsLayout lay = lds_layout(K, C, false, int(sizeof(V)), 0);
+    V* t_ket = reinterpret_cast<V*>(smem + lay.psi);
+    uint64_t* offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
+
+    const uint64_t sample = blockIdx.y;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = pdep64(tile, pass.nonlocal_mask);
+    for (uint32_t h = tid; h < (1u << H); h += NT) offtab[h] = pdep64(h, pass.local_hi_mask);
+    __syncthreads();
+
+    const V* g_ket = static_cast<const V*>(args.ket) + sample * args.stride + base;
+    V* g_out = static_cast<V*>(args.out) + sample * args.stride + base;
+    tile_load<T>(t_ket, g_ket, offtab, C, tid, NT);
+
+    const uint32_t cmask = (1u << C) - 1u;
+    const int JS = K - 3;  // tile-index position of register bit 1
+    T ar[NA], ai[NA];
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int J = 0; J < NA; J++) ar[J] = ai[J] = T(0);
+    } else {
+#pragma unroll
+        for (int P = 0; P < NA / 2; P++) {
+            const uint32_t u = (2u * tid) | (uint32_t(P) << JS);
+            pair_load<T>(g_out + ((u & cmask) | offtab[u >> C]), ar[2 * P], ai[2 * P], ar[2 * P + 1], ai[2 * P + 1]);
+        }
+    }
+    // LDS slot of owned amplitude J: swz(2 tid) ^ swz(jb(J)), the swizzle being XOR-linear
+    const uint32_t a0 = swz<T>(2u * tid);
+    auto jb = [JS](int J) { return uint32_t(J & 1) | (uint32_t(J >> 1) << JS); };
+    __syncthreads();
+
+    const uint32_t tmask = ((1u << JS) - 1u) & ~1u;
+    for (int gi = 0; gi < pass.n_groups; gi++) {
+        const DevHamTileGroup grp = args.groups[pass.group_begin + gi];
+        const DevHamTileTerm* tm = args.terms + grp.term_begin;
+        if (grp.kind == HAMW_TILE) {
+            T wr = 0, wi = 0;
+            for (int t = 0; t < grp.n_terms; t++) {
+                const uint32_t neg = uint32_t(__popcll(base & tm[t].yz_nl)) & 1u;
+                wr += flip_sign(T(tm[t].coef_re), neg);
+                wi += flip_sign(T(tm[t].coef_im), neg);
+            }
+#pragma unroll
+            for (int J = 0; J < NA; J++) {
+                const V x = t_ket[a0 ^ swz<T>(jb(J) ^ grp.flipl)];
+                ar[J] += wr * x.x - wi * x.y;
+                ai[J] += wr * x.y + wi * x.x;
+            }
+        } else if (grp.kind == HAMW_THREAD) {
+            // source index bits on this thread: (2 tid) ^ flip restricted to the thread bits
+            const uint32_t tsrc = ((2u * tid) ^ grp.flipl) & tmask;
+            T wr = 0, wi = 0;
+            for (int t = 0; t < grp.n_terms; t++) {
+                const uint32_t neg = (uint32_t(__popcll(base & tm[t].yz_nl)) + __popc(tsrc & tm[t].yzl)) & 1u;
+                wr += flip_sign(T(tm[t].coef_re), neg);
+                wi += flip_sign(T(tm[t].coef_im), neg);
+            }
+#pragma unroll
+            for (int J = 0; J < NA; J++) {
+                const V x = t_ket[a0 ^ swz<T>(jb(J) ^ grp.flipl)];
+                ar[J] += wr * x.x - wi * x.y;
+                ai[J] += wr * x.y + wi * x.x;
+            }
+        } else {
+            const uint32_t tsrc = ((2u * tid) ^ grp.flipl) & tmask;
+            // two halves of the owned amplitudes bound the weight registers
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                constexpr int NH = NA / 2;
+                T cr0 = 0, ci0 = 0;
+                T wr[NH], wi[NH];
+#pragma unroll
+                for (int J = 0; J < NH; J++) wr[J] = wi[J] = T(0);
+                for (int t = 0; t < grp.n_terms; t++) {
+                    const uint32_t yzl = tm[t].yzl;
+                    const uint32_t neg = (uint32_t(__popcll(base & tm[t].yz_nl)) + __popc(tsrc & yzl)) & 1u;
+                    const T cr = flip_sign(T(tm[t].coef_re), neg);
+                    const T ci = flip_sign(T(tm[t].coef_im), neg);
+                    const uint32_t jm = yzl & ~tmask;  // yz bits on register bits
+                    if (jm == 0) {
+                        cr0 += cr;
+                        ci0 += ci;
+                    } else {
+                        const uint32_t p0 = uint32_t(__popc(grp.flipl & jm)) & 1u;
+#pragma unroll
+                        for (int J = 0; J < NH; J++) {
+                            const uint32_t nj = p0 ^ (uint32_t(__popc(jb(half * NH + J) & jm)) & 1u);
+                            wr[J] += flip_sign(cr, nj);
+                            wi[J] += flip_sign(ci, nj);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int J = 0; J < NH; J++) {
+                    const int JJ = half * NH + J;
+                    const V x = t_ket[a0 ^ swz<T>(jb(JJ) ^ grp.flipl)];
+                    const T w_r = cr0 + wr[J], w_i = ci0 + wi[J];
+                    ar[JJ] += w_r * x.x - w_i * x.y;
+                    ai[JJ] += w_r * x.y + w_i * x.x;
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int P = 0; P < NA / 2; P++) {
+        const uint32_t u = (2u * tid) | (uint32_t(P) << JS);
+        pair_store<T>(g_out + ((u & cmask) | offtab[u >> C]), ar[2 * P], ai[2 * P], ar[2 * P + 1], ai[2 * P + 1]);
+    }
+    if constexpr (LAST) {
+        if (args.partial) {  // <ket|out> of this tile, summed in double
+            double fr = 0, fi = 0;
+#pragma unroll
+            for (int J = 0; J < NA; J++) {
+                const V x = t_ket[a0 ^ swz<T>(jb(J))];
+                fr += double(x.x * ar[J] + x.y * ai[J]);
+                fi += double(x.x * ai[J] - x.y * ar[J]);
+            }
+            block_sum2(fr, fi, args.partial + 2 * (sample * gridDim.x + tile));
+        }
     }
 }
 
@@ -942,6 +1122,42 @@ hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, h
         else hipLaunchKernelGGL((k_ham<double, false>), grid, dim3(256), 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <typename T, bool FIRST, bool LAST>
+static hipError_t launch_ht(const HamTileArgs& a, uint64_t n_tiles, uint64_t batch, size_t lds, hipStream_t s) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        // the block reduction's static LDS comes out of the same 160 KB
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ham_tile<T, FIRST, LAST>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           int(kLdsLimit - kBlockSumLds));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const uint32_t threads = 1u << (a.K - kGroupWidth);
+    if (threads > uint32_t(ham_tile_threads<T>())) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_ham_tile<T, FIRST, LAST>), dim3(uint32_t(n_tiles), uint32_t(batch)), dim3(threads), lds, s,
+                       a);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_ht_t(bool first, bool last, const HamTileArgs& a, uint64_t n_tiles, uint64_t batch,
+                              size_t lds, hipStream_t s) {
+    if (first) return last ? launch_ht<T, true, true>(a, n_tiles, batch, lds, s)
+                           : launch_ht<T, true, false>(a, n_tiles, batch, lds, s);
+    return last ? launch_ht<T, false, true>(a, n_tiles, batch, lds, s)
+                : launch_ht<T, false, false>(a, n_tiles, batch, lds, s);
+}
+
+hipError_t launch_ham_tile(int dtype, bool first, bool last, const HamTileArgs& a, int C, uint64_t n_tiles,
+                           uint64_t batch, hipStream_t s) {
+    const size_t amp = dtype == 0 ? 8 : 16;
+    const size_t lds = lds_layout(a.K, C, false, int(amp), 0).total;
+    if (a.K < kMinTileQubits || a.K > kMaxHamTileQubits || lds + kBlockSumLds > kLdsLimit) return hipErrorInvalidValue;
+    if (dtype == 0) return launch_ht_t<float>(first, last, a, n_tiles, batch, lds, s);
+    return launch_ht_t<double>(first, last, a, n_tiles, batch, lds, s);
 }
 
 hipError_t launch_vdot(int dtype, const VdotArgs& a, uint64_t batch, hipStream_t s) {

@@ -45,6 +45,20 @@ struct HamArgs {
     double* partial;  // [batch][n_blocks] (re, im) or NULL
 };
 
+// One pass of the Hamiltonian tile engine: out (+)= sum over the pass's flip groups of the
+// weighted ket gathered inside LDS tiles; the last pass also writes per-tile <ket|out> partials.
+struct HamTileArgs {
+    const void* ket;
+    void* out;
+    uint64_t stride;
+    int K;
+    int pass_index;
+    const DevHamPass* passes;
+    const DevHamTileGroup* groups;
+    const DevHamTileTerm* terms;
+    double* partial;  // [batch][n_tiles] (re, im) or NULL
+};
+
 struct VdotArgs {
     const void* a;
     uint64_t a_stride;
@@ -75,6 +89,8 @@ size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps);
 hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
                             int max_taps, hipStream_t s);
 hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s);
+hipError_t launch_ham_tile(int dtype, bool first, bool last, const HamTileArgs& a, int C, uint64_t n_tiles,
+                           uint64_t batch, hipStream_t s);
 hipError_t launch_vdot(int dtype, const VdotArgs& a, uint64_t batch, hipStream_t s);
 hipError_t launch_fill(int dtype, void* dst, const void* src, uint64_t dim, uint64_t stride, uint64_t src_dim,
                        uint64_t batch, hipStream_t s);

@@ -129,4 +129,37 @@ struct DevHamTerm {
     double pad;
 };
 
+// Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are
+// applied from an LDS-staged ket tile; the passes of a plan together cover every group.
+constexpr int kMaxHamTileQubits = 14;
+
+struct DevHamPass {          // 32 bytes
+    uint64_t nonlocal_mask;  // global qubits that index the tiles
+    uint64_t local_hi_mask;  // local qubits above the contiguous low run
+    int32_t group_begin;
+    int32_t n_groups;
+    int32_t low_run;         // C
+    int32_t pad;
+};
+// Weight kinds of a group inside a pass: the sign (-1)^{popcount(src & yz)} of each term splits
+// into a tile factor (yz bits outside the tile), a per-thread factor and a per-register factor.
+enum HamWeightKind : int32_t {
+    HAMW_TILE = 0,    // no term has yz bits inside the tile: one weight per tile
+    HAMW_THREAD = 1,  // yz bits inside the tile only on thread-index bits: one weight per thread
+    HAMW_FULL = 2,    // a weight per amplitude
+};
+struct DevHamTileGroup {     // 16 bytes
+    uint32_t flipl;          // flip mask in tile-index coordinates
+    int32_t term_begin;
+    int32_t n_terms;
+    int32_t kind;            // HamWeightKind
+};
+struct DevHamTileTerm {      // 32 bytes
+    uint64_t yz_nl;          // yz bits outside the tile (global index positions)
+    uint32_t yzl;            // yz bits inside the tile (tile-index coordinates)
+    int32_t pad;
+    double coef_re;          // c * i^{|y|}
+    double coef_im;
+};
+
 }  // namespace mqa

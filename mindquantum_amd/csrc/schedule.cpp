@@ -328,4 +328,91 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     return plan;
 }
 
+
+namespace {
+uint64_t pext64(uint64_t src, uint64_t mask) {
+    uint64_t out = 0;
+    int k = 0;
+    for (uint64_t m = mask; m; m &= m - 1, k++)
+        if (src & (m & (~m + 1))) out |= uint64_t(1) << k;
+    return out;
+}
+}  // namespace
+
+bool build_ham_plan(const std::vector<DevHamGroup>& groups, const std::vector<DevHamTerm>& terms, int n_eff, int K,
+                    int C, HamTilePlan& out) {
+    if (n_eff < kMinTileQubits) throw std::invalid_argument("state too small for the tile engine");
+    K = std::min(std::max(K, kMinTileQubits), std::min(n_eff, kMaxHamTileQubits));
+    C = std::min(std::max(C, 1), K - kGroupWidth);
+    if (n_eff <= K) C = K;
+    const int H = K - C;
+    const uint64_t all = (n_eff >= 64) ? ~uint64_t(0) : ((uint64_t(1) << n_eff) - 1);
+    const uint64_t low = (uint64_t(1) << C) - 1;
+
+    // first-fit decreasing packing of the high flip bits into passes of H local qubits
+    std::vector<int> order(groups.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return __builtin_popcountll(groups[a].flip & ~low) > __builtin_popcountll(groups[b].flip & ~low);
+    });
+    struct P {
+        uint64_t hi = 0;
+        std::vector<int> members;
+    };
+    std::vector<P> ps;
+    for (int gi : order) {
+        const uint64_t hi = groups[gi].flip & ~low;
+        if ((groups[gi].flip & ~all) != 0 || __builtin_popcountll(hi) > H) return false;
+        bool placed = false;
+        for (auto& p : ps) {
+            if (__builtin_popcountll(p.hi | hi) <= H) {
+                p.hi |= hi;
+                p.members.push_back(gi);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) ps.push_back(P{hi, {gi}});
+    }
+    if (ps.empty()) ps.push_back(P{});
+
+    HamTilePlan plan;
+    plan.K = K;
+    plan.C = C;
+    const uint32_t tmask = ((uint32_t(1) << (K - 3)) - 1u) & ~1u;  // thread-index bits of a tile index
+    for (auto& p : ps) {
+        // pad the high local set to exactly H qubits with the lowest unused ones
+        for (int q = C; q < n_eff && __builtin_popcountll(p.hi) < H; q++) p.hi |= uint64_t(1) << q;
+        DevHamPass dp{};
+        dp.local_hi_mask = p.hi;
+        dp.nonlocal_mask = all & ~low & ~p.hi;
+        dp.group_begin = int32_t(plan.groups.size());
+        dp.n_groups = int32_t(p.members.size());
+        dp.low_run = C;
+        std::sort(p.members.begin(), p.members.end());
+        for (int gi : p.members) {
+            const DevHamGroup& g = groups[gi];
+            DevHamTileGroup tg{};
+            tg.flipl = uint32_t((g.flip & low) | (pext64(g.flip, p.hi) << C));
+            tg.term_begin = int32_t(plan.terms.size());
+            tg.n_terms = g.n_terms;
+            tg.kind = HAMW_TILE;
+            for (int t = g.term_begin; t < g.term_begin + g.n_terms; t++) {
+                DevHamTileTerm tt{};
+                tt.yz_nl = terms[t].yz & dp.nonlocal_mask;
+                tt.yzl = uint32_t((terms[t].yz & low) | (pext64(terms[t].yz, p.hi) << C));
+                tt.coef_re = terms[t].coef_re;
+                tt.coef_im = terms[t].coef_im;
+                if (tt.yzl & ~tmask) tg.kind = HAMW_FULL;
+                else if (tt.yzl && tg.kind == HAMW_TILE) tg.kind = HAMW_THREAD;
+                plan.terms.push_back(tt);
+            }
+            plan.groups.push_back(tg);
+        }
+        plan.passes.push_back(dp);
+    }
+    out = std::move(plan);
+    return true;
+}
+
 }  // namespace mqa

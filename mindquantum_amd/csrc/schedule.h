@@ -79,4 +79,20 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
 // Matrix of a fixed (non-parameterised) gate kind; 2^m x 2^m row-major.
 std::vector<cplx> fixed_matrix(const CGate& g);
 
+// Passes of the Hamiltonian tile engine.  Tiles have K local qubits: the C lowest qubits and
+// K - C higher ones per pass.  Flip groups are packed first-fit (largest first) into passes
+// whose local qubits contain their flip mask.
+struct HamTilePlan {
+    int K = 0;
+    int C = 0;
+    std::vector<DevHamPass> passes;
+    std::vector<DevHamTileGroup> groups;
+    std::vector<DevHamTileTerm> terms;
+};
+
+// Returns false when a flip mask has more than K - C qubits above the low run (no tile holds
+// it); the caller then uses the gather kernel.
+bool build_ham_plan(const std::vector<DevHamGroup>& groups, const std::vector<DevHamTerm>& terms, int n_eff, int K,
+                    int C, HamTilePlan& out);
+
 }  // namespace mqa
