@@ -93,7 +93,7 @@ SchedOptions sched_options(int dtype, bool adjoint) {
     o.low_run = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
     o.group_width = int(g_opt.group_width);
     o.max_ops_per_pass = int(g_opt.max_ops);
-    o.taps_per_group = dtype == MQA_C64 ? 8 : 4;  // register budget of the adjoint pass
+    o.taps_per_group = 8;  // register budget of the adjoint pass (real tap accumulators)
     o.amp_bytes = dtype == MQA_C64 ? 8 : 16;
     return o;
 }
@@ -969,7 +969,7 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         const size_t sb = sim->state_bytes();
 
         // samples per chunk from the HBM budget: psi + lam (+ a kept copy for several hamiltonians)
-        const size_t per_sample = sb * (n_hams > 1 ? 3 : 2) + size_t(n_tap_ids) * n_tiles * 16 +
+        const size_t per_sample = sb * (n_hams > 1 ? 3 : 2) + size_t(n_tap_ids) * n_tiles * 8 +
                                   size_t(circ->n_slots) * 16 + size_t(n_tap_ids + n_params) * 16 + 4096;
         const size_t budget = hbm_budget();
         int64_t chunk = int64_t(std::max<size_t>(budget / per_sample, 1));
@@ -979,8 +979,8 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         sim->psi.ensure(sb * chunk);
         sim->lam.ensure(sb * chunk);
         if (n_hams > 1) sim->psi_keep.ensure(sb * chunk);
-        sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 16, 16));
-        sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 16, 16));
+        sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 8, 16));
+        sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 8, 16));
         sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
         std::vector<double> g_host(size_t(chunk) * n_params), f_host(size_t(chunk) * 2);
 
@@ -1003,8 +1003,8 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                 if (n_tap_ids > 0) {
                     run_plan(sim, adj, sim->psi.p, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(),
                              n_tap_ids);
-                    hip_check(launch_reduce_rows(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
-                                                 uint64_t(nb) * n_tap_ids, s),
+                    hip_check(launch_reduce_rows1(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
+                                                  uint64_t(nb) * n_tap_ids, s),
                               "reduce taps");
                 }
                 if (n_params > 0) {

@@ -11,7 +11,7 @@
 //                       132-161 ApplyTerms, 211-261 ExpectationOfTerms, one sweep per term).
 //  k_vdot<T>            partial sums of <a|b> (dot_like.cpp:32-52).
 //  k_fill<T>            broadcast an initial state (or |0>) into a batch of states.
-//  k_reduce_rows        deterministic double-precision row sums of partials.
+//  k_reduce_rows(1)     deterministic double-precision row sums of (complex / real) partials.
 //  k_jacobian           taps -> parameter gradients, g = sum coeff * 2 Re(tap) (gradient.h:
 //                       183-188 MatMul with the PR jacobian, :124-133 GradRealDouble).
 #include <hip/hip_runtime.h>
@@ -87,14 +87,14 @@ __device__ __forceinline__ T wave_sum64(T v) {
     return v;
 }
 
-// Per-thread derivative-tap accumulators of one register group.  The scheduler caps the taps
-// of a group at 8 (complex64) / 4 (complex128) so that the accumulators stay in registers.
+// Per-thread derivative-tap accumulators of one register group.  Only Re(tap) is kept: every
+// parameter gradient is sum coeff * 2 Re(tap) with real coefficients.  The scheduler caps the
+// taps of a group at 8 so that the accumulators stay in registers.
 template <typename T>
-constexpr int kTapSlots = sizeof(T) == 4 ? 8 : 4;
+constexpr int kTapSlots = 8;
 template <typename T>
 struct TapAcc {
     T r[kTapSlots<T>];
-    T i[kTapSlots<T>];
 };
 
 // Register state of one thread inside a group: 16 amplitudes of psi (and of lambda).
@@ -493,23 +493,23 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
 #undef XXT
 #undef YYT
 #undef ZZT
-            // tap = factor * s
-            T tr, ti;
+            // Re(tap), tap = factor * s; only one component of s survives, so the compiler drops
+            // the other half of the accumulation
+            T tr;
             if (kind == OPK_PS) {  // i s
-                tr = -si, ti = sr;
+                tr = -si;
             } else if (kind == OPK_GP) {  // -i s
-                tr = si, ti = -sr;
+                tr = si;
             } else if (kind == OPK_RY) {  // -i/2 * (i s)
-                tr = T(0.5) * sr, ti = T(0.5) * si;
+                tr = T(0.5) * sr;
             } else {  // -i/2 * s
-                tr = T(0.5) * si, ti = T(-0.5) * sr;
+                tr = T(0.5) * si;
             }
             // the group's taps arrive in order: shift them through the register file so that
             // every access uses a constant index (no dynamically indexed, scratch-backed array)
 #pragma unroll
-            for (int j = kTapSlots<T> - 1; j > 0; j--) acc.r[j] = acc.r[j - 1], acc.i[j] = acc.i[j - 1];
+            for (int j = kTapSlots<T> - 1; j > 0; j--) acc.r[j] = acc.r[j - 1];
             acc.r[0] = tr;
-            acc.i[0] = ti;
         }
     }
     apply_op<T, FULL>(kind, op, p, g, cl, c, s, m);
@@ -648,7 +648,7 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
         TapAcc<T> acc;
         if constexpr (ADJ) {
 #pragma unroll
-            for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0), acc.i[j] = T(0);
+            for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0);
         }
         const DevOp op0 = args.ops[grp.op_begin];
         if (FULL && (op0.kind == OPK_M3 || op0.kind == OPK_M4)) {  // a dense block alone in its group
@@ -674,25 +674,18 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
                 if (wave_width == 64) {
 #pragma unroll
                     for (int j = 0; j < kTapSlots<T>; j++) {
-                        if (j < 4 || grp.n_taps > 4) {
-                            acc.r[j] = wave_sum64(acc.r[j]);
-                            acc.i[j] = wave_sum64(acc.i[j]);
-                        }
+                        if (j < 4 || grp.n_taps > 4) acc.r[j] = wave_sum64(acc.r[j]);
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < kTapSlots<T>; j++) {
-                        acc.r[j] = wave_sum(acc.r[j], wave_width);
-                        acc.i[j] = wave_sum(acc.i[j], wave_width);
-                    }
+                    for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = wave_sum(acc.r[j], wave_width);
                 }
                 if (lane0) {
 #pragma unroll
                     for (int j = 0; j < kTapSlots<T>; j++) {
                         if (j < grp.n_taps) {  // acc[j] holds the group's (n_taps - 1 - j)-th tap
                             const int t = grp.tap_begin + grp.n_taps - 1 - j;
-                            lds_taps[2 * (t * n_waves + wave)] = double(acc.r[j]);
-                            lds_taps[2 * (t * n_waves + wave) + 1] = double(acc.i[j]);
+                            lds_taps[t * n_waves + wave] = double(acc.r[j]);
                         }
                     }
                 }
@@ -717,15 +710,10 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
     if constexpr (ADJ) {
         // one partial per (sample, tap id, tile); summed over tiles by k_reduce_rows
         for (int t = int(tid); t < pass.n_taps; t += int(NT)) {
-            double sr = 0, si = 0;
-            for (int w = 0; w < n_waves; w++) {
-                sr += lds_taps[2 * (t * n_waves + w)];
-                si += lds_taps[2 * (t * n_waves + w) + 1];
-            }
+            double sr = 0;
+            for (int w = 0; w < n_waves; w++) sr += lds_taps[t * n_waves + w];
             const int tap_id = args.tap_ids[pass.tap_begin + t];
-            uint64_t idx = ((sample * uint64_t(args.n_tap_ids) + uint64_t(tap_id)) * args.n_tiles + tile);
-            args.partials[2 * idx] = sr;
-            args.partials[2 * idx + 1] = si;
+            args.partials[(sample * uint64_t(args.n_tap_ids) + uint64_t(tap_id)) * args.n_tiles + tile] = sr;
         }
     }
 }
@@ -1060,7 +1048,18 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
     if (lane == 0) out[2 * row] = a, out[2 * row + 1] = b;
 }
 
-// g[b][p] = sum_{e in param p} coeff_e * 2 Re(tap[b][tap_e])
+__global__ void __launch_bounds__(256) k_reduce_rows1(const double* __restrict__ in, uint64_t n,
+                                                     double* __restrict__ out, uint64_t rows) {
+    const uint64_t row = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    const int lane = threadIdx.x & 63;
+    double a = 0;
+    for (uint64_t k = lane; k < n; k += 64) a += in[row * n + k];
+    a = wave_sum(a, 64);
+    if (lane == 0) out[row] = a;
+}
+
+// g[b][p] = sum_{e in param p} coeff_e * 2 Re(tap[b][tap_e]); taps hold Re(tap) only
 __global__ void __launch_bounds__(256) k_jacobian(const double* __restrict__ taps, int n_tap_ids,
                                                  const int32_t* __restrict__ jac_begin,
                                                  const int32_t* __restrict__ jac_tap,
@@ -1072,7 +1071,7 @@ __global__ void __launch_bounds__(256) k_jacobian(const double* __restrict__ tap
     const int p = int(e - b * n_params);
     double acc = 0;
     for (int k = jac_begin[p]; k < jac_begin[p + 1]; k++)
-        acc += jac_coeff[k] * 2.0 * taps[2 * (b * n_tap_ids + jac_tap[k])];
+        acc += jac_coeff[k] * 2.0 * taps[b * n_tap_ids + jac_tap[k]];
     g[e] = acc;
 }
 
@@ -1201,6 +1200,12 @@ hipError_t launch_angles(const AngleArgs& a, hipStream_t s) {
 hipError_t launch_reduce_rows(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s) {
     uint32_t blocks = uint32_t((rows + 3) / 4);
     hipLaunchKernelGGL(k_reduce_rows, dim3(blocks), dim3(256), 0, s, in, n, out, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows1(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s) {
+    uint32_t blocks = uint32_t((rows + 3) / 4);
+    hipLaunchKernelGGL(k_reduce_rows1, dim3(blocks), dim3(256), 0, s, in, n, out, rows);
     return hipGetLastError();
 }
 

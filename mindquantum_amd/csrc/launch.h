@@ -97,6 +97,7 @@ hipError_t launch_fill(int dtype, void* dst, const void* src, uint64_t dim, uint
 hipError_t launch_cast(int src_dtype, const void* src, void* dst, uint64_t n_reals, hipStream_t s);
 hipError_t launch_angles(const AngleArgs& a, hipStream_t s);
 hipError_t launch_reduce_rows(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
+hipError_t launch_reduce_rows1(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
 hipError_t launch_jacobian(const double* taps, int n_tap_ids, const int32_t* jac_begin, const int32_t* jac_tap,
                            const double* jac_coeff, int n_params, double* g, uint64_t batch, hipStream_t s);
 

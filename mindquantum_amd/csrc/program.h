@@ -66,7 +66,7 @@ inline LdsLayout lds_layout(int K, int C, bool adj, int amp_bytes, int max_taps)
     l.lam = tile;
     l.offtab = tile * (adj ? 2u : 1u);
     l.taps = (l.offtab + (uint32_t(1) << (K - C)) * 8u + 15u) & ~15u;
-    l.total = l.taps + uint32_t(max_taps) * n_waves * 16u;
+    l.total = l.taps + uint32_t(max_taps) * n_waves * 8u;  // one double (Re tap) per tap and wave
     return l;
 }
 
