@@ -125,6 +125,13 @@ int mqa_circuit_destroy(mqa_circuit* circ);
  * adjoint != 0 exports the reverse (back-sweep) schedule.  *len gets the words needed. */
 int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint, int64_t* out,
                          int64_t cap, int64_t* len);
+/* Generate and compile (hipRTC, host only, no GPU needed) the per-circuit kernels of the
+ * forward (adjoint == 0) or back-sweep schedule, as the library does in the background for
+ * large launches (option "jit").  *seconds gets the compile wall time.  No counterpart in the
+ * reference: a build check of the code generator for tests and for callers that want to pay
+ * the compile cost up front. */
+int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint,
+                            double* seconds);
 
 /* ---- Hamiltonian (reference: ccsrc/include/ops/hamiltonian.h:30-55, how_to_ == ORIGIN) ---- */
 int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out);

@@ -1,8 +1,10 @@
 """Build the HIP library ``libmqamd.so`` in-tree for gfx950 (hipcc, no cmake).
 
 ``python -m mindquantum_amd._build`` or ``__graft_entry__.build()`` runs it; every call
-recompiles.  Sources: ``csrc/kernels.hip`` (device code), ``csrc/capi.cpp`` and
-``csrc/schedule.cpp`` (host code).
+recompiles.  Sources: ``csrc/kernels.hip`` (device code), ``csrc/capi.cpp``,
+``csrc/schedule.cpp`` and ``csrc/jit.cpp`` (host code).  The text of ``csrc/program.h`` and
+``csrc/device_ops.h`` is embedded into the library (``_build/jit_headers.inc``) for the
+per-circuit kernels that ``jit.cpp`` compiles with hipRTC at run time.
 """
 import os
 import subprocess
@@ -13,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmqamd.so")
+JITC = os.path.join(HERE, "mqamd_jitc")
 ARCH = os.environ.get("MQAMD_ARCH", "gfx950")
 
 COMMON = ["-std=c++20", "-O3", "-fPIC", "-Wall", "-Wno-unused-function"]
@@ -33,15 +36,38 @@ def _run(cmd):
     return out
 
 
+JIT_HEADERS = ["program.h", "device_ops.h"]
+
+
+def _write_jit_headers():
+    """Emit the headers the generated kernels include, as C++ raw string literals."""
+    names, bodies = [], []
+    for h in JIT_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            text = f.read()
+        assert ")MQAJIT\"" not in text
+        names.append('"%s"' % h)
+        bodies.append('R"MQAJIT(' + text + ')MQAJIT"')
+    out = ("// generated by mindquantum_amd/_build.py from csrc/" + ", csrc/".join(JIT_HEADERS) + "\n"
+           "static const char* const kJitHeaderNames[] = {" + ", ".join(names) + "};\n"
+           "static const char* const kJitHeaders[] = {\n" + ",\n".join(bodies) + "};\n")
+    path = os.path.join(BUILD, "jit_headers.inc")
+    with open(path, "w") as f:
+        f.write(out)
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
+    _write_jit_headers()
     hipcc = _hipcc()
     jobs = [
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "kernels.hip"), "-o", os.path.join(BUILD, "kernels.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "capi.cpp"), "-o", os.path.join(BUILD, "capi.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "schedule.cpp"), "-o", os.path.join(BUILD, "schedule.o")],
+        [hipcc, *COMMON, *DEVICE, "-I", BUILD, "-c", os.path.join(CSRC, "jit.cpp"), "-o", os.path.join(BUILD, "jit.o")],
+        [hipcc, *COMMON, "-c", os.path.join(CSRC, "jitc_main.cpp"), "-o", os.path.join(BUILD, "jitc_main.o")],
     ]
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    with ThreadPoolExecutor(max_workers=5) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:
         for o in outs:
@@ -49,8 +75,13 @@ def build(verbose=False):
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, *DEVICE, "-shared", "-fPIC", "-o", tmp,
-          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "capi.o"), os.path.join(BUILD, "schedule.o")])
+          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "capi.o"), os.path.join(BUILD, "schedule.o"),
+          os.path.join(BUILD, "jit.o"), "-lhiprtc"])
     os.replace(tmp, LIB)
+    # compile helper of the code generator: hipRTC runs one compile at a time per process
+    tmp = JITC + ".tmp"
+    _run([hipcc, "-o", tmp, os.path.join(BUILD, "jitc_main.o"), os.path.join(BUILD, "jit.o"), "-lhiprtc"])
+    os.replace(tmp, JITC)
     return LIB
 
 

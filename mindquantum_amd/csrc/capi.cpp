@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/mqamd.h"
+#include "jit.h"
 #include "launch.h"
 #include "program.h"
 #include "schedule.h"
@@ -68,6 +69,9 @@ struct Options {
     std::atomic<int64_t> budget{0};  // bytes of HBM a batched call may use (0: 85% of free memory)
     std::atomic<int64_t> profile{0};  // 1: time kernels with HIP events
     std::atomic<int64_t> ham_tile_c64{14}, ham_tile_c128{13}, ham_engine{1};  // ham_engine 0: gather kernel only
+    // per-circuit compiled passes: 0 off, 1 compile in the background and switch over when
+    // ready, 2 compile before the first launch; only for launches of >= jit_min_amps amplitudes
+    std::atomic<int64_t> jit{1}, jit_min_amps{int64_t(1) << 24};
 };
 Options g_opt;
 
@@ -83,7 +87,8 @@ std::vector<OptEntry> opt_table() {
             {"lowq_c128", &g_opt.lowq_c128, 4},           {"max_ops_per_pass", &g_opt.max_ops, 0},
             {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0},
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
-            {"ham_tile_engine", &g_opt.ham_engine, 1}};
+            {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
+            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -142,6 +147,8 @@ struct DevPlan {
     Plan plan;
     DevBuf passes, groups, ops, pool, tap_ids;
     bool full = false;
+    std::mutex jit_mu;
+    std::unique_ptr<JitPlan> jit;  // compiled passes (created on first large launch)
 };
 
 // ---------------------------------------------------------------- kernel timing
@@ -378,6 +385,9 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
     sim->timer.pending.push_back({a, b, name, bytes});
 }
 
+std::atomic<int64_t> g_jit_launches{0};
+void count_jit_launch() { g_jit_launches.fetch_add(1); }
+
 void sync(mqa_sim* sim, const char* what) {
     hip_check(hipStreamSynchronize(sim->stream), what);
     sim->timer.resolve();
@@ -403,13 +413,26 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     a.tap_ids = dp->tap_ids.as<int32_t>();
     a.partials = partials;
     const double bytes = double(batch) * double(sim->state_bytes()) * (pl.adjoint ? 4.0 : 2.0);
+    JitPlan* jp = nullptr;
+    if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
+        std::lock_guard<std::mutex> lk(dp->jit_mu);
+        if (!dp->jit) dp->jit = std::make_unique<JitPlan>(pl, sim->dtype);
+        if (g_opt.jit == 2) dp->jit->wait();
+        if (dp->jit->ready()) jp = dp->jit.get();  // until then (or if it failed): the interpreted kernel
+    }
+    const size_t lds = tile_pass_lds_bytes(sim->dtype, pl.adjoint, pl.K, pl.C, std::max(pl.max_taps, 1));
     for (size_t i = 0; i < pl.passes.size(); i++) {
         a.pass_index = int(i);
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
-            hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C, std::max(pl.max_taps, 1),
-                                       sim->stream),
-                      "tile pass");
+            if (jp) {
+                hip_check(jp->launch(int(i), a, batch, lds, sim->stream), "compiled tile pass");
+            } else {
+                hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C,
+                                           std::max(pl.max_taps, 1), sim->stream),
+                          "tile pass");
+            }
         });
+        if (jp) count_jit_launch();
     }
 }
 
@@ -588,6 +611,10 @@ int mqa_set_option(const char* key, int64_t value) {
 int mqa_get_option(const char* key, int64_t* value) {
     return guarded([&] {
         require(key != nullptr && value != nullptr, "NULL argument");
+        if (std::strcmp(key, "jit_launches") == 0) {  // read-only: passes launched from compiled kernels
+            *value = g_jit_launches.load();
+            return;
+        }
         for (auto& e : opt_table()) {
             if (std::strcmp(e.key, key) == 0) {
                 *value = *e.v;
@@ -722,6 +749,20 @@ int mqa_circuit_schedule(const mqa_circuit* circ_c, int32_t n_qubits, int32_t dt
         }
         *len = int64_t(w.size());
         if (out && cap >= int64_t(w.size())) std::memcpy(out, w.data(), w.size() * sizeof(int64_t));
+    });
+}
+
+int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint,
+                            double* seconds) {
+    return guarded([&] {
+        require(circ != nullptr, "circuit is NULL");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
+        require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
+        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adjoint != 0, sched_options(dtype, adjoint != 0));
+        JitPlan jp(pl, dtype);
+        jp.wait();
+        if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());
+        if (seconds) *seconds = jp.compile_seconds();
     });
 }
 

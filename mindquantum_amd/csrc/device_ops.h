@@ -1,0 +1,657 @@
+// device_ops.h - device-side building blocks of the tile passes: LDS swizzle helpers, wave
+// reductions, the register-group op bodies, HBM <-> LDS tile transfer and the pass frame.
+// Included by kernels.hip and, as source text, by the per-circuit kernels generated at run
+// time (jit.cpp), so it depends on nothing but program.h.
+#pragma once
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+#include <cstdint>
+
+#include "program.h"
+
+namespace mqa {
+
+template <typename T>
+struct V2t;
+template <>
+struct V2t<float> {
+    using type = float2;
+};
+template <>
+struct V2t<double> {
+    using type = double2;
+};
+template <typename T>
+using V2 = typename V2t<T>::type;
+
+// ---------------------------------------------------------------- small helpers
+__device__ __forceinline__ uint64_t pdep64(uint64_t src, uint64_t mask) {
+    uint64_t out = 0;
+    while (mask) {
+        uint64_t low = mask & (~mask + 1);
+        if (src & 1) out |= low;
+        src >>= 1;
+        mask &= mask - 1;
+    }
+    return out;
+}
+
+// LDS address swizzle: XOR-folds higher index bits into the low bits so that register
+// groups whose positions include low qubits do not hit one bank with every lane.
+template <typename T>
+__device__ __forceinline__ uint32_t swz(uint32_t u) {
+    return lds_swizzle(u, int(sizeof(T) * 2));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v, int width) {
+    for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Wave-wide sum of a full 64-lane wave: DPP within rows of 16 (VALU, no LDS round trip),
+// then two cross-row exchanges.  Every lane ends with the total.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum64(T v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Per-thread derivative-tap accumulators of one register group.  Only Re(tap) is kept: every
+// parameter gradient is sum coeff * 2 Re(tap) with real coefficients.  The scheduler caps the
+// taps of a group at 8 so that the accumulators stay in registers.
+template <typename T>
+constexpr int kTapSlots = 8;
+template <typename T>
+struct TapAcc {
+    T r[kTapSlots<T>];
+};
+
+// Register state of one thread inside a group: 16 amplitudes of psi (and of lambda).
+template <typename T>
+struct Regs {
+    T r[kAmpsPerThread];
+    T i[kAmpsPerThread];
+};
+
+// Context shared by the op bodies.
+struct GroupCtx {
+    uint32_t ub;     // local index of register 0 (group positions cleared)
+    uint32_t pm[4];  // 1 << pos[i]
+};
+
+template <int V>
+__device__ __forceinline__ uint32_t pv(const GroupCtx& g) {
+    return ((V & 1) ? g.pm[0] : 0u) | ((V & 2) ? g.pm[1] : 0u) | ((V & 4) ? g.pm[2] : 0u) | ((V & 8) ? g.pm[3] : 0u);
+}
+
+template <int V>
+__device__ __forceinline__ bool ctrl_ok(const GroupCtx& g, uint32_t cl) {
+    return ((g.ub | pv<V>(g)) & cl) == cl;
+}
+
+// ---------------------------------------------------------------- op bodies
+// Compile-time loops over the 8 pairs (one target) or 4 quads (two targets) of a register set.
+template <int Q, int V = 0, typename F>
+__device__ __forceinline__ void for_pairs(F&& f) {
+    if constexpr (V < kAmpsPerThread) {
+        if constexpr ((V & (1 << Q)) == 0) f.template operator()<V, (V | (1 << Q))>();
+        for_pairs<Q, V + 1>(f);
+    }
+}
+template <int Q0, int Q1, int V = 0, typename F>
+__device__ __forceinline__ void for_quads(F&& f) {
+    if constexpr (V < kAmpsPerThread) {
+        if constexpr ((V & ((1 << Q0) | (1 << Q1))) == 0)
+            f.template operator()<V, (V | (1 << Q0)), (V | (1 << Q1)), (V | (1 << Q0) | (1 << Q1))>();
+        for_quads<Q0, Q1, V + 1>(f);
+    }
+}
+
+// a' = m a on one pair, m a row-major 2x2 complex matrix (uniform: lives in SGPRs)
+template <typename T>
+__device__ __forceinline__ void mat2(const T* __restrict__ m, T& ar, T& ai, T& br, T& bi) {
+    T xr = m[0] * ar - m[1] * ai + m[2] * br - m[3] * bi;
+    T xi = m[0] * ai + m[1] * ar + m[2] * bi + m[3] * br;
+    T yr = m[4] * ar - m[5] * ai + m[6] * br - m[7] * bi;
+    T yi = m[4] * ai + m[5] * ar + m[6] * bi + m[7] * br;
+    ar = xr, ai = xi, br = yr, bi = yi;
+}
+
+// conj(x) * y accumulated as four FMAs (a dependent chain costs nothing extra: a wave64
+// VALU op occupies its SIMD for four cycles, which covers the FMA latency)
+template <typename T>
+__device__ __forceinline__ void cdot_acc(T xr, T xi, T yr, T yi, T& sr, T& si) {
+    sr = fma(xr, yr, sr);
+    sr = fma(xi, yi, sr);
+    si = fma(xr, yi, si);
+    si = fma(-xi, yr, si);
+}
+
+// One-target op on a register set.  c, s: cos / sin of the half angle (RX, RY, RZ) or of the
+// angle (PS, GP); m: dense 2x2 (M1) in the matrix pool.  CTRL: some control qubits are local;
+// their predicate is evaluated per pair and applied with selects (no divergent branches).
+template <typename T, int KIND, int Q, bool CTRL>
+__device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32_t cl, T c, T s,
+                                           const T* __restrict__ m) {
+    for_pairs<Q>([&]<int V, int W>() {
+        T ar = a.r[V], ai = a.i[V], br = a.r[W], bi = a.i[W];
+        T xr, xi, yr, yi;
+        if constexpr (KIND == OPK_X) {
+            xr = br, xi = bi, yr = ar, yi = ai;
+        } else if constexpr (KIND == OPK_RX) {  // [[c, -is], [-is, c]]
+            xr = c * ar + s * bi, xi = c * ai - s * br;
+            yr = c * br + s * ai, yi = c * bi - s * ar;
+        } else if constexpr (KIND == OPK_RY) {  // [[c, -s], [s, c]]
+            xr = c * ar - s * br, xi = c * ai - s * bi;
+            yr = s * ar + c * br, yi = s * ai + c * bi;
+        } else if constexpr (KIND == OPK_RZ) {  // diag(c - is, c + is)
+            xr = c * ar + s * ai, xi = c * ai - s * ar;
+            yr = c * br - s * bi, yi = c * bi + s * br;
+        } else if constexpr (KIND == OPK_PS) {  // diag(1, c + is)
+            xr = ar, xi = ai;
+            yr = c * br - s * bi, yi = c * bi + s * br;
+        } else if constexpr (KIND == OPK_GP) {  // (c - is) I
+            xr = c * ar + s * ai, xi = c * ai - s * ar;
+            yr = c * br + s * bi, yi = c * bi - s * br;
+        } else {  // dense 2x2
+            xr = ar, xi = ai, yr = br, yi = bi;
+            mat2(m, xr, xi, yr, yi);
+        }
+        if constexpr (CTRL) {
+            const bool ok = ctrl_ok<V>(g, cl);
+            xr = ok ? xr : ar, xi = ok ? xi : ai, yr = ok ? yr : br, yi = ok ? yi : bi;
+        }
+        a.r[V] = xr, a.i[V] = xi, a.r[W] = yr, a.i[W] = yi;
+    });
+}
+
+// Controlled X / Z whose (first) control is register position QC of the group: the control
+// bit is known per register at compile time.  cl: further local controls (predicated).
+template <typename T, bool ISX, int QT, int QC, bool CTRL>
+__device__ __forceinline__ void controlled_xz(Regs<T>& a, const GroupCtx& g, uint32_t cl) {
+    for_pairs<QT>([&]<int V, int W>() {
+        if constexpr (((V >> QC) & 1) != 0) {
+            T ar = a.r[V], ai = a.i[V], br = a.r[W], bi = a.i[W];
+            T xr, xi, yr, yi;
+            if constexpr (ISX) {
+                xr = br, xi = bi, yr = ar, yi = ai;
+            } else {
+                xr = ar, xi = ai, yr = -br, yi = -bi;
+            }
+            if constexpr (CTRL) {
+                const bool ok = ctrl_ok<V>(g, cl);
+                xr = ok ? xr : ar, xi = ok ? xi : ai, yr = ok ? yr : br, yi = ok ? yi : bi;
+            }
+            a.r[V] = xr, a.i[V] = xi, a.r[W] = yr, a.i[W] = yi;
+        }
+    });
+}
+
+// Re(tap) of a parameterised op needs one real component of a complex sum: Re(conj(x) y) or
+// Im(conj(x) y).  Accumulates that component only.
+template <bool IM, typename T>
+__device__ __forceinline__ void cdot_part(T xr, T xi, T yr, T yi, T& acc) {
+    if constexpr (IM) {
+        acc = fma(xr, yi, acc);
+        acc = fma(-xi, yr, acc);
+    } else {
+        acc = fma(xr, yr, acc);
+        acc = fma(xi, yi, acc);
+    }
+}
+
+// Which component of the raw sum s gives Re(tap), and the factor: tap = factor * s with
+//   RX, RZ, RXX, RYY, RZZ: s = <l|P|p>, tap = -i/2 s  -> Re(tap) =  Im(s) / 2
+//   RY: s = <l|Y|p> / i, tap = -i/2 (i s)            -> Re(tap) =  Re(s) / 2
+//   PS: s = sum conj(l_b) p_b, tap = i s              -> Re(tap) = -Im(s)
+//   GP: s = <l|p>, tap = -i s                         -> Re(tap) =  Im(s)
+template <int KIND>
+constexpr bool kTapIm = KIND != OPK_RY;
+template <typename T, int KIND>
+__device__ __forceinline__ T tap_factor() {
+    return KIND == OPK_PS ? T(-1) : (KIND == OPK_GP ? T(1) : T(0.5));
+}
+
+// Re(tap) of a one-target parameterised op, taken before G^dagger is applied.
+template <typename T, int KIND, int Q, bool CTRL>
+__device__ __forceinline__ T one_target_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl) {
+    constexpr bool IM = kTapIm<KIND>;
+    T s = 0;
+    for_pairs<Q>([&]<int V, int W>() {
+        T u = 0;
+        T& x = CTRL ? u : s;
+        if constexpr (KIND == OPK_RX) {
+            cdot_part<IM>(l.r[V], l.i[V], p.r[W], p.i[W], x);
+            cdot_part<IM>(l.r[W], l.i[W], p.r[V], p.i[V], x);
+        } else if constexpr (KIND == OPK_RY) {  // <l|Y|p> = i (conj(l_b) p_a - conj(l_a) p_b)
+            cdot_part<IM>(l.r[W], l.i[W], p.r[V], p.i[V], x);
+            cdot_part<IM>(l.r[V], l.i[V], -p.r[W], -p.i[W], x);
+        } else if constexpr (KIND == OPK_RZ) {
+            cdot_part<IM>(l.r[V], l.i[V], p.r[V], p.i[V], x);
+            cdot_part<IM>(l.r[W], l.i[W], -p.r[W], -p.i[W], x);
+        } else if constexpr (KIND == OPK_PS) {
+            cdot_part<IM>(l.r[W], l.i[W], p.r[W], p.i[W], x);
+        } else {  // GP
+            cdot_part<IM>(l.r[V], l.i[V], p.r[V], p.i[V], x);
+            cdot_part<IM>(l.r[W], l.i[W], p.r[W], p.i[W], x);
+        }
+        if constexpr (CTRL) s += ctrl_ok<V>(g, cl) ? u : T(0);
+    });
+    return tap_factor<T, KIND>() * s;
+}
+
+// Two-target ops; quad order (00, 01, 10, 11) with bit 0 <=> Q0 (matrix bit 0).
+template <typename T, int KIND, int Q0, int Q1, bool CTRL>
+__device__ __forceinline__ void two_target(Regs<T>& a, const GroupCtx& g, uint32_t cl, T c, T s,
+                                           const T* __restrict__ m) {
+    for_quads<Q0, Q1>([&]<int V0, int V1, int V2_, int V3>() {
+        if constexpr (CTRL) {
+            if (!ctrl_ok<V0>(g, cl)) return;
+        }
+        if constexpr (KIND == OPK_SWAP) {
+            T tr = a.r[V1], ti = a.i[V1];
+            a.r[V1] = a.r[V2_], a.i[V1] = a.i[V2_];
+            a.r[V2_] = tr, a.i[V2_] = ti;
+        } else if constexpr (KIND == OPK_RZZ) {  // diag(e-, e+, e+, e-), e-+ = c -+ is
+            T r0 = c * a.r[V0] + s * a.i[V0], i0 = c * a.i[V0] - s * a.r[V0];
+            T r1 = c * a.r[V1] - s * a.i[V1], i1 = c * a.i[V1] + s * a.r[V1];
+            T r2 = c * a.r[V2_] - s * a.i[V2_], i2 = c * a.i[V2_] + s * a.r[V2_];
+            T r3 = c * a.r[V3] + s * a.i[V3], i3 = c * a.i[V3] - s * a.r[V3];
+            a.r[V0] = r0, a.i[V0] = i0, a.r[V1] = r1, a.i[V1] = i1;
+            a.r[V2_] = r2, a.i[V2_] = i2, a.r[V3] = r3, a.i[V3] = i3;
+        } else if constexpr (KIND == OPK_RXX || KIND == OPK_RYY) {
+            // RXX = c I - is XX; RYY = c I - is YY with YY|00> = -|11>, YY|01> = |10>
+            constexpr int sg = (KIND == OPK_RXX) ? 1 : -1;
+            T ss = s * T(sg);
+            T r0 = c * a.r[V0] + ss * a.i[V3], i0 = c * a.i[V0] - ss * a.r[V3];
+            T r3 = c * a.r[V3] + ss * a.i[V0], i3 = c * a.i[V3] - ss * a.r[V0];
+            T r1 = c * a.r[V1] + s * a.i[V2_], i1 = c * a.i[V1] - s * a.r[V2_];
+            T r2 = c * a.r[V2_] + s * a.i[V1], i2 = c * a.i[V2_] - s * a.r[V1];
+            a.r[V0] = r0, a.i[V0] = i0, a.r[V1] = r1, a.i[V1] = i1;
+            a.r[V2_] = r2, a.i[V2_] = i2, a.r[V3] = r3, a.i[V3] = i3;
+        } else {  // dense 4x4
+            T xr[4] = {a.r[V0], a.r[V1], a.r[V2_], a.r[V3]};
+            T xi[4] = {a.i[V0], a.i[V1], a.i[V2_], a.i[V3]};
+            T yr[4], yi[4];
+#pragma unroll
+            for (int row = 0; row < 4; row++) {
+                T ur = 0, ui = 0;
+#pragma unroll
+                for (int col = 0; col < 4; col++) {
+                    const T mr = m[2 * (row * 4 + col)], mi = m[2 * (row * 4 + col) + 1];
+                    ur += mr * xr[col] - mi * xi[col];
+                    ui += mr * xi[col] + mi * xr[col];
+                }
+                yr[row] = ur, yi[row] = ui;
+            }
+            a.r[V0] = yr[0], a.i[V0] = yi[0], a.r[V1] = yr[1], a.i[V1] = yi[1];
+            a.r[V2_] = yr[2], a.i[V2_] = yi[2], a.r[V3] = yr[3], a.i[V3] = yi[3];
+        }
+    });
+}
+
+// Re(tap) of RXX / RYY / RZZ: tap = -i/2 s, s = <l|PP|p>
+template <typename T, int KIND, int Q0, int Q1, bool CTRL>
+__device__ __forceinline__ T two_target_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl) {
+    T s = 0;
+    for_quads<Q0, Q1>([&]<int V0, int V1, int V2_, int V3>() {
+        if constexpr (CTRL) {
+            if (!ctrl_ok<V0>(g, cl)) return;
+        }
+        if constexpr (KIND == OPK_RZZ) {
+            cdot_part<true>(l.r[V0], l.i[V0], p.r[V0], p.i[V0], s);
+            cdot_part<true>(l.r[V1], l.i[V1], -p.r[V1], -p.i[V1], s);
+            cdot_part<true>(l.r[V2_], l.i[V2_], -p.r[V2_], -p.i[V2_], s);
+            cdot_part<true>(l.r[V3], l.i[V3], p.r[V3], p.i[V3], s);
+        } else {
+            constexpr int sg = (KIND == OPK_RXX) ? 1 : -1;
+            cdot_part<true>(l.r[V0], l.i[V0], T(sg) * p.r[V3], T(sg) * p.i[V3], s);
+            cdot_part<true>(l.r[V3], l.i[V3], T(sg) * p.r[V0], T(sg) * p.i[V0], s);
+            cdot_part<true>(l.r[V1], l.i[V1], p.r[V2_], p.i[V2_], s);
+            cdot_part<true>(l.r[V2_], l.i[V2_], p.r[V1], p.i[V1], s);
+        }
+    });
+    return T(0.5) * s;
+}
+
+// Dense 2^M x 2^M (M = 3, 4) over register bits 0..M-1 (group positions in the gate's order).
+// Such a gate forms a group of its own.  The inputs are re-read from the LDS tile (which
+// still holds them) inside a rolled column loop, so no second register set is needed.
+template <typename T, int M>
+__device__ __forceinline__ void dense_from_lds(Regs<T>& a, const GroupCtx& g, uint32_t cl, const T* __restrict__ mat,
+                                               const V2<T>* __restrict__ tile) {
+    constexpr int D = 1 << M;
+#pragma unroll
+    for (int hi = 0; hi < kAmpsPerThread / D; hi++) {
+        // controls are never target positions; the pad bit (M == 3) is register bit 3
+        const uint32_t u0 = g.ub | ((M == 3 && hi) ? g.pm[3] : 0u);
+        if (cl && (u0 & cl) != cl) continue;
+#pragma unroll
+        for (int row = 0; row < D; row++) {
+            T yr = 0, yi = 0;
+#pragma unroll 1
+            for (int col = 0; col < D; col++) {
+                const uint32_t u = u0 | ((col & 1) ? g.pm[0] : 0u) | ((col & 2) ? g.pm[1] : 0u) |
+                                   ((col & 4) ? g.pm[2] : 0u) | ((M == 4 && (col & 8)) ? g.pm[3] : 0u);
+                const V2<T> x = tile[swz<T>(u)];
+                const T mr = mat[2 * (row * D + col)], mi = mat[2 * (row * D + col) + 1];
+                yr += mr * x.x - mi * x.y;
+                yi += mr * x.y + mi * x.x;
+            }
+            a.r[hi * D + row] = yr, a.i[hi * D + row] = yi;
+        }
+    }
+}
+
+// Calls f.template operator()<KIND, Q0, Q1>() for an op's kind and register positions: one
+// switch per op, so the tap and both register sets are updated inside a single case body (one
+// control-flow merge per op, which keeps the register allocator from copying the state).
+template <bool FULL, typename F>
+__device__ __forceinline__ void dispatch_op(int kind, int q0, int q1, F&& f) {
+#define MQA_D1(KD)                                  \
+    switch (q0) {                                   \
+        case 0: f.template operator()<KD, 0, 0>(); break; \
+        case 1: f.template operator()<KD, 1, 0>(); break; \
+        case 2: f.template operator()<KD, 2, 0>(); break; \
+        default: f.template operator()<KD, 3, 0>(); break; \
+    }
+#define MQA_D2(KD)                                   \
+    switch (q0 * 4 + q1) {                           \
+        case 1: f.template operator()<KD, 0, 1>(); break;  \
+        case 2: f.template operator()<KD, 0, 2>(); break;  \
+        case 3: f.template operator()<KD, 0, 3>(); break;  \
+        case 4: f.template operator()<KD, 1, 0>(); break;  \
+        case 6: f.template operator()<KD, 1, 2>(); break;  \
+        case 7: f.template operator()<KD, 1, 3>(); break;  \
+        case 8: f.template operator()<KD, 2, 0>(); break;  \
+        case 9: f.template operator()<KD, 2, 1>(); break;  \
+        case 11: f.template operator()<KD, 2, 3>(); break; \
+        case 12: f.template operator()<KD, 3, 0>(); break; \
+        case 13: f.template operator()<KD, 3, 1>(); break; \
+        default: f.template operator()<KD, 3, 2>(); break; \
+    }
+    switch (kind) {
+        case OPK_X: MQA_D1(OPK_X) break;
+        case OPK_RX: MQA_D1(OPK_RX) break;
+        case OPK_RY: MQA_D1(OPK_RY) break;
+        case OPK_RZ: MQA_D1(OPK_RZ) break;
+        case OPK_PS: MQA_D1(OPK_PS) break;
+        case OPK_GP: MQA_D1(OPK_GP) break;
+        case OPK_M1: MQA_D1(OPK_M1) break;
+        case OPK_CX: MQA_D2(OPK_CX) break;
+        case OPK_CZ: MQA_D2(OPK_CZ) break;
+        case OPK_SWAP: MQA_D2(OPK_SWAP) break;
+        case OPK_RZZ: MQA_D2(OPK_RZZ) break;
+        case OPK_RXX: MQA_D2(OPK_RXX) break;
+        case OPK_RYY: MQA_D2(OPK_RYY) break;
+        case OPK_M2:
+            if constexpr (FULL) MQA_D2(OPK_M2)
+            break;
+        default: break;
+    }
+#undef MQA_D1
+#undef MQA_D2
+}
+
+template <int KIND>
+constexpr bool kTwoTarget = KIND == OPK_CX || KIND == OPK_CZ || KIND == OPK_SWAP || KIND == OPK_RZZ ||
+                            KIND == OPK_RXX || KIND == OPK_RYY || KIND == OPK_M2;
+
+// G (forward) or G^dagger (back-sweep, angle already negated) on one register set.
+template <typename T, int KIND, int Q0, int Q1, bool CTRL>
+__device__ __forceinline__ void apply_body(Regs<T>& a, const GroupCtx& g, uint32_t cl, T c, T s,
+                                           const T* __restrict__ m) {
+    if constexpr (KIND == OPK_CX || KIND == OPK_CZ) {
+        controlled_xz<T, KIND == OPK_CX, Q0, Q1, CTRL>(a, g, cl);  // Q0 target, Q1 bound control
+    } else if constexpr (kTwoTarget<KIND>) {
+        two_target<T, KIND, Q0, Q1, CTRL>(a, g, cl, c, s, m);
+    } else {
+        one_target<T, KIND, Q0, CTRL>(a, g, cl, c, s, m);
+    }
+}
+
+// One op of a group.  ADJ: Re(tap) of the derivative, then G^dagger on psi and lambda.  Lean
+// passes (FULL = false) never carry local control predicates: the host picks FULL when they do.
+template <typename T, bool ADJ, bool FULL>
+__device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g, uint64_t base,
+                                       const T* __restrict__ pool, const double* __restrict__ ang_row,
+                                       TapAcc<T>& acc) {
+    if ((base & op.ctrl_global) != op.ctrl_global) return;  // a non-local control is off: no-op, zero tap
+    const uint32_t cl = op.ctrl_local;
+    const int kind = op.kind;
+    T c = 0, s = 0;
+    if (kind >= OPK_RX) {  // (cos, sin) of the half angle, or of the angle for PS / GP
+        c = T(ang_row[2 * op.slot]);
+        s = T(ang_row[2 * op.slot + 1]);
+        if (ADJ) s = -s;  // G^dagger = G(-angle)
+    }
+    const T* m = (op.mat >= 0) ? pool + op.mat : pool;
+    T tr = 0;  // Re(tap)
+    dispatch_op<FULL>(kind, op.q0, op.q1, [&]<int KD, int Q0, int Q1>() {
+        auto body = [&]<bool CT>() {
+            if constexpr (ADJ && KD >= OPK_RX) {
+                if constexpr (kTwoTarget<KD>) tr = two_target_tap<T, KD, Q0, Q1, CT>(p, l, g, cl);
+                else tr = one_target_tap<T, KD, Q0, CT>(p, l, g, cl);
+            }
+            apply_body<T, KD, Q0, Q1, CT>(p, g, cl, c, s, m);
+            if constexpr (ADJ) apply_body<T, KD, Q0, Q1, CT>(l, g, cl, c, s, m);
+        };
+        if constexpr (FULL) {
+            if (cl) body.template operator()<true>();
+            else body.template operator()<false>();
+        } else {
+            body.template operator()<false>();
+        }
+    });
+    if constexpr (ADJ) {
+        if (op.tap >= 0) {
+            // the group's taps arrive in order: shift them through the register file so that
+            // every access uses a constant index (no dynamically indexed, scratch-backed array)
+#pragma unroll
+            for (int j = kTapSlots<T> - 1; j > 0; j--) acc.r[j] = acc.r[j - 1];
+            acc.r[0] = tr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- tile I/O
+// HBM <-> LDS transfer of one tile.  Every access of a thread is issued before the first is
+// waited on (full unroll) so that enough bytes are in flight per CU to stream at HBM rate.
+// complex64 moves two adjacent amplitudes (16 B) per access: for u even, u and u + 1 are
+// adjacent in HBM (the low run C >= 1 is contiguous) and share the 16-byte LDS slot at
+// swz(u) & ~1, since swz(u ^ 1) == swz(u) ^ 1; when the swizzle sets bit 0 of swz(u) the
+// two amplitudes sit in that slot in swapped order.
+__device__ __forceinline__ float4 swap_halves_if(float4 v, bool odd) {
+    return odd ? make_float4(v.z, v.w, v.x, v.y) : v;
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* __restrict__ g,
+                                          const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT) {
+    const uint32_t cmask = (1u << C) - 1u;
+    if constexpr (sizeof(T) == 4) {
+        float4 buf[kAmpsPerThread / 2];
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t u = 2u * (tid + NT * j);
+            buf[j] = *reinterpret_cast<const float4*>(g + ((u & cmask) | offtab[u >> C]));
+        }
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t a = swz<T>(2u * (tid + NT * j));
+            *reinterpret_cast<float4*>(lds + (a & ~1u)) = swap_halves_if(buf[j], a & 1u);
+        }
+    } else {
+        // complex128: two batches of 8 x 16 B, the same bytes in flight as complex64
+#pragma unroll
+        for (int h = 0; h < kAmpsPerThread; h += kAmpsPerThread / 2) {
+            V2<T> buf[kAmpsPerThread / 2];
+#pragma unroll
+            for (int j = 0; j < kAmpsPerThread / 2; j++) {
+                const uint32_t u = tid + NT * (h + j);
+                buf[j] = g[(u & cmask) | offtab[u >> C]];
+            }
+#pragma unroll
+            for (int j = 0; j < kAmpsPerThread / 2; j++) lds[swz<T>(tid + NT * (h + j))] = buf[j];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_store(const V2<T>* __restrict__ lds, V2<T>* __restrict__ g,
+                                           const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT) {
+    const uint32_t cmask = (1u << C) - 1u;
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread / 2; j++) {
+            const uint32_t u = 2u * (tid + NT * j);
+            const uint32_t a = swz<T>(u);
+            *reinterpret_cast<float4*>(g + ((u & cmask) | offtab[u >> C])) =
+                swap_halves_if(*reinterpret_cast<const float4*>(lds + (a & ~1u)), a & 1u);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kAmpsPerThread; j++) {
+            const uint32_t u = tid + NT * j;
+            g[(u & cmask) | offtab[u >> C]] = lds[swz<T>(u)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pass frame
+// State shared by every group of a tile pass: LDS regions, this block's tile, wave geometry.
+template <typename T>
+struct TileFrame {
+    V2<T>* t_psi;
+    V2<T>* t_lam;
+    uint64_t* offtab;
+    double* lds_taps;
+    V2<T>* g_psi;
+    V2<T>* g_lam;
+    uint64_t base;    // global index of tile amplitude 0
+    uint64_t sample;
+    uint64_t tile;
+    int C;
+    uint32_t NT, tid;
+    int wave, n_waves, wave_width;
+};
+
+// Build the offset table and stage the tile (psi, and lambda for the back-sweep) in LDS.
+template <typename T, bool ADJ>
+__device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass,
+                                                   unsigned char* smem) {
+    using V = V2<T>;
+    TileFrame<T> f;
+    const int K = args.K;
+    f.NT = blockDim.x;
+    f.tid = threadIdx.x;
+    f.C = pass.low_run;
+    const int H = K - f.C;
+    const LdsLayout lay = lds_layout(K, f.C, ADJ, int(sizeof(V)), 0);
+    f.t_psi = reinterpret_cast<V*>(smem + lay.psi);
+    f.t_lam = reinterpret_cast<V*>(smem + lay.lam);  // used by the adjoint pass only
+    f.offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
+    f.lds_taps = reinterpret_cast<double*>(smem + lay.taps);
+    f.sample = blockIdx.y;
+    f.tile = blockIdx.x;
+    f.base = pdep64(f.tile, pass.nonlocal_mask);
+    for (uint32_t h = f.tid; h < (1u << H); h += f.NT) f.offtab[h] = pdep64(h, pass.local_hi_mask);
+    __syncthreads();
+    f.g_psi = static_cast<V*>(args.psi) + f.sample * args.stride + f.base;
+    f.g_lam = ADJ ? (static_cast<V*>(args.lam) + f.sample * args.stride + f.base) : nullptr;
+    tile_load<T>(f.t_psi, f.g_psi, f.offtab, f.C, f.tid, f.NT);
+    if constexpr (ADJ) tile_load<T>(f.t_lam, f.g_lam, f.offtab, f.C, f.tid, f.NT);
+    __syncthreads();
+    f.wave = int(f.tid >> 6);
+    f.n_waves = int((f.NT + 63) >> 6);
+    f.wave_width = f.NT < 64 ? int(f.NT) : 64;
+    return f;
+}
+
+// Write the tile back and emit this tile's partial of every tap of the pass.
+template <typename T, bool ADJ>
+__device__ __forceinline__ void tile_end(const TilePassArgs& args, const DevPass& pass, const TileFrame<T>& f) {
+    // launder tid so that the address arithmetic of the load phase is recomputed here rather
+    // than kept live in registers through the whole group loop
+    uint32_t tid_st = f.tid;
+    asm volatile("" : "+v"(tid_st));
+    tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
+    if constexpr (ADJ) tile_store<T>(f.t_lam, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    if constexpr (ADJ) {
+        // one partial per (sample, tap id, tile); summed over tiles by k_reduce_rows1
+        for (int t = int(f.tid); t < pass.n_taps; t += int(f.NT)) {
+            double sr = 0;
+            for (int w = 0; w < f.n_waves; w++) sr += f.lds_taps[t * f.n_waves + w];
+            const int tap_id = args.tap_ids[pass.tap_begin + t];
+            args.partials[(f.sample * uint64_t(args.n_tap_ids) + uint64_t(tap_id)) * args.n_tiles + f.tile] = sr;
+        }
+    }
+}
+
+// Register context of a group with positions pos (spos: ascending).
+__device__ __forceinline__ GroupCtx group_ctx(uint32_t tid, const uint8_t* pos, const uint8_t* spos) {
+    GroupCtx g;
+    uint32_t ub = tid;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t q = spos[k];
+        ub = ((ub >> q) << (q + 1)) | (ub & ((1u << q) - 1u));
+    }
+    g.ub = ub;
+#pragma unroll
+    for (int k = 0; k < 4; k++) g.pm[k] = 1u << pos[k];
+    return g;
+}
+
+// LDS slot of register v is swz(ub) ^ off[v]: a per-thread base XOR a wave-uniform offset.
+template <typename T, bool ADJ>
+__device__ __forceinline__ void group_load(const TileFrame<T>& f, uint32_t a0, const uint16_t* off, Regs<T>& p,
+                                           Regs<T>& l) {
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) {
+        const V2<T> x = f.t_psi[a0 ^ off[v]];
+        p.r[v] = x.x, p.i[v] = x.y;
+        if constexpr (ADJ) {
+            const V2<T> y = f.t_lam[a0 ^ off[v]];
+            l.r[v] = y.x, l.i[v] = y.y;
+        }
+    }
+}
+
+template <typename T, bool ADJ>
+__device__ __forceinline__ void group_store(const TileFrame<T>& f, uint32_t a0, const uint16_t* off,
+                                            const Regs<T>& p, const Regs<T>& l) {
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) {
+        f.t_psi[a0 ^ off[v]] = V2<T>{p.r[v], p.i[v]};
+        if constexpr (ADJ) f.t_lam[a0 ^ off[v]] = V2<T>{l.r[v], l.i[v]};
+    }
+}
+
+// Wave-reduce one Re(tap) and leave the wave's partial in the LDS tap table.
+template <typename T>
+__device__ __forceinline__ void tap_to_lds(const TileFrame<T>& f, T v, int tap) {
+    v = (f.wave_width == 64) ? wave_sum64(v) : wave_sum(v, f.wave_width);
+    if ((f.tid & 63) == 0) f.lds_taps[tap * f.n_waves + f.wave] = double(v);
+}
+
+}  // namespace mqa

@@ -1,0 +1,339 @@
+// jit.cpp - source generation, hipRTC compilation and launch of per-circuit tile passes.
+#include "jit.h"
+
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+#include <spawn.h>
+#include <sys/wait.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cinttypes>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+
+#include "launch.h"
+
+extern char** environ;
+
+namespace mqa {
+namespace {
+
+// Text of program.h and device_ops.h, embedded at build time (kJitHeaderNames / kJitHeaders).
+#include "jit_headers.inc"
+
+const char* kind_name(int k) {
+    switch (k) {
+        case OPK_X: return "OPK_X";
+        case OPK_M1: return "OPK_M1";
+        case OPK_SWAP: return "OPK_SWAP";
+        case OPK_M2: return "OPK_M2";
+        case OPK_CX: return "OPK_CX";
+        case OPK_CZ: return "OPK_CZ";
+        case OPK_RX: return "OPK_RX";
+        case OPK_RY: return "OPK_RY";
+        case OPK_RZ: return "OPK_RZ";
+        case OPK_PS: return "OPK_PS";
+        case OPK_GP: return "OPK_GP";
+        case OPK_RXX: return "OPK_RXX";
+        case OPK_RYY: return "OPK_RYY";
+        case OPK_RZZ: return "OPK_RZZ";
+        default: return nullptr;
+    }
+}
+
+bool two_target_kind(int k) {
+    return k == OPK_CX || k == OPK_CZ || k == OPK_SWAP || k == OPK_RZZ || k == OPK_RXX || k == OPK_RYY ||
+           k == OPK_M2;
+}
+
+// The compile helper sits next to this library (mqamd_jitc); empty if it is not there.
+std::string helper_path() {
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&helper_path), &info) == 0 || info.dli_fname == nullptr) return {};
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.rfind('/');
+    const std::string path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/mqamd_jitc";
+    return access(path.c_str(), X_OK) == 0 ? path : std::string();
+}
+
+bool read_file(const std::string& path, std::string& out) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    std::stringstream ss;
+    ss << in.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+}  // namespace
+
+std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
+    const DevPass& ps = pl.dev_passes.at(size_t(pass_index));
+    const bool adj = pl.adjoint;
+    const char* T = dtype == 0 ? "float" : "double";
+    const char* A = adj ? "true" : "false";
+    const unsigned nt = 1u << (pl.K - kGroupWidth);
+    std::ostringstream o;
+    o << "#include \"device_ops.h\"\n"
+      << "namespace mqa {\n"
+      << "extern \"C\" __global__ void __launch_bounds__(" << nt << ") mqa_jit_pass(TilePassArgs args) {\n"
+      << "  using T = " << T << ";\n"
+      << "  extern __shared__ __align__(16) unsigned char smem[];\n"
+      << "  DevPass pass{};\n"
+      << "  pass.nonlocal_mask = 0x" << std::hex << ps.nonlocal_mask << "ull;\n"
+      << "  pass.local_hi_mask = 0x" << ps.local_hi_mask << "ull;\n"
+      << std::dec << "  pass.group_begin = " << ps.group_begin << ";\n"
+      << "  pass.n_groups = " << ps.n_groups << ";\n"
+      << "  pass.tap_begin = " << ps.tap_begin << ";\n"
+      << "  pass.n_taps = " << ps.n_taps << ";\n"
+      << "  pass.low_run = " << ps.low_run << ";\n"
+      << "  TileFrame<T> f = tile_begin<T, " << A << ">(args, pass, smem);\n"
+      << "  const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;\n"
+      << "  const T* pool = static_cast<const T*>(args.pool);\n"
+      << "  (void)ang_row; (void)pool;\n";
+    for (int gi = 0; gi < ps.n_groups; gi++) {
+        const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
+        o << "  {  // group " << gi << "\n"
+          << "    constexpr uint8_t pos[4] = {" << int(g.pos[0]) << ", " << int(g.pos[1]) << ", " << int(g.pos[2])
+          << ", " << int(g.pos[3]) << "};\n"
+          << "    constexpr uint8_t spos[4] = {" << int(g.spos[0]) << ", " << int(g.spos[1]) << ", " << int(g.spos[2])
+          << ", " << int(g.spos[3]) << "};\n"
+          << "    constexpr uint16_t off[16] = {";
+        for (int v = 0; v < 16; v++) o << (v ? ", " : "") << g.off[v];
+        o << "};\n"
+          << "    const GroupCtx g = group_ctx(f.tid, pos, spos);\n"
+          << "    const uint32_t a0 = swz<T>(g.ub);\n"
+          << "    Regs<T> p, l;\n"
+          << "    group_load<T, " << A << ">(f, a0, off, p, l);\n";
+        std::vector<std::pair<int, int>> taps;  // (variable, pass-level tap index)
+        std::ostringstream b;                   // the group's ops
+        for (int oi = 0; oi < g.n_ops; oi++) {
+            const DevOp& op = pl.dev_ops.at(size_t(g.op_begin + oi));
+            const bool gated = op.ctrl_global != 0;
+            const char* ctrl = op.ctrl_local ? "true" : "false";
+            b << "    ";
+            if (gated) b << "if ((f.base & 0x" << std::hex << op.ctrl_global << "ull) == 0x" << op.ctrl_global << "ull) " << std::dec;
+            b << "{\n";
+            if (op.kind == OPK_M3 || op.kind == OPK_M4) {
+                const int m = op.kind == OPK_M3 ? 3 : 4;
+                b << "      dense_from_lds<T, " << m << ">(p, g, " << op.ctrl_local << "u, pool + " << op.mat
+                  << ", f.t_psi);\n";
+                if (adj)
+                    b << "      dense_from_lds<T, " << m << ">(l, g, " << op.ctrl_local << "u, pool + " << op.mat
+                      << ", f.t_lam);\n";
+                b << "    }\n";
+                continue;
+            }
+            const char* kn = kind_name(op.kind);
+            if (!kn) throw std::invalid_argument("code generation: unknown op kind " + std::to_string(op.kind));
+            const bool two = two_target_kind(op.kind);
+            const int q1 = two ? int(op.q1) : 0;
+            if (opk_is_param(op.kind)) {
+                b << "      const T c = T(ang_row[" << 2 * op.slot << "]);\n"
+                  << "      const T s = " << (adj ? "-" : "") << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
+            } else {
+                b << "      const T c = T(0), s = T(0);\n";
+            }
+            if (adj && opk_is_param(op.kind) && op.tap >= 0) {
+                const int var = int(taps.size());
+                taps.push_back({var, op.tap});
+                b << "      tap" << var << " = ";
+                if (two)
+                    b << "two_target_tap<T, " << kn << ", " << int(op.q0) << ", " << q1 << ", " << ctrl << ">(p, l, g, "
+                      << op.ctrl_local << "u);\n";
+                else
+                    b << "one_target_tap<T, " << kn << ", " << int(op.q0) << ", " << ctrl << ">(p, l, g, "
+                      << op.ctrl_local << "u);\n";
+            }
+            const std::string mat = op.mat >= 0 ? ("pool + " + std::to_string(op.mat)) : std::string("pool");
+            b << "      apply_body<T, " << kn << ", " << int(op.q0) << ", " << q1 << ", " << ctrl << ">(p, g, "
+              << op.ctrl_local << "u, c, s, " << mat << ");\n";
+            if (adj)
+                b << "      apply_body<T, " << kn << ", " << int(op.q0) << ", " << q1 << ", " << ctrl << ">(l, g, "
+                  << op.ctrl_local << "u, c, s, " << mat << ");\n";
+            b << "    }\n";
+        }
+        if (!taps.empty()) {  // tap variables, declared before the ops that assign them
+            o << "    T";
+            for (size_t k = 0; k < taps.size(); k++) o << (k ? ", " : " ") << "tap" << k << " = T(0)";
+            o << ";\n";
+        }
+        o << b.str();
+        for (auto& [var, tap] : taps) o << "    tap_to_lds<T>(f, tap" << var << ", " << tap << ");\n";
+        o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
+          << "    __syncthreads();\n"
+          << "  }\n";
+    }
+    o << "  tile_end<T, " << A << ">(args, pass, f);\n"
+      << "}\n"
+      << "}  // namespace mqa\n";
+    return o.str();
+}
+
+bool jit_compile(const std::string& src, std::string& code, std::string& log) {
+    hiprtcProgram prog;
+    const int n_headers = int(sizeof(kJitHeaders) / sizeof(kJitHeaders[0]));
+    if (hiprtcCreateProgram(&prog, src.c_str(), "mqa_jit_pass.hip", n_headers, kJitHeaders, kJitHeaderNames) !=
+        HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    // the same code generation flags as the static kernels (packed FP32 disabled)
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    log.assign(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, log.data());
+    bool ok = r == HIPRTC_SUCCESS;
+    if (ok) {
+        size_t cs = 0;
+        ok = hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0;
+        if (ok) {
+            code.assign(cs, '\0');
+            ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+        }
+    }
+    hiprtcDestroyProgram(&prog);
+    return ok;
+}
+
+JitPlan::JitPlan(const Plan& pl, int dtype) : plan_(pl), dtype_(dtype) {
+    code_.resize(plan_.dev_passes.size());
+    worker_ = std::thread([this] { compile_all(); });
+}
+
+JitPlan::~JitPlan() {
+    if (worker_.joinable()) worker_.join();
+    for (auto& [dev, mods] : modules_) {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(dev);
+        for (auto m : mods)
+            if (m) (void)hipModuleUnload(m);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+}
+
+void JitPlan::compile_all() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = int(code_.size());
+    std::vector<std::string> src(static_cast<size_t>(n)), err(static_cast<size_t>(n));
+    std::vector<char> done(static_cast<size_t>(n), 0);
+    try {
+        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_);
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu_);
+        error_ = e.what();
+        state_.store(kFailed);
+        return;
+    }
+    if (const char* dir = std::getenv("MQAMD_JIT_DUMP")) {  // keep the generated sources
+        for (int p = 0; p < n; p++) std::ofstream(std::string(dir) + "/pass" + std::to_string(p) + ".hip") << src[size_t(p)];
+    }
+
+    // parallel: helper processes, each compiling every W-th pass
+    const std::string helper = helper_path();
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int workers = std::max(1, std::min<int>({n, int(hw), 16}));
+    if (!helper.empty() && workers > 1) {
+        const char* tmp = std::getenv("TMPDIR");
+        std::string tmpl = std::string(tmp && *tmp ? tmp : "/tmp") + "/mqamd_jit_XXXXXX";
+        if (mkdtemp(tmpl.data()) != nullptr) {
+            const std::string dir = tmpl;
+            auto src_path = [&](int p) { return dir + "/pass" + std::to_string(p) + ".hip"; };
+            auto out_path = [&](int p) { return dir + "/pass" + std::to_string(p) + ".co"; };
+            for (int p = 0; p < n; p++) std::ofstream(src_path(p)) << src[size_t(p)];
+            std::vector<pid_t> pids;
+            for (int w = 0; w < workers; w++) {
+                std::vector<std::string> args{helper};
+                for (int p = w; p < n; p += workers) {
+                    args.push_back(out_path(p));
+                    args.push_back(src_path(p));
+                }
+                std::vector<char*> argv;
+                for (auto& a : args) argv.push_back(a.data());
+                argv.push_back(nullptr);
+                pid_t pid = 0;
+                if (posix_spawn(&pid, helper.c_str(), nullptr, nullptr, argv.data(), environ) == 0) pids.push_back(pid);
+            }
+            for (pid_t pid : pids) {
+                int st = 0;
+                while (waitpid(pid, &st, 0) < 0 && errno == EINTR) {
+                }
+            }
+            for (int p = 0; p < n; p++) {
+                done[size_t(p)] = read_file(out_path(p), code_[size_t(p)]) && !code_[size_t(p)].empty();
+                std::remove(out_path(p).c_str());
+                std::remove(src_path(p).c_str());
+                std::remove((out_path(p) + ".log").c_str());
+            }
+            rmdir(dir.c_str());
+        }
+    }
+    // serial, in this process: whatever the helpers did not produce
+    for (int p = 0; p < n; p++) {
+        if (done[size_t(p)]) continue;
+        std::string log;
+        if (!jit_compile(src[size_t(p)], code_[size_t(p)], log)) err[size_t(p)] = log.empty() ? "compile failed" : log;
+    }
+    seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::string first;
+    for (int p = 0; p < n && first.empty(); p++)
+        if (!err[size_t(p)].empty()) first = "pass " + std::to_string(p) + ": " + err[size_t(p)];
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        error_ = first;
+    }
+    state_.store(first.empty() ? kReady : kFailed);
+}
+
+void JitPlan::wait() {
+    if (worker_.joinable()) worker_.join();
+}
+
+std::string JitPlan::error() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return error_;
+}
+
+hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
+    if (!ready()) return hipErrorNotReady;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto& fns = functions_[dev];
+        if (fns.empty()) {
+            auto& mods = modules_[dev];
+            mods.assign(code_.size(), nullptr);
+            fns.assign(code_.size(), nullptr);
+            for (size_t p = 0; p < code_.size(); p++) {
+                e = hipModuleLoadData(&mods[p], code_[p].data());
+                if (e == hipSuccess) e = hipModuleGetFunction(&fns[p], mods[p], "mqa_jit_pass");
+                if (e != hipSuccess) {
+                    fns.clear();
+                    return e;
+                }
+            }
+        }
+        fn = fns.at(size_t(pass));
+    }
+    TilePassArgs args = a;
+    void* params[] = {&args};
+    const unsigned nt = 1u << (plan_.K - kGroupWidth);
+    e = hipModuleLaunchKernel(fn, unsigned(a.n_tiles), unsigned(batch), 1, nt, 1, 1, unsigned(lds), s, params,
+                              nullptr);
+    return e;
+}
+
+}  // namespace mqa

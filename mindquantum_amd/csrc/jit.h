@@ -1,0 +1,65 @@
+// jit.h - per-circuit code generation for the tile passes.
+//
+// The interpreted pass kernel (k_tile_pass) reads every op descriptor and switches on its kind
+// and register positions; across that switch the register allocator copies the whole register
+// state on many paths.  For a plan that runs often the same passes are emitted as HIP source
+// with every op kind, position, angle slot and tap index a compile-time constant, compiled with
+// hipRTC on host threads (one program per pass, in parallel) and loaded as modules.  Launches
+// use the interpreted kernel until the compiled passes are ready.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "program.h"
+#include "schedule.h"
+
+namespace mqa {
+
+// HIP source of pass `pass` of a plan (dtype 0: complex64, 1: complex128).  It includes
+// "device_ops.h" and defines one kernel, extern "C" mqa_jit_pass(TilePassArgs).
+std::string jit_pass_source(const Plan& pl, int pass, int dtype);
+
+// Compile one generated source for gfx950.  Returns false (log filled) on failure.
+bool jit_compile(const std::string& src, std::string& code, std::string& log);
+
+class JitPlan {
+  public:
+    // Copies the plan and starts compiling every pass on a background thread.
+    JitPlan(const Plan& pl, int dtype);
+    ~JitPlan();
+    JitPlan(const JitPlan&) = delete;
+    JitPlan& operator=(const JitPlan&) = delete;
+
+    bool ready() const { return state_.load() == kReady; }
+    bool failed() const { return state_.load() == kFailed; }
+    void wait();                       // block until compiled (or failed)
+    std::string error() const;
+    double compile_seconds() const { return seconds_; }
+
+    // Launch pass `pass` on the current device and stream (modules load on first use).
+    hipError_t launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s);
+
+  private:
+    enum { kCompiling = 0, kReady = 1, kFailed = 2 };
+    void compile_all();
+
+    Plan plan_;
+    int dtype_;
+    std::atomic<int> state_{kCompiling};
+    std::vector<std::string> code_;
+    std::string error_;
+    double seconds_ = 0;
+    std::thread worker_;
+    mutable std::mutex mu_;
+    std::map<int, std::vector<hipModule_t>> modules_;    // per device
+    std::map<int, std::vector<hipFunction_t>> functions_;
+};
+
+}  // namespace mqa

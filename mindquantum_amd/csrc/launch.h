@@ -11,23 +11,6 @@ namespace mqa {
 
 // dtype: 0 = complex64 (float2 amplitudes), 1 = complex128 (double2 amplitudes).
 
-struct TilePassArgs {
-    void* psi;          // [batch][stride] amplitudes
-    void* lam;          // co-state (adjoint passes only)
-    int K;              // tile qubits
-    int pass_index;
-    uint64_t stride;    // amplitudes between consecutive samples
-    uint64_t n_tiles;   // tiles per sample = 2^(n - K)
-    const DevPass* passes;
-    const DevGroup* groups;
-    const DevOp* ops;
-    const void* pool;      // matrix pool in the state's real type (float or double)
-    const double* angles;  // [batch][n_slots][2] (cos, sin) (NULL without parameterised ops)
-    int n_slots;
-    int n_tap_ids;
-    const int32_t* tap_ids;
-    double* partials;      // [batch][n_tap_ids][n_tiles] (re, im)
-};
 
 struct HamArgs {
     const void* ket;
