@@ -99,7 +99,11 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     code = 0 if dtype == mq.complex64 else 1
     passes = len(ir.NativeCircuit(circ, []).schedule(n, code))
-    sim.apply_circuit(circ[: 2 * n])  # warm plan cache / allocations on a short prefix
+    # untimed run of the same circuit: plan, allocations and the compiled passes (jit = 2)
+    c0 = _native.get_option("jit_compile_ms")
+    sim.apply_circuit(circ)
+    compile_s = (_native.get_option("jit_compile_ms") - c0) / 1e3
+    sim.reset()
     _native.profile_reset()
     _native.set_option("profile", 1)
     t0 = time.perf_counter()
@@ -114,11 +118,15 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth):
         "frac_of_peak": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "gate_equivalent_gbs": round(len(circ) * 2 * state / dt / 1e9, 1),
         "kernel_ms": round(ms, 3), "launches": launches,
+        "compiled_passes": _native.get_option("jit") != 0, "compile_seconds_untimed": round(compile_s, 1),
     }
     log(f"gate apply {n}q {np.dtype(dtype).name}: {res}")
     # unfused: one gate per pass, on a shorter prefix of the same circuit
     short = circ[: unfused_depth * (n + n // 2)]
+    # single-gate passes stream memory; they run on the interpreted kernel
     _native.set_option("max_ops_per_pass", 1)
+    jit = _native.get_option("jit")
+    _native.set_option("jit", 0)
     try:
         sim2 = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
         sim2.apply_circuit(short[:4])
@@ -131,6 +139,7 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth):
         launches2, ms2, byts2 = _native.profile("tile_pass_fwd")
     finally:
         _native.set_option("max_ops_per_pass", -1)
+        _native.set_option("jit", jit)
     res["unfused"] = {"gates": len(short), "seconds": round(dt2, 4), "hbm_gbs": round(byts2 / (ms2 * 1e-3) / 1e9, 1),
                       "launches": launches2}
     del sim, sim2
@@ -190,6 +199,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--apply-qubits", type=int, default=28)
     ap.add_argument("--apply-depth", type=int, default=200)
+    ap.add_argument("--interpreted", action="store_true", help="run the interpreted pass kernels (no code generation)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -203,6 +213,8 @@ def main():
     if _native.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    # per-circuit compiled passes, compiled before the first launch (inside the warmup steps)
+    _native.set_option("jit", 0 if args.interpreted else 2)
 
     value, ms_step, stats, n_params, n_gates = run_adjoint(mq.complex64, args.qubits, args.layers, args.batch,
                                                            args.steps, args.warmup, args.seed, profile=True)
@@ -213,11 +225,12 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_tile_pass<float, adjoint>", "avg_launch_ms": round(avg_ms, 4),
+                "kernel": "k_tile_pass<float, adjoint>" if args.interpreted else "mqa_jit_tile_pass_adj (complex64)",
+                "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": per_launch, "launches": launches}
     share = {k: round(v[1], 2) for k, v in stats.items()}
 
-    extra = {"kernel_ms_in_timed_region": share}
+    extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3}
     if not args.no_extra:
         v128, ms128, _, _, _ = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, 1, 1,
                                            args.seed, profile=False)

@@ -615,6 +615,10 @@ int mqa_get_option(const char* key, int64_t* value) {
             *value = g_jit_launches.load();
             return;
         }
+        if (std::strcmp(key, "jit_compile_ms") == 0) {  // read-only: time spent compiling generated passes
+            *value = int64_t(jit_compile_seconds_total() * 1e3);
+            return;
+        }
         for (auto& e : opt_table()) {
             if (std::strcmp(e.key, key) == 0) {
                 *value = *e.v;

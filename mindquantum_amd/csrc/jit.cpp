@@ -73,6 +73,13 @@ bool read_file(const std::string& path, std::string& out) {
 
 }  // namespace
 
+const char* jit_kernel_name(bool adjoint) { return adjoint ? "mqa_jit_tile_pass_adj" : "mqa_jit_tile_pass_fwd"; }
+
+namespace {
+std::atomic<int64_t> g_compile_us{0};
+}
+double jit_compile_seconds_total() { return double(g_compile_us.load()) * 1e-6; }
+
 std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
     const DevPass& ps = pl.dev_passes.at(size_t(pass_index));
     const bool adj = pl.adjoint;
@@ -82,7 +89,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
     std::ostringstream o;
     o << "#include \"device_ops.h\"\n"
       << "namespace mqa {\n"
-      << "extern \"C\" __global__ void __launch_bounds__(" << nt << ") mqa_jit_pass(TilePassArgs args) {\n"
+      << "extern \"C\" __global__ void __launch_bounds__(" << nt << ") " << jit_kernel_name(adj) << "(TilePassArgs args) {\n"
       << "  using T = " << T << ";\n"
       << "  extern __shared__ __align__(16) unsigned char smem[];\n"
       << "  DevPass pass{};\n"
@@ -285,6 +292,7 @@ void JitPlan::compile_all() {
         if (!jit_compile(src[size_t(p)], code_[size_t(p)], log)) err[size_t(p)] = log.empty() ? "compile failed" : log;
     }
     seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    g_compile_us.fetch_add(int64_t(seconds_ * 1e6));
     std::string first;
     for (int p = 0; p < n && first.empty(); p++)
         if (!err[size_t(p)].empty()) first = "pass " + std::to_string(p) + ": " + err[size_t(p)];
@@ -319,7 +327,7 @@ hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size
             fns.assign(code_.size(), nullptr);
             for (size_t p = 0; p < code_.size(); p++) {
                 e = hipModuleLoadData(&mods[p], code_[p].data());
-                if (e == hipSuccess) e = hipModuleGetFunction(&fns[p], mods[p], "mqa_jit_pass");
+                if (e == hipSuccess) e = hipModuleGetFunction(&fns[p], mods[p], jit_kernel_name(plan_.adjoint));
                 if (e != hipSuccess) {
                     fns.clear();
                     return e;

@@ -22,9 +22,15 @@
 
 namespace mqa {
 
+// Name of the generated kernel: mqa_jit_tile_pass_fwd / mqa_jit_tile_pass_adj.
+const char* jit_kernel_name(bool adjoint);
+
 // HIP source of pass `pass` of a plan (dtype 0: complex64, 1: complex128).  It includes
-// "device_ops.h" and defines one kernel, extern "C" mqa_jit_pass(TilePassArgs).
+// "device_ops.h" and defines one extern "C" kernel (jit_kernel_name) taking TilePassArgs.
 std::string jit_pass_source(const Plan& pl, int pass, int dtype);
+
+// Wall time spent compiling generated passes in this process.
+double jit_compile_seconds_total();
 
 // Compile one generated source for gfx950.  Returns false (log filled) on failure.
 bool jit_compile(const std::string& src, std::string& code, std::string& log);
