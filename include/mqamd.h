@@ -97,8 +97,14 @@ const char* mqa_last_error(void);
 const char* mqa_version(void);
 /* Number of visible HIP devices (0 without a GPU).  Never fails. */
 int mqa_device_count(void);
-/* Tuning knob ("tile_qubits_c64", "tile_qubits_c128", "adj_tile_qubits_c64",
- * "adj_tile_qubits_c128", "group_width", "lowq", "fuse"); value < 0 restores the default. */
+/* Tuning knobs (value < 0 restores the default): "tile_qubits_c64", "tile_qubits_c128",
+ * "adj_tile_qubits_c64", "adj_tile_qubits_c128", "group_width", "lowq_c64", "lowq_c128",
+ * "max_ops_per_pass" (1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
+ * MQAMD_HBM_BUDGET_BYTES; 0 = 85% of free memory), "profile", "ham_tile_qubits_c64",
+ * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
+ * the first launch), "jit_min_amps".  mqa_get_option also reads the counters "jit_launches",
+ * "jit_compile_ms" and "grad_recomputes".  No counterpart in the reference (its CPU kernels have
+ * no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
 int mqa_get_option(const char* key, int64_t* value);
 
@@ -167,12 +173,26 @@ int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ha
  * hermitian adjoint gradient for a batch of encoder rows sharing one ansatz vector.
  * Param index p < n_enc reads enc[b*n_enc + p]; p >= n_enc reads ans[p - n_enc].
  * f_out: [batch][n_hams][2] (complex <psi|H|psi>); g_out: [batch][n_hams][n_enc+n_ans]
- * (real; the reference returns 2*Re with a zero imaginary part, gradient.h:124-133). */
+ * (real; the reference returns 2*Re with a zero imaginary part, gradient.h:124-133).
+ * Samples run in chunks that fit the HBM budget (two states per sample, three when several
+ * hamiltonians share a kept copy of the forward state; if that copy does not fit, the forward
+ * sweep is recomputed per hamiltonian). */
 int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ,
                        const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
                        double* g_out);
 /* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
  * state cast to dtype. */
+/* Expectation and hermitian adjoint gradient of many independent circuits of uneven size in
+ * shared launches (one launch per sweep step over the tiles of every circuit).  Item i: circuit
+ * circs[i] on n_qubits[i] qubits starting from |0...0>, hamiltonian hams[i], parameter values
+ * params[param_offset[i] ..] (circs[i]'s own parameter order).  Outputs: f_out[2i], f_out[2i+1]
+ * (<psi|H|psi>), g_out[param_offset[i] ..] (real gradient).  Items run in chunks that fit the HBM
+ * budget.  The reference has no batched entry for different circuits: a caller makes one
+ * get_expectation_with_grad call per circuit and simulator (mqsim.py:255-439); this is the same
+ * result for a whole list. */
+int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
+                   const int32_t* n_qubits, const double* params, const int64_t* param_offset, int32_t dtype,
+                   double* f_out, double* g_out);
 int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out);
 /* blas.inner_product(a, b) (bind_vec_state.h:86-90, blas.h): <a|b>. */
 int mqa_inner_product(const mqa_sim* bra, const mqa_sim* ket, double out[2]);

@@ -9,6 +9,6 @@ from .core import (CNOT, H, I, ISWAP, RX, RY, RZ, SWAP, SX, UN, Circuit, GlobalP
                    PhaseShift, QubitOperator, Rxx, Ryy, Rzz, S, T, UnivMathGate, X, Y, Z)
 from .core import gates
 from .dtype import complex64, complex128
-from .simulator import Simulator, inner_product
+from .simulator import Simulator, expectation_with_grad_batch, inner_product
 
 __version__ = "0.1.0"

@@ -386,6 +386,7 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 }
 
 std::atomic<int64_t> g_jit_launches{0};
+std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
 void count_jit_launch() { g_jit_launches.fetch_add(1); }
 
 void sync(mqa_sim* sim, const char* what) {
@@ -485,6 +486,7 @@ bool ham_apply_tiles(mqa_sim* sim, mqa_ham* ham, const void* ket, void* out, boo
     h.passes = tp->passes.as<DevHamPass>();
     h.groups = tp->groups.as<DevHamTileGroup>();
     h.terms = tp->terms.as<DevHamTileTerm>();
+    h.n_tiles = n_tiles;
     if (expect) {
         sim->fpart.ensure(batch * n_tiles * 2 * sizeof(double));
         sim->fsum.ensure(batch * 2 * sizeof(double));
@@ -613,6 +615,10 @@ int mqa_get_option(const char* key, int64_t* value) {
         require(key != nullptr && value != nullptr, "NULL argument");
         if (std::strcmp(key, "jit_launches") == 0) {  // read-only: passes launched from compiled kernels
             *value = g_jit_launches.load();
+            return;
+        }
+        if (std::strcmp(key, "grad_recomputes") == 0) {  // read-only: see mqa_sim_grad_batch
+            *value = g_recomputes.load();
             return;
         }
         if (std::strcmp(key, "jit_compile_ms") == 0) {  // read-only: time spent compiling generated passes
@@ -1013,17 +1019,22 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         const uint64_t n_tiles = uint64_t(1) << (adj->plan.n_eff - adj->plan.K);
         const size_t sb = sim->state_bytes();
 
-        // samples per chunk from the HBM budget: psi + lam (+ a kept copy for several hamiltonians)
-        const size_t per_sample = sb * (n_hams > 1 ? 3 : 2) + size_t(n_tap_ids) * n_tiles * 8 +
-                                  size_t(circ->n_slots) * 16 + size_t(n_tap_ids + n_params) * 16 + 4096;
+        // samples per chunk from the HBM budget: psi + lam, plus a kept copy of the forward state
+        // when several hamiltonians share it.  If one sample with that copy exceeds the budget,
+        // the forward sweep is recomputed from the initial state for every further hamiltonian
+        // instead (the back-sweep consumes psi, so it must be restored one way or the other).
+        const size_t side = size_t(n_tap_ids) * n_tiles * 8 + size_t(circ->n_slots) * 16 +
+                            size_t(n_tap_ids + n_params) * 16 + 4096;
         const size_t budget = hbm_budget();
+        const bool keep = n_hams > 1 && budget >= sb * 3 + side;
+        const size_t per_sample = sb * (keep ? 3 : 2) + side;
         int64_t chunk = int64_t(std::max<size_t>(budget / per_sample, 1));
         chunk = std::min<int64_t>(chunk, batch);
         chunk = std::min<int64_t>(chunk, 65535);
 
         sim->psi.ensure(sb * chunk);
         sim->lam.ensure(sb * chunk);
-        if (n_hams > 1) sim->psi_keep.ensure(sb * chunk);
+        if (keep) sim->psi_keep.ensure(sb * chunk);
         sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 8, 16));
         sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 8, 16));
         sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
@@ -1035,11 +1046,18 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
             hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff, nb, s),
                       "fill");
             run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
-            if (n_hams > 1)
+            if (keep)
                 hip_check(hipMemcpyAsync(sim->psi_keep.p, sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s), "keep");
             for (int h = 0; h < n_hams; h++) {
-                if (h > 0)
+                if (h > 0 && keep) {
                     hip_check(hipMemcpyAsync(sim->psi.p, sim->psi_keep.p, sb * nb, hipMemcpyDeviceToDevice, s), "restore");
+                } else if (h > 0) {  // recompute the forward state
+                    hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff,
+                                          sim->dim_eff, nb, s),
+                              "fill");
+                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                    g_recomputes.fetch_add(1);
+                }
                 auto* ham = const_cast<mqa_ham*>(hams[h]);
                 // lam = H psi, f = <psi|H psi>
                 ham_apply(sim, ham, sim->psi.p, sim->psi.p, sim->lam.p, nb);
@@ -1074,6 +1092,342 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                     for (int p = 0; p < n_params; p++) go[p] = g_host[size_t(b) * n_params + p];
                 }
             }
+        }
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// One circuit of an uneven batch and its place in the chunk's device arrays.
+struct GradItem {
+    int index = 0;  // position in the caller's arrays
+    int n_eff = 0;
+    uint64_t dim = 0;
+    DevPlan* fwd = nullptr;
+    DevPlan* adj = nullptr;
+    mqa_ham::TilePlan* hp = nullptr;  // NULL: gather kernel
+    uint64_t psi = 0, lam = 0;        // amplitude offsets in the chunk's arenas
+    uint64_t ang = 0;                 // offset (doubles) in the angle table
+    uint64_t taps_part = 0, taps = 0; // offsets in the tap partials / reduced taps
+    uint64_t fpart = 0, f_len = 0;    // offset (doubles) and (re, im) pairs of the <psi|H psi> partials
+    size_t bytes = 0;                 // device bytes this item needs
+};
+
+struct StreamOwner {
+    hipStream_t s = nullptr;
+    StreamOwner() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~StreamOwner() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// Launch groups of one pass step: items that share the launch shape.
+template <typename A>
+struct StepGroup {
+    int K = 0, C = 0, max_taps = 0;
+    bool full = false, first = false, last = false;
+    std::vector<A> args;
+    std::vector<uint64_t> prefix;
+    uint64_t blocks = 0;
+    size_t args_at = 0, prefix_at = 0;  // element offsets in the uploaded tables
+};
+
+void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
+                    const double* params, const int64_t* param_offset, int dtype, hipStream_t s, double* f_out,
+                    double* g_out) {
+    const size_t amp = dtype == MQA_C64 ? 8 : 16;
+    uint64_t amps = 0, n_ang = 0, n_part = 0, n_taps = 0, n_fpart = 0;
+    for (auto& it : items) {
+        const mqa_circuit* c = circs[it.index];
+        it.psi = amps;
+        it.lam = amps + it.dim;
+        amps += 2 * it.dim;
+        it.ang = n_ang;
+        n_ang += uint64_t(c->n_slots) * 2;
+        const uint64_t adj_tiles = uint64_t(1) << (it.n_eff - it.adj->plan.K);
+        it.taps_part = n_part;
+        n_part += uint64_t(c->n_tap_ids) * adj_tiles;
+        it.taps = n_taps;
+        n_taps += uint64_t(c->n_tap_ids);
+        it.fpart = n_fpart;
+        it.f_len = it.hp ? (uint64_t(1) << (it.n_eff - it.hp->plan.K))
+                         : (it.dim + 2047) / 2048;  // gather kernel: one partial per 2048 amplitudes
+        n_fpart += 2 * it.f_len;
+    }
+    DevBuf arena, ang_d, part_d, taps_d, fpart_d, table_d;
+    arena.ensure(amps * amp);
+    part_d.ensure(std::max<uint64_t>(n_part, 1) * 8);
+    taps_d.ensure(std::max<uint64_t>(n_taps, 1) * 8);
+    fpart_d.ensure(std::max<uint64_t>(n_fpart, 1) * 8);
+
+    // angle table, computed on the host (a few doubles per gate)
+    std::vector<double> ang(std::max<uint64_t>(n_ang, 1));
+    for (auto& it : items) {
+        const mqa_circuit* c = circs[it.index];
+        const double* pv = params + param_offset[it.index];
+        for (int sl = 0; sl < c->n_slots; sl++) {
+            double th = c->slot_const[size_t(sl)];
+            for (int k = c->slot_term_begin[size_t(sl)]; k < c->slot_term_begin[size_t(sl) + 1]; k++)
+                th += c->slot_term_coeff[size_t(k)] * pv[c->slot_term_param[size_t(k)]];
+            if (c->slot_half[size_t(sl)]) th *= 0.5;
+            ang[it.ang + 2 * uint64_t(sl)] = std::cos(th);
+            ang[it.ang + 2 * uint64_t(sl) + 1] = std::sin(th);
+        }
+    }
+    upload(ang_d, ang, s);
+
+    // |0...0> for every item
+    std::vector<uint64_t> unit;
+    for (auto& it : items) unit.push_back(it.psi);
+    DevBuf unit_d;
+    upload(unit_d, unit, s);
+    hip_check(hipMemsetAsync(arena.p, 0, amps * amp, s), "memset");
+    hip_check(launch_unit_items(dtype, arena.p, unit_d.as<uint64_t>(), int(unit.size()), s), "unit");
+
+    char* base = static_cast<char*>(arena.p);
+    auto tile_args = [&](const GradItem& it, DevPlan* dp, int pass, bool adjoint) {
+        TilePassArgs a{};
+        a.psi = base + it.psi * amp;
+        a.lam = adjoint ? base + it.lam * amp : nullptr;
+        a.K = dp->plan.K;
+        a.pass_index = pass;
+        a.stride = it.dim;
+        a.n_tiles = uint64_t(1) << (it.n_eff - dp->plan.K);
+        a.passes = dp->passes.as<DevPass>();
+        a.groups = dp->groups.as<DevGroup>();
+        a.ops = dp->ops.as<DevOp>();
+        a.pool = dp->pool.p;
+        a.angles = ang_d.as<double>() + it.ang;
+        a.n_slots = circs[it.index]->n_slots;
+        a.n_tap_ids = circs[it.index]->n_tap_ids;
+        a.tap_ids = dp->tap_ids.as<int32_t>();
+        a.partials = adjoint ? part_d.as<double>() + it.taps_part : nullptr;
+        return a;
+    };
+
+    // every step of the forward sweep, the Hamiltonian passes and the back-sweep: tables first
+    std::vector<std::vector<StepGroup<TilePassArgs>>> fwd_steps, adj_steps;
+    std::vector<std::vector<StepGroup<HamTileArgs>>> ham_steps;
+    auto build_tile_steps = [&](bool adjoint, std::vector<std::vector<StepGroup<TilePassArgs>>>& steps) {
+        size_t max_p = 0;
+        for (auto& it : items) max_p = std::max(max_p, (adjoint ? it.adj : it.fwd)->plan.passes.size());
+        for (size_t st = 0; st < max_p; st++) {
+            std::map<std::tuple<int, int, int>, StepGroup<TilePassArgs>> by;
+            for (auto& it : items) {
+                DevPlan* dp = adjoint ? it.adj : it.fwd;
+                if (st >= dp->plan.passes.size()) continue;
+                if (adjoint && circs[it.index]->n_tap_ids == 0) continue;  // nothing to differentiate
+                auto& g = by[std::make_tuple(dp->plan.K, dp->plan.C, int(dp->full))];
+                g.K = dp->plan.K;
+                g.C = dp->plan.C;
+                g.full = dp->full;
+                g.max_taps = std::max(g.max_taps, std::max(dp->plan.max_taps, 1));
+                g.prefix.push_back(g.blocks);
+                g.args.push_back(tile_args(it, dp, int(st), adjoint));
+                g.blocks += uint64_t(1) << (it.n_eff - dp->plan.K);
+            }
+            std::vector<StepGroup<TilePassArgs>> v;
+            for (auto& [k, g] : by) v.push_back(std::move(g));
+            steps.push_back(std::move(v));
+        }
+    };
+    build_tile_steps(false, fwd_steps);
+    build_tile_steps(true, adj_steps);
+    {
+        size_t max_p = 0;
+        for (auto& it : items)
+            if (it.hp) max_p = std::max(max_p, it.hp->plan.passes.size());
+        for (size_t st = 0; st < max_p; st++) {
+            std::map<std::tuple<int, int, int, int>, StepGroup<HamTileArgs>> by;
+            for (auto& it : items) {
+                if (!it.hp || st >= it.hp->plan.passes.size()) continue;
+                const HamTilePlan& pl = it.hp->plan;
+                const bool first = st == 0, last = st + 1 == pl.passes.size();
+                auto& g = by[std::make_tuple(pl.K, pl.C, int(first), int(last))];
+                g.K = pl.K;
+                g.C = pl.C;
+                g.first = first;
+                g.last = last;
+                HamTileArgs h{};
+                h.ket = base + it.psi * amp;
+                h.out = base + it.lam * amp;
+                h.stride = it.dim;
+                h.K = pl.K;
+                h.pass_index = int(st);
+                h.passes = it.hp->passes.as<DevHamPass>();
+                h.groups = it.hp->groups.as<DevHamTileGroup>();
+                h.terms = it.hp->terms.as<DevHamTileTerm>();
+                h.partial = last ? fpart_d.as<double>() + it.fpart : nullptr;
+                h.n_tiles = uint64_t(1) << (it.n_eff - pl.K);
+                g.prefix.push_back(g.blocks);
+                g.args.push_back(h);
+                g.blocks += h.n_tiles;
+            }
+            std::vector<StepGroup<HamTileArgs>> v;
+            for (auto& [k, g] : by) v.push_back(std::move(g));
+            ham_steps.push_back(std::move(v));
+        }
+    }
+    // one upload of every table: [tile args][ham args][prefixes]
+    std::vector<TilePassArgs> targs;
+    std::vector<HamTileArgs> hargs;
+    std::vector<uint64_t> prefix;
+    for (auto* steps : {&fwd_steps, &adj_steps})
+        for (auto& v : *steps)
+            for (auto& g : v) {
+                g.args_at = targs.size();
+                targs.insert(targs.end(), g.args.begin(), g.args.end());
+                g.prefix_at = prefix.size();
+                prefix.insert(prefix.end(), g.prefix.begin(), g.prefix.end());
+            }
+    for (auto& v : ham_steps)
+        for (auto& g : v) {
+            g.args_at = hargs.size();
+            hargs.insert(hargs.end(), g.args.begin(), g.args.end());
+            g.prefix_at = prefix.size();
+            prefix.insert(prefix.end(), g.prefix.begin(), g.prefix.end());
+        }
+    const size_t t_bytes = targs.size() * sizeof(TilePassArgs), h_bytes = hargs.size() * sizeof(HamTileArgs);
+    const size_t h_at = (t_bytes + 255) & ~size_t(255), p_at = (h_at + h_bytes + 255) & ~size_t(255);
+    std::vector<char> table(p_at + prefix.size() * sizeof(uint64_t) + 8);
+    std::memcpy(table.data(), targs.data(), t_bytes);
+    std::memcpy(table.data() + h_at, hargs.data(), h_bytes);
+    std::memcpy(table.data() + p_at, prefix.data(), prefix.size() * sizeof(uint64_t));
+    upload(table_d, table, s);
+    char* tab = static_cast<char*>(table_d.p);
+    const auto* targs_d = reinterpret_cast<const TilePassArgs*>(tab);
+    const auto* hargs_d = reinterpret_cast<const HamTileArgs*>(tab + h_at);
+    const auto* prefix_d = reinterpret_cast<const uint64_t*>(tab + p_at);
+
+    auto run_tile_steps = [&](const std::vector<std::vector<StepGroup<TilePassArgs>>>& steps, bool adjoint) {
+        for (auto& v : steps)
+            for (auto& g : v) {
+                const size_t lds = tile_pass_lds_bytes(dtype, adjoint, g.K, g.C, g.max_taps);
+                hip_check(launch_tile_pass_items(dtype, adjoint, g.full, targs_d + g.args_at, prefix_d + g.prefix_at,
+                                                 int(g.args.size()), g.blocks, g.K, lds, s),
+                          "tile pass (items)");
+            }
+    };
+    run_tile_steps(fwd_steps, false);
+    for (auto& v : ham_steps)
+        for (auto& g : v)
+            hip_check(launch_ham_tile_items(dtype, g.first, g.last, hargs_d + g.args_at, prefix_d + g.prefix_at,
+                                            int(g.args.size()), g.blocks, g.K, g.C, s),
+                      "hamiltonian (items)");
+    for (auto& it : items) {  // flip masks wider than a tile: gather kernel, one launch per item
+        if (it.hp) continue;
+        auto* ham = const_cast<mqa_ham*>(hams[it.index]);
+        DevBuf* hd = ham->on_device(s);
+        HamArgs h{};
+        h.ket = base + it.psi * amp;
+        h.ket_stride = it.dim;
+        h.bra = h.ket;
+        h.bra_stride = it.dim;
+        h.out = base + it.lam * amp;
+        h.out_stride = it.dim;
+        h.dim = it.dim;
+        h.amps_per_block = 2048;
+        h.n_blocks = uint32_t(it.f_len);
+        h.groups = hd[0].as<DevHamGroup>();
+        h.n_groups = int(ham->groups.size());
+        h.terms = hd[1].as<DevHamTerm>();
+        h.partial = fpart_d.as<double>() + it.fpart;
+        hip_check(launch_ham(dtype, true, h, 1, s), "hamiltonian (gather)");
+    }
+    run_tile_steps(adj_steps, true);
+
+    // taps: one segment per (item, tap) over that item's tiles
+    std::vector<uint64_t> seg;  // [begin..., len...]
+    for (auto& it : items) {
+        const uint64_t tiles = uint64_t(1) << (it.n_eff - it.adj->plan.K);
+        for (int t = 0; t < circs[it.index]->n_tap_ids; t++) seg.push_back(it.taps_part + uint64_t(t) * tiles);
+    }
+    const size_t n_seg = seg.size();
+    for (auto& it : items) {
+        const uint64_t tiles = uint64_t(1) << (it.n_eff - it.adj->plan.K);
+        for (int t = 0; t < circs[it.index]->n_tap_ids; t++) seg.push_back(tiles);
+    }
+    DevBuf seg_d;
+    upload(seg_d, seg, s);
+    hip_check(launch_reduce_segments(part_d.as<double>(), seg_d.as<uint64_t>(), seg_d.as<uint64_t>() + n_seg,
+                                     taps_d.as<double>(), n_seg, s),
+              "reduce taps");
+    std::vector<double> taps_h(std::max<uint64_t>(n_taps, 1)), fpart_h(std::max<uint64_t>(n_fpart, 1));
+    hip_check(hipMemcpyAsync(taps_h.data(), taps_d.p, n_taps * 8, hipMemcpyDeviceToHost, s), "d2h taps");
+    hip_check(hipMemcpyAsync(fpart_h.data(), fpart_d.p, n_fpart * 8, hipMemcpyDeviceToHost, s), "d2h f");
+    hip_check(hipStreamSynchronize(s), "grad items");
+
+    // f = sum of the partials; g = jacobian^T * 2 Re(taps) (gradient.h:124-133, 183-188)
+    for (auto& it : items) {
+        const mqa_circuit* c = circs[it.index];
+        double fr = 0, fi = 0;
+        for (uint64_t k = 0; k < it.f_len; k++) fr += fpart_h[it.fpart + 2 * k], fi += fpart_h[it.fpart + 2 * k + 1];
+        f_out[2 * it.index] = fr;
+        f_out[2 * it.index + 1] = fi;
+        double* go = g_out + param_offset[it.index];
+        for (int p = 0; p < c->n_params; p++) {
+            double acc = 0;
+            for (int k = c->jac_begin[size_t(p)]; k < c->jac_begin[size_t(p) + 1]; k++)
+                acc += c->jac_coeff[size_t(k)] * 2.0 * taps_h[it.taps + uint64_t(c->jac_tap[size_t(k)])];
+            go[p] = acc;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
+                   const int32_t* n_qubits, const double* params, const int64_t* param_offset, int32_t dtype,
+                   double* f_out, double* g_out) {
+    return guarded([&] {
+        require(n_items >= 0, "n_items must be >= 0");
+        if (n_items == 0) return;
+        require(circs && hams && n_qubits && param_offset && f_out && g_out, "NULL argument");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "dtype must be MQA_C64 or MQA_C128");
+        if (mqa_device_count() <= 0) throw Error(MQA_ERR_HIP, "no HIP device is visible");
+        StreamOwner so;
+        const size_t amp = dtype == MQA_C64 ? 8 : 16;
+        std::vector<GradItem> all;
+        for (int i = 0; i < n_items; i++) {
+            auto* c = const_cast<mqa_circuit*>(circs[i]);
+            const mqa_ham* h = hams[i];
+            require(c != nullptr && h != nullptr, "item " + std::to_string(i) + ": NULL circuit or hamiltonian");
+            const int n = n_qubits[i];
+            require(n >= 1 && n <= 36, "item " + std::to_string(i) + ": bad qubit count");
+            require(c->max_qubit < n, "item " + std::to_string(i) + ": circuit is wider than its qubit count");
+            require((h->qubit_mask >> n) == 0, "item " + std::to_string(i) + ": hamiltonian is wider than its qubit count");
+            require(c->n_params == 0 || params != nullptr, "params is NULL");
+            GradItem it;
+            it.index = i;
+            it.n_eff = n_eff_of(n);
+            it.dim = uint64_t(1) << it.n_eff;
+            it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true);
+            it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true);
+            const int K = int(dtype == MQA_C64 ? g_opt.ham_tile_c64.load() : std::min<int64_t>(g_opt.ham_tile_c128, 13));
+            const int C = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
+            if (g_opt.ham_engine) {
+                auto* tp = const_cast<mqa_ham*>(h)->tile_plan(it.n_eff, K, C, so.s);
+                it.hp = tp->ok ? tp : nullptr;
+            }
+            const uint64_t adj_tiles = uint64_t(1) << (it.n_eff - it.adj->plan.K);
+            it.bytes = 2 * it.dim * amp + uint64_t(c->n_tap_ids) * adj_tiles * 8 + uint64_t(c->n_tap_ids) * 8 +
+                       uint64_t(c->n_slots) * 16 + 2 * ((it.dim + 2047) / 2048 + 1) * 8 + 4 * sizeof(TilePassArgs) *
+                       (it.fwd->plan.passes.size() + it.adj->plan.passes.size() + 4);
+            all.push_back(it);
+        }
+        // chunks that fit the HBM budget, largest states first
+        std::stable_sort(all.begin(), all.end(), [](const GradItem& a, const GradItem& b) { return a.n_eff > b.n_eff; });
+        const size_t budget = hbm_budget();
+        for (size_t a = 0; a < all.size();) {
+            size_t b = a, used = 0;
+            while (b < all.size() && (b == a || used + all[b].bytes <= budget)) used += all[b++].bytes;
+            std::vector<GradItem> chunk(all.begin() + long(a), all.begin() + long(b));
+            run_grad_chunk(chunk, circs, hams, params, param_offset, dtype, so.s, f_out, g_out);
+            a = b;
         }
     });
 }

@@ -557,8 +557,8 @@ struct TileFrame {
 
 // Build the offset table and stage the tile (psi, and lambda for the back-sweep) in LDS.
 template <typename T, bool ADJ>
-__device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass,
-                                                   unsigned char* smem) {
+__device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass, unsigned char* smem,
+                                                   uint64_t tile, uint64_t sample) {
     using V = V2<T>;
     TileFrame<T> f;
     const int K = args.K;
@@ -571,8 +571,8 @@ __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, con
     f.t_lam = reinterpret_cast<V*>(smem + lay.lam);  // used by the adjoint pass only
     f.offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
     f.lds_taps = reinterpret_cast<double*>(smem + lay.taps);
-    f.sample = blockIdx.y;
-    f.tile = blockIdx.x;
+    f.sample = sample;
+    f.tile = tile;
     f.base = pdep64(f.tile, pass.nonlocal_mask);
     for (uint32_t h = f.tid; h < (1u << H); h += f.NT) f.offtab[h] = pdep64(h, pass.local_hi_mask);
     __syncthreads();
@@ -585,6 +585,25 @@ __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, con
     f.n_waves = int((f.NT + 63) >> 6);
     f.wave_width = f.NT < 64 ? int(f.NT) : 64;
     return f;
+}
+
+// The usual mapping: blockIdx.x = tile, blockIdx.y = sample.
+template <typename T, bool ADJ>
+__device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass,
+                                                   unsigned char* smem) {
+    return tile_begin<T, ADJ>(args, pass, smem, blockIdx.x, blockIdx.y);
+}
+
+// Launches over the tiles of several independent states (items of different sizes and
+// schedules): tile_prefix[i] = first block of item i (ascending, n_items entries).
+__device__ __forceinline__ int item_of_block(const uint64_t* __restrict__ tile_prefix, int n_items, uint64_t b) {
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Write the tile back and emit this tile's partial of every tap of the pass.

@@ -29,10 +29,10 @@ namespace mqa {
 // The interpreted pass: walks the group and op tables of any plan.  Circuits that run often
 // are also compiled into straight-line kernels of the same structure (jit.cpp).
 template <typename T, bool ADJ, bool FULL>
-__global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigned char* smem, uint64_t tile,
+                                               uint64_t sample) {
     const DevPass pass = args.passes[args.pass_index];
-    TileFrame<T> f = tile_begin<T, ADJ>(args, pass, smem);
+    TileFrame<T> f = tile_begin<T, ADJ>(args, pass, smem, tile, sample);
     const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;
 
     for (int gi = 0; gi < pass.n_groups; gi++) {
@@ -91,6 +91,23 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
         __syncthreads();
     }
     tile_end<T, ADJ>(args, pass, f);
+}
+
+template <typename T, bool ADJ, bool FULL>
+__global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    tile_pass_body<T, ADJ, FULL>(args, smem, blockIdx.x, blockIdx.y);
+}
+
+// One pass step over several independent single-sample states (uneven circuits): block b runs
+// tile b - tile_prefix[i] of item i.  Each item carries its own schedule, buffers and pass index.
+template <typename T, bool ADJ, bool FULL>
+__global__ void __launch_bounds__(512) k_tile_pass_items(const TilePassArgs* __restrict__ items,
+                                                         const uint64_t* __restrict__ tile_prefix, int n_items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int i = item_of_block(tile_prefix, n_items, blockIdx.x);
+    const TilePassArgs args = items[i];
+    tile_pass_body<T, ADJ, FULL>(args, smem, uint64_t(blockIdx.x) - tile_prefix[i], 0);
 }
 
 // ---------------------------------------------------------------- Hamiltonian / reductions
@@ -212,8 +229,8 @@ constexpr int ham_tile_threads() {
 }
 
 template <typename T, bool FIRST, bool LAST>
-__global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile(HamTileArgs args) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void ham_tile_body(const HamTileArgs& args, unsigned char* smem, uint64_t tile,
+                                              uint64_t sample) {
     using V = V2<T>;
     constexpr int NA = kAmpsPerThread;
     const int K = args.K;
@@ -226,8 +243,6 @@ __global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile(HamTileArgs 
     V* t_ket = reinterpret_cast<V*>(smem + lay.psi);
     uint64_t* offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
 
-    const uint64_t sample = blockIdx.y;
-    const uint64_t tile = blockIdx.x;
     const uint64_t base = pdep64(tile, pass.nonlocal_mask);
     for (uint32_t h = tid; h < (1u << H); h += NT) offtab[h] = pdep64(h, pass.local_hi_mask);
     __syncthreads();
@@ -341,9 +356,26 @@ __global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile(HamTileArgs 
                 fr += double(x.x * ar[J] + x.y * ai[J]);
                 fi += double(x.x * ai[J] - x.y * ar[J]);
             }
-            block_sum2(fr, fi, args.partial + 2 * (sample * gridDim.x + tile));
+            block_sum2(fr, fi, args.partial + 2 * (sample * args.n_tiles + tile));
         }
     }
+}
+
+template <typename T, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile(HamTileArgs args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    ham_tile_body<T, FIRST, LAST>(args, smem, blockIdx.x, blockIdx.y);
+}
+
+// One Hamiltonian pass step over several independent single-sample states (see k_tile_pass_items).
+template <typename T, bool FIRST, bool LAST>
+__global__ void __launch_bounds__(ham_tile_threads<T>()) k_ham_tile_items(const HamTileArgs* __restrict__ items,
+                                                                       const uint64_t* __restrict__ tile_prefix,
+                                                                       int n_items) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int i = item_of_block(tile_prefix, n_items, blockIdx.x);
+    const HamTileArgs args = items[i];
+    ham_tile_body<T, FIRST, LAST>(args, smem, uint64_t(blockIdx.x) - tile_prefix[i], 0);
 }
 
 template <typename T>
@@ -434,6 +466,27 @@ __global__ void __launch_bounds__(256) k_reduce_rows1(const double* __restrict__
     if (lane == 0) out[row] = a;
 }
 
+// base[offset[i]] = 1 for every item (states already zeroed): |0...0> for a set of items
+template <typename T>
+__global__ void __launch_bounds__(256) k_unit_items(V2<T>* base, const uint64_t* __restrict__ offset, int n) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) base[offset[i]] = V2<T>{T(1), T(0)};
+}
+
+// out[k] = sum of in[begin[k] .. begin[k] + len[k]) in double, one wave per segment, fixed order
+__global__ void __launch_bounds__(256) k_reduce_segments(const double* __restrict__ in,
+                                                         const uint64_t* __restrict__ begin,
+                                                         const uint64_t* __restrict__ len, double* __restrict__ out,
+                                                         uint64_t n_segments) {
+    const uint64_t k = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (k >= n_segments) return;
+    const int lane = threadIdx.x & 63;
+    double a = 0;
+    for (uint64_t j = lane; j < len[k]; j += 64) a += in[begin[k] + j];
+    a = wave_sum(a, 64);
+    if (lane == 0) out[k] = a;
+}
+
 // g[b][p] = sum_{e in param p} coeff_e * 2 Re(tap[b][tap_e]); taps hold Re(tap) only
 __global__ void __launch_bounds__(256) k_jacobian(const double* __restrict__ taps, int n_tap_ids,
                                                  const int32_t* __restrict__ jac_begin,
@@ -486,6 +539,38 @@ hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& 
     return launch_tp_t<double>(adj, full, a, batch, lds, s);
 }
 
+template <typename T, bool ADJ, bool FULL>
+static hipError_t launch_tpi(const TilePassArgs* items, const uint64_t* prefix, int n_items, uint64_t blocks,
+                             uint32_t threads, size_t lds, hipStream_t s) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass_items<T, ADJ, FULL>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsLimit));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_tile_pass_items<T, ADJ, FULL>), dim3(uint32_t(blocks)), dim3(threads), lds, s, items, prefix,
+                       n_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, const TilePassArgs* items, const uint64_t* prefix,
+                                  int n_items, uint64_t blocks, int K, size_t lds, hipStream_t s) {
+    if (lds > kLdsLimit || blocks == 0 || blocks > 0x7fffffffull || K < kMinTileQubits || K > kMaxTileQubits)
+        return hipErrorInvalidValue;
+    const uint32_t threads = 1u << (K - kGroupWidth);
+    if (dtype == 0) {
+        if (adj) return full ? launch_tpi<float, true, true>(items, prefix, n_items, blocks, threads, lds, s)
+                             : launch_tpi<float, true, false>(items, prefix, n_items, blocks, threads, lds, s);
+        return full ? launch_tpi<float, false, true>(items, prefix, n_items, blocks, threads, lds, s)
+                    : launch_tpi<float, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+    }
+    if (adj) return full ? launch_tpi<double, true, true>(items, prefix, n_items, blocks, threads, lds, s)
+                         : launch_tpi<double, true, false>(items, prefix, n_items, blocks, threads, lds, s);
+    return full ? launch_tpi<double, false, true>(items, prefix, n_items, blocks, threads, lds, s)
+                : launch_tpi<double, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+}
+
 hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s) {
     dim3 grid(uint32_t(a.n_blocks), uint32_t(batch));
     if (dtype == 0) {
@@ -523,6 +608,59 @@ static hipError_t launch_ht_t(bool first, bool last, const HamTileArgs& a, uint6
                            : launch_ht<T, true, false>(a, n_tiles, batch, lds, s);
     return last ? launch_ht<T, false, true>(a, n_tiles, batch, lds, s)
                 : launch_ht<T, false, false>(a, n_tiles, batch, lds, s);
+}
+
+template <typename T, bool FIRST, bool LAST>
+static hipError_t launch_hti(const HamTileArgs* items, const uint64_t* prefix, int n_items, uint64_t blocks,
+                             uint32_t threads, size_t lds, hipStream_t s) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ham_tile_items<T, FIRST, LAST>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           int(kLdsLimit - kBlockSumLds));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    if (threads > uint32_t(ham_tile_threads<T>())) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_ham_tile_items<T, FIRST, LAST>), dim3(uint32_t(blocks)), dim3(threads), lds, s, items,
+                       prefix, n_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_ham_tile_items(int dtype, bool first, bool last, const HamTileArgs* items, const uint64_t* prefix,
+                                 int n_items, uint64_t blocks, int K, int C, hipStream_t s) {
+    const size_t amp = dtype == 0 ? 8 : 16;
+    const size_t lds = lds_layout(K, C, false, int(amp), 0).total;
+    if (K < kMinTileQubits || K > kMaxHamTileQubits || lds + kBlockSumLds > kLdsLimit || blocks == 0 ||
+        blocks > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    const uint32_t threads = 1u << (K - kGroupWidth);
+    if (dtype == 0) {
+        if (first) return last ? launch_hti<float, true, true>(items, prefix, n_items, blocks, threads, lds, s)
+                               : launch_hti<float, true, false>(items, prefix, n_items, blocks, threads, lds, s);
+        return last ? launch_hti<float, false, true>(items, prefix, n_items, blocks, threads, lds, s)
+                    : launch_hti<float, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+    }
+    if (first) return last ? launch_hti<double, true, true>(items, prefix, n_items, blocks, threads, lds, s)
+                           : launch_hti<double, true, false>(items, prefix, n_items, blocks, threads, lds, s);
+    return last ? launch_hti<double, false, true>(items, prefix, n_items, blocks, threads, lds, s)
+                : launch_hti<double, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+}
+
+hipError_t launch_unit_items(int dtype, void* base, const uint64_t* offset, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const uint32_t blocks = uint32_t((n + 255) / 256);
+    if (dtype == 0) hipLaunchKernelGGL(k_unit_items<float>, dim3(blocks), dim3(256), 0, s, static_cast<float2*>(base), offset, n);
+    else hipLaunchKernelGGL(k_unit_items<double>, dim3(blocks), dim3(256), 0, s, static_cast<double2*>(base), offset, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_segments(const double* in, const uint64_t* begin, const uint64_t* len, double* out,
+                                  uint64_t n_segments, hipStream_t s) {
+    if (n_segments == 0) return hipSuccess;
+    const uint32_t blocks = uint32_t((n_segments + 3) / 4);
+    hipLaunchKernelGGL(k_reduce_segments, dim3(blocks), dim3(256), 0, s, in, begin, len, out, n_segments);
+    return hipGetLastError();
 }
 
 hipError_t launch_ham_tile(int dtype, bool first, bool last, const HamTileArgs& a, int C, uint64_t n_tiles,

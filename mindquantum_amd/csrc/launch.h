@@ -40,6 +40,7 @@ struct HamTileArgs {
     const DevHamTileGroup* groups;
     const DevHamTileTerm* terms;
     double* partial;  // [batch][n_tiles] (re, im) or NULL
+    uint64_t n_tiles; // tiles per sample (partial row length)
 };
 
 struct VdotArgs {
@@ -74,6 +75,15 @@ hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& 
 hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s);
 hipError_t launch_ham_tile(int dtype, bool first, bool last, const HamTileArgs& a, int C, uint64_t n_tiles,
                            uint64_t batch, hipStream_t s);
+// Several independent single-sample states in one launch: items (device array) with ascending
+// first-block offsets prefix (device array, n_items entries); blocks = total tiles.
+hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, const TilePassArgs* items, const uint64_t* prefix,
+                                  int n_items, uint64_t blocks, int K, size_t lds, hipStream_t s);
+hipError_t launch_ham_tile_items(int dtype, bool first, bool last, const HamTileArgs* items, const uint64_t* prefix,
+                                 int n_items, uint64_t blocks, int K, int C, hipStream_t s);
+hipError_t launch_unit_items(int dtype, void* base, const uint64_t* offset, int n, hipStream_t s);
+hipError_t launch_reduce_segments(const double* in, const uint64_t* begin, const uint64_t* len, double* out,
+                                  uint64_t n_segments, hipStream_t s);
 hipError_t launch_vdot(int dtype, const VdotArgs& a, uint64_t batch, hipStream_t s);
 hipError_t launch_fill(int dtype, void* dst, const void* src, uint64_t dim, uint64_t stride, uint64_t src_dim,
                        uint64_t batch, hipStream_t s);

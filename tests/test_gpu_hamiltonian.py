@@ -129,3 +129,26 @@ def test_grad_with_tile_hamiltonian(n, batch, dtype):
     g = np.concatenate([g_enc, g_ans], axis=2)
     scale = max(1.0, float(np.max(np.abs(g_ref))))
     assert np.max(np.abs(g - g_ref)) < tol * scale * 4
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_grad_recomputes_forward_when_copy_exceeds_budget(dtype):
+    """Several Hamiltonians with an HBM budget below three states per sample: the forward state is
+    recomputed per Hamiltonian instead of kept; results equal the unconstrained run."""
+    n, batch = 16, 3
+    circ = hea(n, 3, encoder=True)
+    ops = [ring_hamiltonian(n, 1), random_hamiltonian(n, 10, 2, max_weight=3), ring_hamiltonian(n, 3)]
+    hams = [mq.Hamiltonian(o, dtype) for o in ops]
+    rng = np.random.default_rng(5)
+    enc = rng.normal(size=(batch, n))
+    ans = rng.uniform(0, 2 * np.pi, size=len(circ.ansatz_params_name))
+    sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
+    f0, ge0, ga0 = sim.get_expectation_with_grad(hams, circ)(enc, ans)
+    state = (1 << n) * np.dtype(dtype).itemsize
+    before = _native.get_option("grad_recomputes")
+    with options(hbm_budget_bytes=int(2.5 * state)):
+        f1, ge1, ga1 = sim.get_expectation_with_grad(hams, circ)(enc, ans)
+    # the budget holds one sample (two states) per chunk: every sample recomputes for ops 2 and 3
+    assert _native.get_option("grad_recomputes") == before + batch * (len(ops) - 1)
+    tol = 1e-12 if dtype == mq.complex128 else 1e-6
+    assert np.max(np.abs(f1 - f0)) < tol and np.max(np.abs(ge1 - ge0)) < tol and np.max(np.abs(ga1 - ga0)) < tol
