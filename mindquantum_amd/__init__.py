@@ -5,8 +5,9 @@ get_expectation, get_expectation_with_grad); every gate, expectation and adjoint
 in the HIP library ``libmqamd.so`` (C ABI in include/mqamd.h).
 """
 from . import dtype
-from .core import (CNOT, H, I, ISWAP, RX, RY, RZ, SWAP, SX, UN, Circuit, GlobalPhase, Hamiltonian, ParameterResolver,
-                   PhaseShift, QubitOperator, Rxx, Ryy, Rzz, S, T, UnivMathGate, X, Y, Z)
+from .core import (CNOT, FSIM, H, I, ISWAP, RX, RY, RZ, SWAP, SX, U3, UN, BarrierGate, Circuit, FSim, Givens,
+                   GlobalPhase, GroupedPauli, Hamiltonian, ParameterResolver, PhaseShift, Power, QubitOperator, Rn,
+                   RotPauliString, Rxx, Rxy, Rxz, Ryy, Ryz, Rzz, S, SWAPalpha, T, UnivMathGate, X, Y, Z)
 from .core import gates
 from .dtype import complex64, complex128
 from .simulator import Simulator, expectation_with_grad_batch, inner_product

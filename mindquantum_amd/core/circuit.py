@@ -89,8 +89,7 @@ class Circuit(list):
     def _names(self, pred):
         out = _NameList()
         for g in self:
-            pr = getattr(g, "coeff", None)
-            if isinstance(pr, ParameterResolver):
+            for pr in g.param_resolvers():
                 for k in pr.keys():
                     if pred(pr, k) and k not in out:
                         out.append(k)
@@ -131,8 +130,7 @@ class Circuit(list):
     # ---- parameter flags
     def _each_pr(self, fn):
         for g in self:
-            pr = getattr(g, "coeff", None)
-            if isinstance(pr, ParameterResolver):
+            for pr in g.param_resolvers():
                 fn(pr)
 
     def as_encoder(self, inplace=True):

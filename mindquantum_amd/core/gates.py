@@ -1,8 +1,11 @@
 """Quantum gates of the state-vector path.
 
 Host-side mirror of mindquantum/core/gates/basicgate.py for the gates the backend executes:
-fixed gates (X, Y, Z, H, S, T, SX, I, SWAP, ISWAP, CNOT, UnivMathGate) and one-parameter
-rotations (RX, RY, RZ, PhaseShift, GlobalPhase, Rxx, Ryy, Rzz).  Matrix conventions follow
+fixed gates (X, Y, Z, H, S, T, SX, I, SWAP, ISWAP, CNOT, UnivMathGate, GroupedPauli, Power,
+Barrier), one-parameter rotations (RX, RY, RZ, PhaseShift, GlobalPhase, Rxx, Ryy, Rzz, Rxy, Rxz,
+Ryz, RotPauliString, Givens, SWAPalpha) and multi-parameter gates (U3, FSim, constant Rn).  Gates
+the kernels do not run directly are lowered (``lowered()``) into an exact product of gates they do
+run, with the same controls and affine parameter expressions, so derivative taps stay exact.  Matrix conventions follow
 the reference: the matrix of a multi-qubit gate is little-endian in ``obj_qubits`` order
 (bit i of the matrix index <=> obj_qubits[i], cpu_vector_core_matrix_gate.cpp:85-149), and
 parameterised gates take the hermitian by negating the parameter (core/gates/basic.py:510-514).
@@ -57,6 +60,15 @@ class BasicGate:
 
     def hermitian(self):
         return copy.deepcopy(self)
+
+    def param_resolvers(self):
+        """The gate's parameter expressions (one per parameter slot of its matrix)."""
+        return []
+
+    def lowered(self):
+        """The gate as a sequence of gates the backend executes directly (the gate itself for
+        those); composite gates return an exact decomposition with the same controls."""
+        return [self]
 
     @property
     def acted(self):
@@ -237,6 +249,13 @@ class ParameterGate(BasicGate):
         """Derivative of the matrix with respect to the angle value."""
         return self._dmat(self._value(pr))
 
+    def param_resolvers(self):
+        return [self.coeff]
+
+    def diff_matrices(self, pr=None):
+        """Derivative of the matrix with respect to each parameter expression's value."""
+        return [self.diff_matrix(pr)]
+
     def no_grad(self):
         self.coeff.no_grad()
         return self
@@ -355,6 +374,360 @@ class GlobalPhase(ParameterGate):
         return -1j * np.exp(-1j * th) * np.eye(2, dtype=np.complex128)
 
 
+_PAULI_OF = {"X": _PX, "Y": _PY, "Z": _PZ, "I": np.eye(2, dtype=np.complex128)}
+
+
+def _kron_string(paulis):
+    """Matrix of a Pauli string, paulis[i] acting on matrix bit i (little-endian)."""
+    m = np.ones((1, 1), dtype=np.complex128)
+    for p in reversed(paulis):
+        m = np.kron(m, _PAULI_OF[p])
+    return m
+
+
+def _basis_to_z(p, q, inverse):
+    """Fixed gate that maps Pauli p on qubit q to Z (inverse: back), or None for Z.
+    H X H = Z; RX(pi/2) Y RX(-pi/2) = Z."""
+    if p == "X":
+        return H.on(q)
+    if p == "Y":
+        return RX(-np.pi / 2 if inverse else np.pi / 2).on(q)
+    return None
+
+
+def _pauli_rotation(paulis, objs, ctrl, pr):
+    """exp(-i pr/2 P) for the Pauli string P = paulis[i] on objs[i] (identities dropped): basis
+    change to Z...Z, then RZ / Rzz on the parity (CNOT ladder for more than two qubits).  Only the
+    rotation carries the controls: V^dag C(R) V is C(V^dag R V) for any V."""
+    act = [(p, q) for p, q in zip(paulis, objs) if p != "I"]
+    if not act:
+        return [GlobalPhase(pr * 0.5).on(objs[0], ctrl)]
+    if len(act) == 1:
+        p, q = act[0]
+        return [{"X": RX, "Y": RY, "Z": RZ}[p](pr).on(q, ctrl)]
+    pre = [g for g in (_basis_to_z(p, q, False) for p, q in act) if g is not None]
+    post = [g for g in (_basis_to_z(p, q, True) for p, q in act) if g is not None]
+    qs = [q for _, q in act]
+    if len(qs) == 2:
+        core = [Rzz(pr).on(qs, ctrl)]
+    else:
+        ladder = [X.on(qs[i + 1], qs[i]) for i in range(len(qs) - 2)]
+        core = ladder + [Rzz(pr).on(qs[-2:], ctrl)] + ladder[::-1]
+    return pre + core + post
+
+
+class PauliRotationGate(ParameterGate):
+    """exp(-i th/2 P) for a fixed Pauli string P (``_paulis[i]`` on obj_qubits[i])."""
+
+    _paulis = ""
+
+    def _mat(self, th):
+        return _rot(_kron_string(self._paulis), th)
+
+    def _dmat(self, th):
+        return _drot(_kron_string(self._paulis), th)
+
+    def lowered(self):
+        out = []
+        for g in _pauli_rotation(self._paulis, self.obj_qubits, self.ctrl_qubits, self.coeff):
+            out.extend(g.lowered())
+        return out
+
+
+class Rxy(PauliRotationGate):
+    """exp(-i th/2 X_{q0} Y_{q1}) (basicgate.py:1201-1281, core PauliStringGate([X, Y]))."""
+
+    _paulis = "XY"
+
+    def __init__(self, pr):
+        super().__init__("Rxy", 2, pr)
+
+
+class Rxz(PauliRotationGate):
+    """exp(-i th/2 X_{q0} Z_{q1}) (basicgate.py:1283-1361)."""
+
+    _paulis = "XZ"
+
+    def __init__(self, pr):
+        super().__init__("Rxz", 2, pr)
+
+
+class Ryz(PauliRotationGate):
+    """exp(-i th/2 Y_{q0} Z_{q1}) (basicgate.py:1363-1441)."""
+
+    _paulis = "YZ"
+
+    def __init__(self, pr):
+        super().__init__("Ryz", 2, pr)
+
+
+class RotPauliString(PauliRotationGate):
+    """exp(-i th/2 P) for an arbitrary Pauli string, character i on obj_qubits[i]
+    (basicgate.py:1567-1674)."""
+
+    def __init__(self, pauli_string, pr):
+        if not isinstance(pauli_string, str) or any(c.upper() not in "IXYZ" for c in pauli_string):
+            raise ValueError(f"Unknown pauli string {pauli_string!r}")
+        self._paulis = pauli_string.upper()
+        super().__init__("RPS", len(pauli_string), pr)
+
+
+class Givens(ParameterGate):
+    """[[1,0,0,0],[0,cos,-sin,0],[0,sin,cos,0],[0,0,0,1]] = exp(-i th/2 (Y X - X Y))
+    (basicgate.py:617-704).  Y X (kron) is X on q0, Y on q1; the two terms commute, so the gate
+    is Rxy(th) on (q0, q1) followed by Rxy(-th) on (q1, q0)."""
+
+    def __init__(self, pr):
+        super().__init__("Givens", 2, pr)
+
+    def _mat(self, th):
+        c, s = np.cos(th), np.sin(th)
+        return np.array([[1, 0, 0, 0], [0, c, -s, 0], [0, s, c, 0], [0, 0, 0, 1]], dtype=np.complex128)
+
+    def _dmat(self, th):
+        c, s = np.cos(th), np.sin(th)
+        return np.array([[0, 0, 0, 0], [0, -s, -c, 0], [0, c, -s, 0], [0, 0, 0, 0]], dtype=np.complex128)
+
+    def lowered(self):
+        a, b = self.obj_qubits
+        return Rxy(self.coeff).on([a, b], self.ctrl_qubits).lowered() + \
+            Rxy(-self.coeff).on([b, a], self.ctrl_qubits).lowered()
+
+
+class SWAPalpha(ParameterGate):
+    """SWAP^alpha: e = exp(i pi alpha), [[1,0,0,0],[0,(1+e)/2,(1-e)/2,0],[0,(1-e)/2,(1+e)/2,0],
+    [0,0,0,1]] (basicgate.py:849-937).  With SWAP = (I + XX + YY + ZZ)/2 it equals
+    exp(i pi alpha/4) exp(-i pi alpha/4 XX) exp(-i pi alpha/4 YY) exp(-i pi alpha/4 ZZ), i.e.
+    GlobalPhase(-pi alpha/4) Rxx(pi alpha/2) Ryy(pi alpha/2) Rzz(pi alpha/2)."""
+
+    def __init__(self, pr):
+        super().__init__("SWAPalpha", 2, pr)
+
+    def _mat(self, a):
+        e = np.exp(1j * np.pi * a)
+        return np.array([[1, 0, 0, 0], [0, (1 + e) / 2, (1 - e) / 2, 0], [0, (1 - e) / 2, (1 + e) / 2, 0],
+                         [0, 0, 0, 1]], dtype=np.complex128)
+
+    def _dmat(self, a):
+        e = 0.5j * np.pi * np.exp(1j * np.pi * a)
+        return np.array([[0, 0, 0, 0], [0, e, -e, 0], [0, -e, e, 0], [0, 0, 0, 0]], dtype=np.complex128)
+
+    def lowered(self):
+        half = self.coeff * (np.pi / 2)
+        q, c = self.obj_qubits, self.ctrl_qubits
+        return [GlobalPhase(self.coeff * (-np.pi / 4)).on(q[0], c), Rxx(half).on(q, c), Ryy(half).on(q, c),
+                Rzz(half).on(q, c)]
+
+
+class MultiParamsGate(BasicGate):
+    """Gate whose matrix depends on several affine parameter expressions ``prs``
+    (basicgate.py:1962-2230)."""
+
+    def __init__(self, name, n_qubits, prs):
+        super().__init__(name, n_qubits)
+        self.prs = []
+        for pr in prs:
+            if not isinstance(pr, (ParameterResolver, str, dict, numbers.Number)):
+                raise TypeError(f"Unsupported parameter type {type(pr)}")
+            p = ParameterResolver(pr)
+            if p.is_complex:
+                raise ValueError(f"{name} gate only supports real parameters")
+            self.prs.append(p)
+
+    @property
+    def parameterized(self):
+        return any(not p.is_const() for p in self.prs)
+
+    def param_resolvers(self):
+        return self.prs
+
+    def _values(self, pr):
+        out = []
+        for p in self.prs:
+            if p.is_const():
+                out.append(float(np.real(p.const)))
+            elif pr is None:
+                raise ValueError("parameterized gate needs a parameter resolver")
+            else:
+                out.append(float(np.real(p.combination(pr))))
+        return out
+
+    def matrix(self, pr=None):
+        return self._mat(*self._values(pr))
+
+    def diff_matrices(self, pr=None):
+        return self._dmats(*self._values(pr))
+
+    def no_grad(self):
+        for p in self.prs:
+            p.no_grad()
+        return self
+
+    def requires_grad(self):
+        for p in self.prs:
+            p.requires_grad()
+        return self
+
+    def __repr__(self):
+        s = self.name + "(" + ", ".join(p.expression() for p in self.prs) + ")"
+        if self.obj_qubits:
+            s += "(" + " ".join(map(str, self.obj_qubits))
+            if self.ctrl_qubits:
+                s += " <-: " + " ".join(map(str, self.ctrl_qubits))
+            s += ")"
+        return s
+
+
+class U3(MultiParamsGate):
+    """[[cos(t/2), -e^{i l} sin(t/2)], [e^{i p} sin(t/2), e^{i(p+l)} cos(t/2)]]
+    (basicgate.py:2370-2510) = PhaseShift(p) RY(t) PhaseShift(l)."""
+
+    def __init__(self, theta, phi, lamda):
+        super().__init__("U3", 1, [theta, phi, lamda])
+
+    @property
+    def theta(self):
+        return self.prs[0]
+
+    @property
+    def phi(self):
+        return self.prs[1]
+
+    @property
+    def lamda(self):
+        return self.prs[2]
+
+    def hermitian(self):
+        """U3(t, p, l)^dag = U3(-t, -l, -p) (basicgate.py:2445-2458)."""
+        out = U3(-self.prs[0], -self.prs[2], -self.prs[1])
+        out.obj_qubits, out.ctrl_qubits = list(self.obj_qubits), list(self.ctrl_qubits)
+        return out
+
+    def _mat(self, t, p, l):
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        return np.array([[c, -np.exp(1j * l) * s], [np.exp(1j * p) * s, np.exp(1j * (p + l)) * c]],
+                        dtype=np.complex128)
+
+    def _dmats(self, t, p, l):
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        el, ep, epl = np.exp(1j * l), np.exp(1j * p), np.exp(1j * (p + l))
+        d_t = np.array([[-s / 2, -el * c / 2], [ep * c / 2, -epl * s / 2]], dtype=np.complex128)
+        d_p = np.array([[0, 0], [1j * ep * s, 1j * epl * c]], dtype=np.complex128)
+        d_l = np.array([[0, -1j * el * s], [0, 1j * epl * c]], dtype=np.complex128)
+        return [d_t, d_p, d_l]
+
+    def lowered(self):
+        q, c = self.obj_qubits, self.ctrl_qubits
+        return [PhaseShift(self.prs[2]).on(q, c), RY(self.prs[0]).on(q, c), PhaseShift(self.prs[1]).on(q, c)]
+
+
+class FSim(MultiParamsGate):
+    """[[1,0,0,0],[0,cos t,-i sin t,0],[0,-i sin t,cos t,0],[0,0,0,e^{-i p}]]
+    (basicgate.py:2512-2600).  (XX + YY)/2 swaps |01> and |10> and annihilates |00>, |11>, so the
+    middle block is Rxx(t) Ryy(t); the phase on |11> is PhaseShift(-p) on q1 controlled by q0."""
+
+    def __init__(self, theta, phi):
+        super().__init__("FSim", 2, [theta, phi])
+
+    @property
+    def theta(self):
+        return self.prs[0]
+
+    @property
+    def phi(self):
+        return self.prs[1]
+
+    def hermitian(self):
+        out = FSim(-self.prs[0], -self.prs[1])
+        out.obj_qubits, out.ctrl_qubits = list(self.obj_qubits), list(self.ctrl_qubits)
+        return out
+
+    def _mat(self, t, p):
+        a, b = np.cos(t), -1j * np.sin(t)
+        return np.array([[1, 0, 0, 0], [0, a, b, 0], [0, b, a, 0], [0, 0, 0, np.exp(-1j * p)]], dtype=np.complex128)
+
+    def _dmats(self, t, p):
+        a, b = -np.sin(t), -1j * np.cos(t)
+        d_t = np.array([[0, 0, 0, 0], [0, a, b, 0], [0, b, a, 0], [0, 0, 0, 0]], dtype=np.complex128)
+        d_p = np.zeros((4, 4), dtype=np.complex128)
+        d_p[3, 3] = -1j * np.exp(-1j * p)
+        return [d_t, d_p]
+
+    def lowered(self):
+        q, c = self.obj_qubits, list(self.ctrl_qubits)
+        return [Rxx(self.prs[0]).on(q, c), Ryy(self.prs[0]).on(q, c), PhaseShift(-self.prs[1]).on(q[1], [q[0]] + c)]
+
+
+class Rn(MultiParamsGate):
+    """exp(-i (a X + b Y + c Z)/2) (basicgate.py:2232-2368).  The rotation axis depends on the
+    parameters non-linearly, so only constant Rn gates run here (as a dense matrix)."""
+
+    def __init__(self, alpha, beta, gamma):
+        super().__init__("Rn", 1, [alpha, beta, gamma])
+
+    def hermitian(self):
+        out = Rn(-self.prs[0], -self.prs[1], -self.prs[2])
+        out.obj_qubits, out.ctrl_qubits = list(self.obj_qubits), list(self.ctrl_qubits)
+        return out
+
+    def _mat(self, a, b, c):
+        f = np.sqrt(a * a + b * b + c * c)
+        if f == 0:
+            return np.eye(2, dtype=np.complex128)
+        gen = (a * _PX + b * _PY + c * _PZ) / f
+        return np.cos(f / 2) * np.eye(2) - 1j * np.sin(f / 2) * gen
+
+    def lowered(self):
+        if self.parameterized:
+            raise ValueError("Rn with variable parameters is not supported by the HIP backend "
+                             "(its rotation axis depends on the parameters non-linearly)")
+        m = UnivMathGate("Rn", self.matrix())
+        return [m.on(self.obj_qubits, self.ctrl_qubits)]
+
+
+class GroupedPauli(NoneParameterGate):
+    """Tensor product of Paulis, character i on obj_qubits[i] (basicgate.py:126-202)."""
+
+    def __init__(self, pauli_string):
+        if not isinstance(pauli_string, str) or any(c.upper() not in "IXYZ" for c in pauli_string):
+            raise ValueError(f"Unknown pauli string {pauli_string!r}")
+        self.pauli_string = pauli_string.upper()
+        super().__init__("GroupedPauli", len(pauli_string), _kron_string(self.pauli_string))
+
+    def lowered(self):
+        return [{"X": X, "Y": Y, "Z": Z}[p].on(q, self.ctrl_qubits)
+                for p, q in zip(self.pauli_string, self.obj_qubits) if p != "I"]
+
+
+class Power(UnivMathGate):
+    """A fixed gate's matrix raised to ``exponent`` (basicgate.py:1756-1800)."""
+
+    def __init__(self, gate, exponent=0.5):
+        from scipy.linalg import fractional_matrix_power
+
+        if not isinstance(gate, NoneParameterGate):
+            raise TypeError("Power needs a non-parameterized gate")
+        super().__init__(f"{gate.name}^{exponent}", fractional_matrix_power(gate.matrix(), exponent))
+        self.gate = gate
+        self.t = exponent
+
+
+class BarrierGate(BasicGate):
+    """No operation (basicgate.py:1443-1493)."""
+
+    def __init__(self, show=True):
+        super().__init__("BARRIER", 0)
+        self.show = show
+
+    def on(self, obj_qubits, ctrl_qubits=None):
+        new = copy.deepcopy(self)
+        new.obj_qubits = _qubits(obj_qubits)
+        return new
+
+    def lowered(self):
+        return []
+
+
 X = XGate()
 Y = YGate()
 Z = ZGate()
@@ -366,10 +739,13 @@ I = IGate()  # noqa: E741
 SWAP = SWAPGate()
 ISWAP = ISWAPGate()
 CNOT = CNOTGate()
+FSIM = FSim  # reference alias
 
 __all__ = [
     "BasicGate", "NoneParameterGate", "ParameterGate", "RotationGate",
     "X", "Y", "Z", "H", "S", "T", "SX", "I", "SWAP", "ISWAP", "CNOT",
     "XGate", "YGate", "ZGate", "HGate", "SGate", "TGate", "SXGate", "IGate", "SWAPGate", "ISWAPGate", "CNOTGate",
     "UnivMathGate", "RX", "RY", "RZ", "Rxx", "Ryy", "Rzz", "PhaseShift", "GlobalPhase",
+    "PauliRotationGate", "Rxy", "Rxz", "Ryz", "RotPauliString", "Givens", "SWAPalpha", "MultiParamsGate", "U3", "FSim",
+    "Rn", "GroupedPauli", "Power", "BarrierGate", "FSIM",
 ]

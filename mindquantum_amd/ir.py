@@ -2,7 +2,9 @@
 
 A circuit is compiled against an ordered parameter-name list (encoder names first, then
 ansatz names, gradient.h:149-158): every parameterised gate gets ``angle = const +
-sum coeff * params[index]`` terms, fixed gates their kind or a dense matrix.
+sum coeff * params[index]`` terms, fixed gates their kind or a dense matrix.  Composite gates
+(U3, FSim, Givens, SWAPalpha, Rxy/Rxz/Ryz, RotPauliString, GroupedPauli) are expanded into
+their exact lowering first (``BasicGate.lowered``), so gate indices refer to that expansion.
 """
 import ctypes
 
@@ -16,10 +18,11 @@ from .core.parameterresolver import ParameterResolver
 def lower(circ, names):
     """Return (gates, term_param, term_coeff, mats, requires_grad_per_param) numpy arrays."""
     index = {n: i for i, n in enumerate(names)}
-    gates = np.zeros(len(circ), dtype=_native.GATE_DTYPE)
+    prims = [p for g in circ for p in g.lowered()]
+    gates = np.zeros(len(prims), dtype=_native.GATE_DTYPE)
     t_param, t_coeff, mats = [], [], []
     no_grad = set()
-    for i, g in enumerate(circ):
+    for i, g in enumerate(prims):
         rec = gates[i]
         rec["n_obj"] = len(g.obj_qubits)
         for k, q in enumerate(g.obj_qubits):

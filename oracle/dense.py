@@ -57,16 +57,19 @@ def embed_diff(dm, objs, ctrls, n):
 
 
 def gate_matrix(g, values):
-    from mindquantum_amd.core.gates import ParameterGate
-
-    if isinstance(g, ParameterGate):
+    """Closed-form matrix of the gate itself (never its lowering into other gates)."""
+    if g.param_resolvers():
         return g.matrix(values)
     return g.matrix()
 
 
+def _acting(circ):
+    return [g for g in circ if g.obj_qubits and getattr(g, "name", "") != "BARRIER"]
+
+
 def unitary(circ, values, n):
     u = np.eye(1 << n, dtype=np.complex128)
-    for g in circ:
+    for g in _acting(circ):
         u = embed(gate_matrix(g, values), g.obj_qubits, g.ctrl_qubits, n) @ u
     return u
 
@@ -100,9 +103,8 @@ def expectation(circ, values, n, qubit_operator, init=None):
 
 def gradient(circ, values, names, n, qubit_operator, init=None):
     """(f, dE/dparam for every name in ``names``) by the direct product formula."""
-    from mindquantum_amd.core.gates import ParameterGate
-
     dim = 1 << n
+    circ = _acting(circ)
     psi0 = np.zeros(dim, dtype=np.complex128)
     if init is None:
         psi0[0] = 1
@@ -124,11 +126,16 @@ def gradient(circ, values, names, n, qubit_operator, init=None):
     grad = np.zeros(len(names))
     idx = {nm: i for i, nm in enumerate(names)}
     for k, g in enumerate(circ):
-        if not isinstance(g, ParameterGate) or g.coeff.is_const():
+        prs = g.param_resolvers()
+        if not prs or all(pr.is_const() for pr in prs):
             continue
-        dm = embed_diff(g.diff_matrix(values), g.obj_qubits, g.ctrl_qubits, n)
-        val = bra @ (suffix[k + 1] @ (dm @ prefix[k]))
-        for name, c in g.coeff.items():
-            if name in idx and name in g.coeff.requires_grad_parameters:
-                grad[idx[name]] += 2 * np.real(val) * np.real(c)
+        # one derivative matrix per parameter expression; chain rule through its coefficients
+        for pr, dmat in zip(prs, g.diff_matrices(values)):
+            if pr.is_const():
+                continue
+            dm = embed_diff(dmat, g.obj_qubits, g.ctrl_qubits, n)
+            val = bra @ (suffix[k + 1] @ (dm @ prefix[k]))
+            for name, c in pr.items():
+                if name in idx and name in pr.requires_grad_parameters:
+                    grad[idx[name]] += 2 * np.real(val) * np.real(c)
     return f, grad
