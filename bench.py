@@ -7,9 +7,11 @@ qubit, CNOT ladder i -> i+1; 320 parameters ~ U[0, 2pi)), H = sum_i w_i Z_i Z_{i
 sum_i v_i X_i with w, v ~ U(-1, 1) (40 terms).  One step = one forward + backward of the
 torch.autograd operator (MQOps) over the whole batch; the adjoint sweep runs in the forward.
 
-Also reported in the same JSON line: the same workload in complex128, and 28-qubit gate
+Also reported in the same JSON line: the same workload in complex128; 28-qubit gate
 application (layered random circuit, depth 200, complex64 / complex128) in HBM GB/s, fused
-(tile passes) and unfused (one gate per pass).  ``roofline`` is the dominant kernel of the
+(tile passes) and unfused (one gate per pass); the uneven batch of 512 MaxCut QAOA circuits
+(12..24 qubits, p = 4, complex64, one mqa_grad_items call); one 32-qubit complex64 and one
+31-qubit complex128 gradient (HEA-4, 64 random Pauli strings, 4 checkpoints).  ``roofline`` is the dominant kernel of the
 headline (the adjoint tile pass) timed with HIP events on the library's stream;
 ``cpu_baseline`` times the CPU oracle (a restatement of the reference CPU mqvector path) on a
 bounded sample of the same workload on this host's cores.
@@ -146,6 +148,94 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth):
     return res
 
 
+def run_uneven_items(count, seed, cpu_sample):
+    """Uneven MaxCut batch: `count` QAOA circuits (p = 4) on 12..24 qubits, expectation and
+    adjoint gradient of every circuit in complex64 through one mqa_grad_items call (shared
+    per-step launches over the tiles of all circuits).  Optionally the CPU oracle on a subset
+    (circuits of at most 18 qubits) beside the GPU on the same subset."""
+    import mindquantum_amd as mq
+    from mindquantum_amd.algorithm import uneven_qaoa_batch
+
+    circs, ops, nqs, params = uneven_qaoa_batch(count, seed)
+    hams = [mq.Hamiltonian(o, mq.complex64) for o in ops]
+    # algorithmic bytes of the reference schedule per circuit: (90.5 n + 8) 2^n 8 B
+    ref_bytes = sum((90.5 * n + 8) * (1 << n) * 8 for n in nqs)
+    mq.expectation_with_grad_batch(circs, hams, params, nqs, mq.complex64)  # plans, code objects
+    log(f"uneven batch: warm-up done ({count} circuits)")
+    t0 = time.perf_counter()
+    f, grads = mq.expectation_with_grad_batch(circs, hams, params, nqs, mq.complex64)
+    dt = time.perf_counter() - t0
+    res = {"circuits": count, "qubits": [min(nqs), max(nqs)], "layers": 4, "seconds": round(dt, 4),
+           "circuits_per_s": round(count / dt, 1),
+           "reference_schedule_tb": round(ref_bytes / 1e12, 3),
+           "reference_schedule_equivalent_gbs": round(ref_bytes / dt / 1e9, 1)}
+    log(f"uneven batch: {res}")
+    if cpu_sample:
+        from mindquantum_amd import ir
+        from oracle import cpu
+
+        idx = []  # two circuits of every size from 12 to 20 qubits
+        for n in range(12, 21):
+            idx += [i for i, m in enumerate(nqs) if m == n][:2]
+        sub = ([circs[i] for i in idx], [hams[i] for i in idx], [params[i] for i in idx], [nqs[i] for i in idx])
+        mq.expectation_with_grad_batch(*sub, mq.complex64)
+        t0 = time.perf_counter()
+        mq.expectation_with_grad_batch(*sub, mq.complex64)
+        gpu_dt = time.perf_counter() - t0
+        cores = len(os.sched_getaffinity(0))
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        cpu.set_num_threads(cores)
+        t0 = time.perf_counter()
+        for i in idx:
+            names = list(circs[i].params_name)
+            cpu.grad_batch(nqs[i], np.complex64, ir.lower(circs[i], names), [hams[i].terms_ir()],
+                           np.zeros((1, 0)), params[i], batch_threads=1)
+        cpu_dt = time.perf_counter() - t0
+        sub_bytes = sum((90.5 * nqs[i] + 8) * (1 << nqs[i]) * 8 for i in idx)
+        res["cpu_subset"] = {"circuits": len(idx), "qubits": [min(nqs[i] for i in idx), max(nqs[i] for i in idx)],
+                             "cpu_seconds": round(cpu_dt, 3), "gpu_seconds": round(gpu_dt, 4), "cores": cores,
+                             "kind": "port", "speedup": round(cpu_dt / gpu_dt, 1),
+                             "cpu_seconds_full_batch_extrapolated": round(cpu_dt * ref_bytes / sub_bytes, 1)}
+        log(f"uneven batch cpu subset: {res['cpu_subset']}")
+    return res
+
+
+def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
+    """One large instance: HEA of `layers` layers on n qubits (RY, RZ, CNOT ladder), 64 random
+    Pauli strings of weight 2-4 with N(0,1) coefficients; one (f, grad) through the Simulator API."""
+    import mindquantum_amd as mq
+    from mindquantum_amd import _native
+    from mindquantum_amd.algorithm import hardware_efficient_ansatz, random_pauli_hamiltonian
+
+    rng = np.random.default_rng(seed)
+    circ = hardware_efficient_ansatz(n, layers)
+    op = random_pauli_hamiltonian(n, n_terms, seed)
+    ans = rng.uniform(0, 2 * np.pi, size=len(circ.params_name))
+    sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
+    grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
+    _native.set_option("checkpoints", checkpoints)
+    try:
+        grad_ops(ans)  # plans and compiled passes
+        _native.profile_reset()
+        _native.set_option("profile", 1)
+        t0 = time.perf_counter()
+        f, g = grad_ops(ans)
+        dt = time.perf_counter() - t0
+        _native.set_option("profile", 0)
+    finally:
+        _native.set_option("checkpoints", -1)
+    amp = 8 if dtype == mq.complex64 else 16
+    kern = {k: _native.profile(k) for k in ("tile_pass_fwd", "tile_pass_adj", "hamiltonian")}
+    res = {"qubits": n, "dtype": np.dtype(dtype).name, "params": len(circ.params_name), "gates": len(circ),
+           "terms": n_terms, "state_gb": round((1 << n) * amp / 1e9, 2), "seconds": round(dt, 3),
+           "checkpoints": checkpoints, "f": float(np.real(f[0, 0])),
+           "kernel_ms": {k: round(v[1], 1) for k, v in kern.items()},
+           "kernel_hbm_gbs": {k: round(v[2] / (v[1] * 1e-3) / 1e9, 1) for k, v in kern.items() if v[1] > 0}}
+    log(f"large adjoint: {res}")
+    del sim, grad_ops
+    return res
+
+
 def cpu_baseline(n, layers, seed, budget_s):
     """The CPU oracle (reference mqvector algorithm restated, C++/OpenMP, -O3 -mavx2) on a
     bounded sample of the headline workload; best of two thread layouts."""
@@ -200,6 +290,9 @@ def main():
     ap.add_argument("--apply-qubits", type=int, default=28)
     ap.add_argument("--apply-depth", type=int, default=200)
     ap.add_argument("--interpreted", action="store_true", help="run the interpreted pass kernels (no code generation)")
+    ap.add_argument("--items", type=int, default=512, help="circuits of the uneven QAOA batch (0: skip)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 32q c64 / 31q c128 single-instance gradients")
+    ap.add_argument("--large-checkpoints", type=int, default=4)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -240,6 +333,11 @@ def main():
         extra["gate_apply_28q_c64"] = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 10)
         extra["gate_apply_28q_c128"] = run_gate_apply(mq.complex128, args.apply_qubits, args.apply_depth, args.seed,
                                                       10)
+        if args.items > 0:
+            extra["uneven_qaoa_c64"] = run_uneven_items(args.items, args.seed, not args.no_cpu)
+    if not args.no_large and not args.no_extra:
+        extra["large_32q_c64"] = run_large_adjoint(mq.complex64, 32, 4, 64, args.seed, args.large_checkpoints)
+        extra["large_31q_c128"] = run_large_adjoint(mq.complex128, 31, 4, 64, args.seed, args.large_checkpoints)
 
     cpu = None
     if not args.no_cpu:

@@ -102,9 +102,9 @@ int mqa_device_count(void);
  * "max_ops_per_pass" (1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
  * MQAMD_HBM_BUDGET_BYTES; 0 = 85% of free memory), "profile", "ham_tile_qubits_c64",
  * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
- * the first launch), "jit_min_amps".  mqa_get_option also reads the counters "jit_launches",
- * "jit_compile_ms" and "grad_recomputes".  No counterpart in the reference (its CPU kernels have
- * no tuning surface). */
+ * the first launch), "jit_min_amps", "checkpoints".  mqa_get_option also reads the counters
+ * "jit_launches", "jit_compile_ms", "grad_recomputes" and "checkpoints_used".  No counterpart
+ * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
 int mqa_get_option(const char* key, int64_t* value);
 
@@ -176,12 +176,23 @@ int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ha
  * (real; the reference returns 2*Re with a zero imaginary part, gradient.h:124-133).
  * Samples run in chunks that fit the HBM budget (two states per sample, three when several
  * hamiltonians share a kept copy of the forward state; if that copy does not fit, the forward
- * sweep is recomputed per hamiltonian). */
+ * sweep is recomputed per hamiltonian).  With the option "checkpoints" = c > 0 the forward sweep
+ * also stores psi at up to c back-sweep pass boundaries (as many as the budget holds for one
+ * sample, one state each; read back with the counter "checkpoints_used") and the back-sweep
+ * continues from each stored state instead of the uncomputed one, bounding the rounding drift of
+ * the uncompute; the forward sweep is then recomputed per hamiltonian. */
 int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ,
                        const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
                        double* g_out);
-/* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
- * state cast to dtype. */
+/* get_expectation_with_grad_non_hermitian_multi_multi (vector_state.tpp:998-1008 ->
+ * gradient.h:83-98, 199-268): f = <phi_l|H|psi_r> with psi_r = circ_right |sim>, phi_l =
+ * circ_left |left> (left NULL: sim), and its complex derivative df/dp = <phi_l|H|d psi_r> +
+ * <d phi_l|H|psi_r> for every parameter (both circuits share the parameter list; no hermitian
+ * doubling).  f_out: [batch][n_hams][2]; g_out: [batch][n_hams][n_enc+n_ans][2] (re, im).  The
+ * hamiltonians have real coefficients (hermitian), as everywhere in this ABI. */
+int mqa_sim_grad_nonherm(const mqa_sim* sim, const mqa_sim* left, const mqa_ham* const* hams, int32_t n_hams,
+                         const mqa_circuit* circ_right, const mqa_circuit* circ_left, const double* enc, int64_t batch,
+                         int32_t n_enc, const double* ans, int32_t n_ans, double* f_out, double* g_out);
 /* Expectation and hermitian adjoint gradient of many independent circuits of uneven size in
  * shared launches (one launch per sweep step over the tiles of every circuit).  Item i: circuit
  * circs[i] on n_qubits[i] qubits starting from |0...0>, hamiltonian hams[i], parameter values
@@ -193,6 +204,8 @@ int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n
 int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
                    const int32_t* n_qubits, const double* params, const int64_t* param_offset, int32_t dtype,
                    double* f_out, double* g_out);
+/* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
+ * state cast to dtype. */
 int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out);
 /* blas.inner_product(a, b) (bind_vec_state.h:86-90, blas.h): <a|b>. */
 int mqa_inner_product(const mqa_sim* bra, const mqa_sim* ket, double out[2]);

@@ -1,6 +1,7 @@
 """Circuit builders used by the benchmarks and tests (mindquantum.algorithm.nisq subset)."""
 from .builders import (hardware_efficient_ansatz, layered_random_circuit, max_cut_ansatz, max_cut_hamiltonian,
-                       random_pauli_hamiltonian, random_regular_graph)
+                       qaoa_graph, random_pauli_hamiltonian, random_regular_graph,
+                       uneven_qaoa_batch)
 
 __all__ = ["hardware_efficient_ansatz", "layered_random_circuit", "max_cut_ansatz", "max_cut_hamiltonian",
-           "random_pauli_hamiltonian", "random_regular_graph"]
+           "qaoa_graph", "random_pauli_hamiltonian", "random_regular_graph", "uneven_qaoa_batch"]

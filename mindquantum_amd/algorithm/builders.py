@@ -53,6 +53,46 @@ def random_regular_graph(n, degree, seed):
     raise RuntimeError("failed to sample a regular graph")
 
 
+def qaoa_graph(n, seed):
+    """Seeded 3-regular graph on n nodes; for odd n (no 3-regular graph exists) the closest one:
+    every node of degree 3 except node n - 1, of degree 2."""
+    if n % 2 == 0:
+        return random_regular_graph(n, 3, seed)
+    rng = np.random.default_rng(seed)
+    for _ in range(10000):
+        stubs = np.concatenate([np.repeat(np.arange(n - 1), 3), [n - 1, n - 1]])
+        rng.shuffle(stubs)
+        edges = set()
+        ok = True
+        for a, b in zip(stubs[0::2], stubs[1::2]):
+            a, b = int(a), int(b)
+            if a == b or (min(a, b), max(a, b)) in edges:
+                ok = False
+                break
+            edges.add((min(a, b), max(a, b)))
+        if ok:
+            return sorted(edges)
+    raise RuntimeError("failed to sample a graph")
+
+
+def uneven_qaoa_batch(count, seed, n_min=12, n_max=24, depth=4):
+    """The uneven MaxCut batch: `count` circuits with qubit counts uniform in [n_min, n_max],
+    each a QAOA ansatz of `depth` layers on its own seeded graph (qaoa_graph) with angles
+    gamma, beta ~ U(0, pi).  Returns (circuits, hamiltonians (QubitOperator), qubit counts,
+    parameter vectors in each circuit's params_name order)."""
+    rng = np.random.default_rng(seed)
+    circs, ops, nqs, params = [], [], [], []
+    for i in range(count):
+        n = int(rng.integers(n_min, n_max + 1))
+        graph = qaoa_graph(n, int(rng.integers(1 << 31)))
+        c = max_cut_ansatz(graph, depth)
+        circs.append(c)
+        ops.append(max_cut_hamiltonian(graph))
+        nqs.append(n)
+        params.append(rng.uniform(0, np.pi, size=len(c.params_name)))
+    return circs, ops, nqs, params
+
+
 def max_cut_ansatz(graph, depth):
     nodes = sorted({q for e in graph for q in e})
     circ = UN(G.H, nodes)

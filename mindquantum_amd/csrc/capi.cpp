@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/mqamd.h"
+#include "aux_kernels.h"
 #include "jit.h"
 #include "launch.h"
 #include "program.h"
@@ -72,6 +73,8 @@ struct Options {
     // per-circuit compiled passes: 0 off, 1 compile in the background and switch over when
     // ready, 2 compile before the first launch; only for launches of >= jit_min_amps amplitudes
     std::atomic<int64_t> jit{1}, jit_min_amps{int64_t(1) << 24};
+    // forward-state checkpoints of the batched gradient (0: none; the back-sweep uncomputes psi)
+    std::atomic<int64_t> checkpoints{0};
 };
 Options g_opt;
 
@@ -88,7 +91,7 @@ std::vector<OptEntry> opt_table() {
             {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0},
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
-            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}};
+            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -236,8 +239,15 @@ struct mqa_circuit {
         std::lock_guard<std::mutex> lk(mu);
         auto it = plans.find(key);
         if (it != plans.end()) return it->second.get();
+        auto dp = make_devplan(mqa::build_plan(gates, n_eff, adjoint, o), dtype, s, upload_dev);
+        auto* raw = dp.get();
+        plans[key] = std::move(dp);
+        return raw;
+    }
+
+    static std::unique_ptr<mqa::DevPlan> make_devplan(mqa::Plan&& plan, int dtype, hipStream_t s, bool upload_dev) {
         auto dp = std::make_unique<mqa::DevPlan>();
-        dp->plan = mqa::build_plan(gates, n_eff, adjoint, o);
+        dp->plan = std::move(plan);
         // dense blocks or controls that stay per-amplitude predicates need the FULL instantiation
         for (auto& op : dp->plan.dev_ops)
             if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0)
@@ -256,8 +266,49 @@ struct mqa_circuit {
             }
             mqa::hip_check(hipStreamSynchronize(s), "plan upload");
         }
-        auto* raw = dp.get();
-        plans[key] = std::move(dp);
+        return dp;
+    }
+
+    // The forward sweep split at back-sweep pass boundaries, for checkpointed gradients.
+    // After bound[i] back-sweep passes, psi must equal the forward state of the gates the
+    // back-sweep has not undone yet (a dependency-closed subset, so its circuit-order product
+    // is well defined).  Forward segment k (k < n) completes the state of boundary n-1-k (the
+    // deepest boundary first); segment n completes the full forward state.
+    struct Checkpointed {
+        std::vector<int> bound;                           // increasing back-sweep pass counts
+        std::vector<std::unique_ptr<mqa::DevPlan>> segs;  // bound.size() + 1 forward plans
+    };
+    std::map<std::tuple<const mqa::DevPlan*, int, int, int, int, int>, std::unique_ptr<Checkpointed>> ckpt_plans;
+
+    Checkpointed* checkpointed(int n_eff, int dtype, int n_ck, const mqa::DevPlan* adj, hipStream_t s) {
+        auto o = mqa::sched_options(dtype, false);
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_tuple(adj, n_ck, o.tile_qubits, o.low_run, o.group_width, o.max_ops_per_pass);
+        auto it = ckpt_plans.find(key);
+        if (it != ckpt_plans.end()) return it->second.get();
+        auto cp = std::make_unique<Checkpointed>();
+        const int P = int(adj->plan.passes.size());
+        for (int i = 1; i <= n_ck; i++) {
+            const int b = int((int64_t(i) * P) / (n_ck + 1));
+            if (b >= 1 && b <= P - 1 && (cp->bound.empty() || b > cp->bound.back())) cp->bound.push_back(b);
+        }
+        std::vector<int> pass_of(gates.size(), -1);  // back-sweep pass that undoes each gate
+        for (int j = 0; j < P; j++)
+            for (auto& g : adj->plan.passes[size_t(j)].groups)
+                for (auto& op : g.ops) pass_of[size_t(op.gate)] = j;
+        const int n = int(cp->bound.size());
+        for (int k = 0; k <= n; k++) {
+            std::vector<mqa::CGate> seg = gates;
+            for (size_t gi = 0; gi < gates.size(); gi++) {
+                const int p = pass_of[gi];
+                int below = 0;  // boundaries at or before the gate's undo pass
+                for (int b : cp->bound) below += (b <= p) ? 1 : 0;
+                if (p < 0 || n - below != k) seg[gi].kind = MQA_GATE_I;
+            }
+            cp->segs.push_back(make_devplan(mqa::build_plan(seg, n_eff, false, o), dtype, s, true));
+        }
+        auto* raw = cp.get();
+        ckpt_plans[key] = std::move(cp);
         return raw;
     }
 
@@ -345,7 +396,7 @@ struct mqa_sim {
     hipStream_t stream = nullptr;
     mqa::DevBuf state;
     // workspaces (grow-only)
-    mqa::DevBuf psi, lam, psi_keep, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
+    mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
     mqa::Timer timer;
     size_t amp_bytes() const { return dtype == MQA_C64 ? 8 : 16; }
     size_t state_bytes() const { return dim_eff * amp_bytes(); }
@@ -387,6 +438,7 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 
 std::atomic<int64_t> g_jit_launches{0};
 std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
+std::atomic<int64_t> g_ckpt_used{0};    // checkpoints the last batched gradient call held
 void count_jit_launch() { g_jit_launches.fetch_add(1); }
 
 void sync(mqa_sim* sim, const char* what) {
@@ -395,9 +447,12 @@ void sync(mqa_sim* sim, const char* what) {
 }
 
 // Run every pass of a plan on `batch` states.
+// Passes [p0, p1) of the plan (p1 < 0: to the end).
 void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, const double* angles, int n_slots,
-              double* partials, int n_tap_ids) {
+              double* partials, int n_tap_ids, int p0 = 0, int p1 = -1) {
     const Plan& pl = dp->plan;
+    if (p1 < 0 || p1 > int(pl.passes.size())) p1 = int(pl.passes.size());
+    if (p0 >= p1) return;
     TilePassArgs a{};
     a.psi = psi;
     a.lam = lam;
@@ -422,11 +477,11 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         if (dp->jit->ready()) jp = dp->jit.get();  // until then (or if it failed): the interpreted kernel
     }
     const size_t lds = tile_pass_lds_bytes(sim->dtype, pl.adjoint, pl.K, pl.C, std::max(pl.max_taps, 1));
-    for (size_t i = 0; i < pl.passes.size(); i++) {
-        a.pass_index = int(i);
+    for (int i = p0; i < p1; i++) {
+        a.pass_index = i;
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
             if (jp) {
-                hip_check(jp->launch(int(i), a, batch, lds, sim->stream), "compiled tile pass");
+                hip_check(jp->launch(i, a, batch, lds, sim->stream), "compiled tile pass");
             } else {
                 hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C,
                                            std::max(pl.max_taps, 1), sim->stream),
@@ -439,8 +494,9 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
 
 // (cos, sin) table for `batch` rows: params p < n_enc from enc rows, the rest from ans.
 const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_host, uint64_t batch, int n_enc,
-                           const double* ans_host, int n_ans) {
+                           const double* ans_host, int n_ans, DevBuf* out = nullptr) {
     if (circ->n_slots == 0) return nullptr;
+    DevBuf& table = out ? *out : sim->angles;
     DevBuf* sl = circ->slots_on_device(sim->stream);
     hipStream_t s = sim->stream;
     sim->enc.ensure(std::max<size_t>(batch * n_enc, 1) * sizeof(double));
@@ -450,7 +506,7 @@ const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_ho
                   "enc upload");
     if (n_ans > 0)
         hip_check(hipMemcpyAsync(sim->ans.p, ans_host, n_ans * sizeof(double), hipMemcpyHostToDevice, s), "ans upload");
-    sim->angles.ensure(batch * circ->n_slots * 2 * sizeof(double));
+    table.ensure(batch * circ->n_slots * 2 * sizeof(double));
     AngleArgs aa{};
     aa.enc = sim->enc.as<double>();
     aa.ans = sim->ans.as<double>();
@@ -462,9 +518,9 @@ const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_ho
     aa.term_param = sl[2].as<int32_t>();
     aa.term_coeff = sl[3].as<double>();
     aa.half = sl[4].as<int8_t>();
-    aa.out = sim->angles.as<double>();
+    aa.out = table.as<double>();
     hip_check(launch_angles(aa, s), "angles");
-    return sim->angles.as<double>();
+    return table.as<double>();
 }
 
 // out = H ket over a batch with the tile engine; with `expect`, per-sample <ket|H|ket> in
@@ -619,6 +675,10 @@ int mqa_get_option(const char* key, int64_t* value) {
         }
         if (std::strcmp(key, "grad_recomputes") == 0) {  // read-only: see mqa_sim_grad_batch
             *value = g_recomputes.load();
+            return;
+        }
+        if (std::strcmp(key, "checkpoints_used") == 0) {  // read-only: see mqa_sim_grad_batch
+            *value = g_ckpt_used.load();
             return;
         }
         if (std::strcmp(key, "jit_compile_ms") == 0) {  // read-only: time spent compiling generated passes
@@ -1023,11 +1083,23 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         // when several hamiltonians share it.  If one sample with that copy exceeds the budget,
         // the forward sweep is recomputed from the initial state for every further hamiltonian
         // instead (the back-sweep consumes psi, so it must be restored one way or the other).
+        // With checkpoints, the forward sweep also stores psi at back-sweep pass boundaries and
+        // the back-sweep continues from each stored state (a pointer switch, no copy), so
+        // rounding drift of the uncomputed psi cannot build up over more than one segment; as
+        // many checkpoints are held as the budget allows for one sample.
         const size_t side = size_t(n_tap_ids) * n_tiles * 8 + size_t(circ->n_slots) * 16 +
                             size_t(n_tap_ids + n_params) * 16 + 4096;
         const size_t budget = hbm_budget();
-        const bool keep = n_hams > 1 && budget >= sb * 3 + side;
-        const size_t per_sample = sb * (keep ? 3 : 2) + side;
+        int n_ck = 0;
+        if (g_opt.checkpoints > 0 && n_tap_ids > 0) {
+            n_ck = int(std::min<int64_t>(g_opt.checkpoints, int64_t(adj->plan.passes.size()) - 1));
+            while (n_ck > 0 && budget < sb * size_t(2 + n_ck) + side) n_ck--;
+        }
+        mqa_circuit::Checkpointed* ck = n_ck > 0 ? circ->checkpointed(sim->n_eff, sim->dtype, n_ck, adj, s) : nullptr;
+        if (ck) n_ck = int(ck->bound.size());
+        g_ckpt_used.store(n_ck);
+        const bool keep = n_ck == 0 && n_hams > 1 && budget >= sb * 3 + side;
+        const size_t per_sample = sb * size_t(2 + n_ck + (keep ? 1 : 0)) + side;
         int64_t chunk = int64_t(std::max<size_t>(budget / per_sample, 1));
         chunk = std::min<int64_t>(chunk, batch);
         chunk = std::min<int64_t>(chunk, 65535);
@@ -1035,27 +1107,40 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         sim->psi.ensure(sb * chunk);
         sim->lam.ensure(sb * chunk);
         if (keep) sim->psi_keep.ensure(sb * chunk);
+        if (n_ck > 0) sim->ckpt.ensure(sb * chunk * size_t(n_ck));
         sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 8, 16));
         sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 8, 16));
         sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
         std::vector<double> g_host(size_t(chunk) * n_params), f_host(size_t(chunk) * 2);
+        auto ckpt_at = [&](int i) { return static_cast<char*>(sim->ckpt.p) + size_t(i) * sb * size_t(chunk); };
 
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t nb = std::min<int64_t>(chunk, batch - b0);
             const double* ang = build_angles(sim, circ, n_enc ? enc + b0 * n_enc : nullptr, nb, n_enc, ans, n_ans);
-            hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff, nb, s),
-                      "fill");
-            run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+            // forward sweep into psi (and the checkpoints)
+            auto forward = [&] {
+                hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff,
+                                      nb, s),
+                          "fill");
+                if (!ck) {
+                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                    return;
+                }
+                for (int k = 0; k <= n_ck; k++) {
+                    run_plan(sim, ck->segs[size_t(k)].get(), sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                    if (k < n_ck)
+                        hip_check(hipMemcpyAsync(ckpt_at(n_ck - 1 - k), sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s),
+                                  "checkpoint");
+                }
+            };
+            forward();
             if (keep)
                 hip_check(hipMemcpyAsync(sim->psi_keep.p, sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s), "keep");
             for (int h = 0; h < n_hams; h++) {
                 if (h > 0 && keep) {
                     hip_check(hipMemcpyAsync(sim->psi.p, sim->psi_keep.p, sb * nb, hipMemcpyDeviceToDevice, s), "restore");
-                } else if (h > 0) {  // recompute the forward state
-                    hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff,
-                                          sim->dim_eff, nb, s),
-                              "fill");
-                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                } else if (h > 0) {  // recompute the forward state (and the checkpoints)
+                    forward();
                     g_recomputes.fetch_add(1);
                 }
                 auto* ham = const_cast<mqa_ham*>(hams[h]);
@@ -1064,8 +1149,16 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                 hip_check(hipMemcpyAsync(f_host.data(), sim->fsum.p, nb * 2 * sizeof(double), hipMemcpyDeviceToHost, s),
                           "d2h f");
                 if (n_tap_ids > 0) {
-                    run_plan(sim, adj, sim->psi.p, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(),
-                             n_tap_ids);
+                    int p0 = 0;
+                    void* cur = sim->psi.p;
+                    for (int i = 0; i < n_ck; i++) {  // back-sweep segment, then continue from checkpoint i
+                        run_plan(sim, adj, cur, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(),
+                                 n_tap_ids, p0, ck->bound[size_t(i)]);
+                        p0 = ck->bound[size_t(i)];
+                        cur = ckpt_at(i);
+                    }
+                    run_plan(sim, adj, cur, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(), n_tap_ids,
+                             p0);
                     hip_check(launch_reduce_rows1(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
                                                   uint64_t(nb) * n_tap_ids, s),
                               "reduce taps");
@@ -1428,6 +1521,130 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             std::vector<GradItem> chunk(all.begin() + long(a), all.begin() + long(b));
             run_grad_chunk(chunk, circs, hams, params, param_offset, dtype, so.s, f_out, g_out);
             a = b;
+        }
+    });
+}
+
+int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_ham* const* hams, int32_t n_hams,
+                         const mqa_circuit* circ_r_c, const mqa_circuit* circ_l_c, const double* enc, int64_t batch,
+                         int32_t n_enc, const double* ans, int32_t n_ans, double* f_out, double* g_out) {
+    return guarded([&] {
+        require(sim_c != nullptr && circ_r_c != nullptr && circ_l_c != nullptr, "NULL argument");
+        require(n_hams >= 1 && hams != nullptr, "need at least one hamiltonian");
+        require(batch >= 1, "batch must be positive");
+        require(n_enc >= 0 && n_ans >= 0, "bad parameter counts");
+        require(n_enc == 0 || enc != nullptr, "enc is NULL");
+        require(n_ans == 0 || ans != nullptr, "ans is NULL");
+        require(f_out != nullptr && g_out != nullptr, "output is NULL");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        auto* left = const_cast<mqa_sim*>(left_c ? left_c : sim_c);
+        require(left->n == sim->n && left->dtype == sim->dtype, "simulator_left does not match this simulator");
+        auto* cr = const_cast<mqa_circuit*>(circ_r_c);
+        auto* cl = const_cast<mqa_circuit*>(circ_l_c);
+        const int n_params = n_enc + n_ans;
+        check_circuit(sim, cr, n_params);
+        check_circuit(sim, cl, n_params);
+        for (int h = 0; h < n_hams; h++) check_ham(sim, hams[h]);
+        DeviceGuard dg(sim->device);
+        hipStream_t s = sim->stream;
+        if (left->stream != s) hip_check(hipStreamSynchronize(left->stream), "sync");
+        const size_t sb = sim->state_bytes();
+        const uint64_t dim = sim->dim_eff;
+
+        // f = <phi_l|H|psi_r> with phi_l = Ul|left>, psi_r = Ur|sim>;
+        // df/dp = <phi_l|H|d psi_r> + conj(<H psi_r|d phi_l>)  (gradient.h:199-268, without the
+        // hermitian doubling).  Each term is a back-sweep with complex taps; the kernels accumulate
+        // Re<lam|dG|psi>, and Re<i lam|dG|psi> = Im<lam|dG|psi>, so every sweep runs as a
+        // two-state batch with co-states (lam, i lam).  Buffers: psi = [psi_r, psi_r, phi_l, phi_l],
+        // lam = [H phi_l, i H phi_l, H psi_r, i H psi_r].
+        sim->psi.ensure(sb * 4);
+        sim->lam.ensure(sb * 4);
+        char* P = static_cast<char*>(sim->psi.p);
+        char* L = static_cast<char*>(sim->lam.p);
+        struct Side {
+            mqa_circuit* c;
+            DevPlan* fwd;
+            DevPlan* adj;
+            uint64_t n_tiles;
+            DevBuf ang, part, taps;
+        };
+        Side side[2];
+        side[0].c = cr;
+        side[1].c = cl;
+        for (auto& sd : side) {
+            sd.fwd = sd.c->get_plan(sim->n_eff, sim->dtype, false, s, true);
+            sd.adj = sd.c->get_plan(sim->n_eff, sim->dtype, true, s, true);
+            sd.n_tiles = uint64_t(1) << (sd.adj->plan.n_eff - sd.adj->plan.K);
+            sd.part.ensure(std::max<size_t>(2 * size_t(sd.c->n_tap_ids) * sd.n_tiles * 8, 16));
+            sd.taps.ensure(std::max<size_t>(2 * size_t(sd.c->n_tap_ids) * 8, 16));
+        }
+        std::vector<double> enc2(size_t(2) * std::max(n_enc, 1));
+        std::vector<double> taps_h[2];
+        double f_h[2];
+        for (int64_t b = 0; b < batch; b++) {
+            // the same angles for both states of a pair
+            for (int k = 0; k < n_enc; k++) enc2[size_t(k)] = enc2[size_t(n_enc + k)] = enc[b * n_enc + k];
+            const double* ang[2];
+            for (int i = 0; i < 2; i++)
+                ang[i] = build_angles(sim, side[i].c, n_enc ? enc2.data() : nullptr, 2, n_enc, ans, n_ans, &side[i].ang);
+            // forward states: psi_r twice (slots 0, 1), phi_l twice (slots 2, 3)
+            hip_check(launch_fill(sim->dtype, P, sim->state.p, dim, dim, dim, 2, s), "fill");
+            run_plan(sim, side[0].fwd, P, nullptr, 2, ang[0], cr->n_slots, nullptr, 0);
+            hip_check(launch_fill(sim->dtype, P + 2 * sb, left->state.p, dim, dim, dim, 2, s), "fill");
+            run_plan(sim, side[1].fwd, P + 2 * sb, nullptr, 2, ang[1], cl->n_slots, nullptr, 0);
+            for (int h = 0; h < n_hams; h++) {
+                auto* ham = const_cast<mqa_ham*>(hams[h]);
+                ham_apply(sim, ham, P + 2 * sb, nullptr, L, 1);  // H phi_l (H is hermitian: real coefficients)
+                ham_apply(sim, ham, P, nullptr, L + 2 * sb, 1);  // H psi_r
+                hip_check(launch_scale(sim->dtype, L, L + sb, dim, 0.0, 1.0, s), "scale");
+                hip_check(launch_scale(sim->dtype, L + 2 * sb, L + 3 * sb, dim, 0.0, 1.0, s), "scale");
+                vdot(sim, L, P, f_h);  // <H phi_l|psi_r> = <phi_l|H|psi_r> (synchronises)
+                // back-sweeps consume psi: restore both forward pairs for every further hamiltonian
+                for (int i = 0; i < 2; i++) {
+                    if (side[i].c->n_tap_ids == 0) {
+                        taps_h[i].clear();
+                        continue;
+                    }
+                    char* psi_pair = P + size_t(2 * i) * sb;
+                    char* lam_pair = L + size_t(2 * i) * sb;  // right sweep: H phi_l; left: H psi_r
+                    if (h > 0) {
+                        hip_check(launch_fill(sim->dtype, psi_pair, (i == 0 ? sim : left)->state.p, dim, dim, dim, 2, s),
+                                  "fill");
+                        run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0);
+                    }
+                    run_plan(sim, side[i].adj, psi_pair, lam_pair, 2, ang[i], side[i].c->n_slots,
+                             side[i].part.as<double>(), side[i].c->n_tap_ids);
+                    hip_check(launch_reduce_rows1(side[i].part.as<double>(), side[i].n_tiles, side[i].taps.as<double>(),
+                                                  uint64_t(2) * side[i].c->n_tap_ids, s),
+                              "reduce taps");
+                    taps_h[i].assign(size_t(2) * side[i].c->n_tap_ids, 0.0);
+                    hip_check(hipMemcpyAsync(taps_h[i].data(), side[i].taps.p, taps_h[i].size() * 8,
+                                             hipMemcpyDeviceToHost, s),
+                              "d2h taps");
+                }
+                sync(sim, "grad nonherm");
+                double* fo = f_out + (b * n_hams + h) * 2;
+                fo[0] = f_h[0];
+                fo[1] = f_h[1];
+                // jacobian: right taps enter as z, left taps as conj(z); z = taps[0][t] + i taps[1][t]
+                double* go = g_out + (b * n_hams + h) * size_t(n_params) * 2;
+                for (int p = 0; p < n_params; p++) {
+                    double gr = 0, gi = 0;
+                    for (int i = 0; i < 2; i++) {
+                        const mqa_circuit* c = side[i].c;
+                        if (taps_h[i].empty()) continue;
+                        const size_t nt = size_t(c->n_tap_ids);
+                        for (int k = c->jac_begin[size_t(p)]; k < c->jac_begin[size_t(p) + 1]; k++) {
+                            const size_t t = size_t(c->jac_tap[size_t(k)]);
+                            const double co = c->jac_coeff[size_t(k)];
+                            gr += co * taps_h[i][t];
+                            gi += co * (i == 0 ? taps_h[i][nt + t] : -taps_h[i][nt + t]);
+                        }
+                    }
+                    go[2 * p] = gr;
+                    go[2 * p + 1] = gi;
+                }
+            }
         }
     });
 }
