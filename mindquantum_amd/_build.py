@@ -1,7 +1,7 @@
 """Build the HIP library ``libmqamd.so`` in-tree for gfx950 (hipcc, no cmake).
 
 ``python -m mindquantum_amd._build`` or ``__graft_entry__.build()`` runs it; every call
-recompiles.  Sources: ``csrc/kernels.hip`` (device code), ``csrc/capi.cpp``,
+recompiles.  Sources: ``csrc/kernels.hip`` and ``csrc/aux_kernels.hip`` (device code), ``csrc/capi.cpp``,
 ``csrc/schedule.cpp`` and ``csrc/jit.cpp`` (host code).  The text of ``csrc/program.h`` and
 ``csrc/device_ops.h`` is embedded into the library (``_build/jit_headers.inc``) for the
 per-circuit kernels that ``jit.cpp`` compiles with hipRTC at run time.
@@ -62,12 +62,13 @@ def build(verbose=False):
     hipcc = _hipcc()
     jobs = [
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "kernels.hip"), "-o", os.path.join(BUILD, "kernels.o")],
+        [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "aux_kernels.hip"), "-o", os.path.join(BUILD, "aux_kernels.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "capi.cpp"), "-o", os.path.join(BUILD, "capi.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "schedule.cpp"), "-o", os.path.join(BUILD, "schedule.o")],
         [hipcc, *COMMON, *DEVICE, "-I", BUILD, "-c", os.path.join(CSRC, "jit.cpp"), "-o", os.path.join(BUILD, "jit.o")],
         [hipcc, *COMMON, "-c", os.path.join(CSRC, "jitc_main.cpp"), "-o", os.path.join(BUILD, "jitc_main.o")],
     ]
-    with ThreadPoolExecutor(max_workers=5) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:
         for o in outs:
@@ -75,7 +76,8 @@ def build(verbose=False):
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, *DEVICE, "-shared", "-fPIC", "-o", tmp,
-          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "capi.o"), os.path.join(BUILD, "schedule.o"),
+          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), os.path.join(BUILD, "capi.o"),
+          os.path.join(BUILD, "schedule.o"),
           os.path.join(BUILD, "jit.o"), "-lhiprtc"])
     os.replace(tmp, LIB)
     # compile helper of the code generator: hipRTC runs one compile at a time per process

@@ -210,8 +210,24 @@ class MQVectorAMD:
         for i, h in enumerate(hams):
             self._check_ham(h, i)
         self._check_circ(circ_right)
-        if circ_left is not None or simulator_left is not None:
-            raise NotImplementedError("non-hermitian (circ_left / simulator_left) gradients are not on this path yet")
+        non_hermitian = False
+        if circ_left is not None:
+            self._check_circ(circ_left)
+            non_hermitian = True
+        if simulator_left is not None:
+            if not isinstance(simulator_left, type(self)):
+                raise TypeError("simulator_left should be a simulator of the same backend")
+            if simulator_left.n_qubits != self.n_qubits:
+                raise ValueError("simulator_left should have the same n_qubits as this simulator, "
+                                 f"which is {self.n_qubits}, but get {simulator_left.n_qubits}")
+            if simulator_left.dtype != self.dtype:
+                raise ValueError("simulator_left should have the same data type as this simulator "
+                                 f"(aka {self.dtype}), but get {simulator_left.dtype}.")
+            non_hermitian = True
+        if non_hermitian and simulator_left is None:
+            simulator_left = self
+        if circ_left is None:
+            circ_left = circ_right
         if pr_shift:
             raise NotImplementedError("parameter-shift gradients are not on this path; the adjoint is used")
         if circ_right.has_measure_gate:
@@ -220,6 +236,9 @@ class MQVectorAMD:
             raise TypeError(f"parallel_worker requires a int, but get {type(parallel_worker)}")
         ansatz_params_name = list(circ_right.all_ansatz.keys())
         encoder_params_name = list(circ_right.all_encoder.keys())
+        if non_hermitian:  # mqsim.py:323-331: the left circuit's names follow the right circuit's
+            ansatz_params_name += [n for n in circ_left.all_ansatz.keys() if n not in ansatz_params_name]
+            encoder_params_name += [n for n in circ_left.all_encoder.keys() if n not in encoder_params_name]
         if set(ansatz_params_name) & set(encoder_params_name):
             raise RuntimeError("Parameter cannot be both encoder and ansatz parameter.")
         version = "both"
@@ -228,7 +247,11 @@ class MQVectorAMD:
         if not encoder_params_name:
             version = "ansatz"
         names = encoder_params_name + ansatz_params_name
+        if max(circ_left.n_qubits, circ_right.n_qubits) > self.n_qubits:
+            raise ValueError(f"Simulator has {self.n_qubits} qubits, but circuit has "
+                             f"{max(circ_left.n_qubits, circ_right.n_qubits)} qubits.")
         nc = self._native_circuit(circ_right, names)
+        ncl = self._native_circuit(circ_left, names) if non_hermitian else None
         ham_handles = (ctypes.c_void_p * len(hams))(*[h.native().value for h in hams])
         n_enc, n_ans = len(encoder_params_name), len(ansatz_params_name)
 
@@ -254,12 +277,20 @@ class MQVectorAMD:
             batch = enc.shape[0]
             _thread_balance(batch, len(hams), parallel_worker)  # validates like the reference
             f = np.zeros((batch, len(hams), 2), dtype=np.float64)
-            g = np.zeros((batch, len(hams), n_enc + n_ans), dtype=np.float64)
-            _native.check(_native.lib().mqa_sim_grad_batch(
-                self._h, ham_handles, len(hams), nc.handle, _native.ptr(enc) if n_enc else None, batch, n_enc,
-                _native.ptr(ans) if n_ans else None, n_ans, _native.ptr(f), _native.ptr(g)))
+            if non_hermitian:
+                g = np.zeros((batch, len(hams), n_enc + n_ans, 2), dtype=np.float64)
+                _native.check(_native.lib().mqa_sim_grad_nonherm(
+                    self._h, simulator_left._h, ham_handles, len(hams), nc.handle, ncl.handle,
+                    _native.ptr(enc) if n_enc else None, batch, n_enc, _native.ptr(ans) if n_ans else None, n_ans,
+                    _native.ptr(f), _native.ptr(g)))
+                gc = (g[..., 0] + 1j * g[..., 1]).astype(self.dtype)
+            else:
+                g = np.zeros((batch, len(hams), n_enc + n_ans), dtype=np.float64)
+                _native.check(_native.lib().mqa_sim_grad_batch(
+                    self._h, ham_handles, len(hams), nc.handle, _native.ptr(enc) if n_enc else None, batch, n_enc,
+                    _native.ptr(ans) if n_ans else None, n_ans, _native.ptr(f), _native.ptr(g)))
+                gc = g.astype(self.dtype)
             fc = (f[..., 0] + 1j * f[..., 1]).astype(self.dtype)
-            gc = g.astype(self.dtype)
             if version == "both":
                 return fc, gc[:, :, :n_enc], gc[:, :, n_enc:]
             return fc, gc

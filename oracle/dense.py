@@ -139,3 +139,45 @@ def gradient(circ, values, names, n, qubit_operator, init=None):
                 if name in idx and name in pr.requires_grad_parameters:
                     grad[idx[name]] += 2 * np.real(val) * np.real(c)
     return f, grad
+
+
+def _state_derivatives(circ, values, names, n, init):
+    """Final state and d state / d name for every name (product rule over closed-form gates)."""
+    dim = 1 << n
+    circ = _acting(circ)
+    psi0 = np.zeros(dim, dtype=np.complex128)
+    if init is None:
+        psi0[0] = 1
+    else:
+        psi0[:] = init
+    mats = [embed(gate_matrix(g, values), g.obj_qubits, g.ctrl_qubits, n) for g in circ]
+    prefix = [psi0]
+    for m in mats:
+        prefix.append(m @ prefix[-1])
+    suffix = [np.eye(dim, dtype=np.complex128)]
+    for m in reversed(mats):
+        suffix.append(suffix[-1] @ m)
+    suffix = suffix[::-1]
+    idx = {nm: i for i, nm in enumerate(names)}
+    d = np.zeros((len(names), dim), dtype=np.complex128)
+    for k, g in enumerate(circ):
+        prs = g.param_resolvers()
+        for pr, dmat in zip(prs, g.diff_matrices(values) if prs else []):
+            if pr.is_const():
+                continue
+            v = suffix[k + 1] @ (embed_diff(dmat, g.obj_qubits, g.ctrl_qubits, n) @ prefix[k])
+            for name, c in pr.items():
+                if name in idx and name in pr.requires_grad_parameters:
+                    d[idx[name]] += np.real(c) * v
+    return prefix[-1], d
+
+
+def nonhermitian_gradient(circ_r, circ_l, values, names, n, qubit_operator, init_r=None, init_l=None):
+    """f = <phi_l|H|psi_r> (phi_l = circ_l init_l, psi_r = circ_r init_r) and its complex
+    derivative df/dname = <d phi_l|H|psi_r> + <phi_l|H|d psi_r> for every name."""
+    hm = hamiltonian_matrix(qubit_operator, n)
+    psi_r, d_r = _state_derivatives(circ_r, values, names, n, init_r)
+    phi_l, d_l = _state_derivatives(circ_l, values, names, n, init_l)
+    f = np.vdot(phi_l, hm @ psi_r)
+    g = np.array([np.vdot(d_l[j], hm @ psi_r) + np.vdot(phi_l, hm @ d_r[j]) for j in range(len(names))])
+    return f, g
