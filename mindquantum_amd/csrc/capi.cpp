@@ -1593,25 +1593,27 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
             hip_check(launch_fill(sim->dtype, P + 2 * sb, left->state.p, dim, dim, dim, 2, s), "fill");
             run_plan(sim, side[1].fwd, P + 2 * sb, nullptr, 2, ang[1], cl->n_slots, nullptr, 0);
             for (int h = 0; h < n_hams; h++) {
+                // the back-sweeps of the previous hamiltonian consumed the forward pairs: redo them
+                for (int i = 0; h > 0 && i < 2; i++) {
+                    if (side[i].c->n_tap_ids == 0) continue;
+                    char* psi_pair = P + size_t(2 * i) * sb;
+                    hip_check(launch_fill(sim->dtype, psi_pair, (i == 0 ? sim : left)->state.p, dim, dim, dim, 2, s),
+                              "fill");
+                    run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0);
+                }
                 auto* ham = const_cast<mqa_ham*>(hams[h]);
                 ham_apply(sim, ham, P + 2 * sb, nullptr, L, 1);  // H phi_l (H is hermitian: real coefficients)
                 ham_apply(sim, ham, P, nullptr, L + 2 * sb, 1);  // H psi_r
                 hip_check(launch_scale(sim->dtype, L, L + sb, dim, 0.0, 1.0, s), "scale");
                 hip_check(launch_scale(sim->dtype, L + 2 * sb, L + 3 * sb, dim, 0.0, 1.0, s), "scale");
                 vdot(sim, L, P, f_h);  // <H phi_l|psi_r> = <phi_l|H|psi_r> (synchronises)
-                // back-sweeps consume psi: restore both forward pairs for every further hamiltonian
-                for (int i = 0; i < 2; i++) {
+                for (int i = 0; i < 2; i++) {  // the back-sweeps (they consume the forward pairs)
                     if (side[i].c->n_tap_ids == 0) {
                         taps_h[i].clear();
                         continue;
                     }
                     char* psi_pair = P + size_t(2 * i) * sb;
                     char* lam_pair = L + size_t(2 * i) * sb;  // right sweep: H phi_l; left: H psi_r
-                    if (h > 0) {
-                        hip_check(launch_fill(sim->dtype, psi_pair, (i == 0 ? sim : left)->state.p, dim, dim, dim, 2, s),
-                                  "fill");
-                        run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0);
-                    }
                     run_plan(sim, side[i].adj, psi_pair, lam_pair, 2, ang[i], side[i].c->n_slots,
                              side[i].part.as<double>(), side[i].c->n_tap_ids);
                     hip_check(launch_reduce_rows1(side[i].part.as<double>(), side[i].n_tiles, side[i].taps.as<double>(),
