@@ -28,6 +28,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel, measured with FETCH_SIZE / WRITE_SIZE counter runs
+# scaled by a copy kernel of known size (the summary file names the commands and the factors)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_20q_b1024_c64_hbm_counters.md")
+
+
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 METRIC = "adjoint grads/s (20q, batch 1024, c64 vs c128); 28q gate-apply HBM GB/s"
 
 
@@ -236,6 +250,22 @@ def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
     return res
 
 
+def copy_probe(gib=4, reps=10):
+    """Plain device-to-device copy of `gib` GiB (the library's element-wise copy kernel, timed with
+    HIP events): the achievable streaming rate on this GPU, and the kernel of known traffic that
+    calibrates the HBM byte counters of the profiling runs."""
+    import ctypes
+
+    from mindquantum_amd import _native
+
+    ms = ctypes.c_double(0)
+    _native.check(_native.lib().mqa_copy_probe(gib << 30, reps, ctypes.byref(ms)))
+    byts = 2 * (gib << 30)
+    res = {"bytes_per_copy": byts, "avg_ms": round(ms.value, 4), "gbs": round(byts / (ms.value * 1e-3) / 1e9, 1)}
+    log(f"copy probe: {res}")
+    return res
+
+
 def cpu_baseline(n, layers, seed, budget_s):
     """The CPU oracle (reference mqvector algorithm restated, C++/OpenMP, -O3 -mavx2) on a
     bounded sample of the headline workload; best of two thread layouts."""
@@ -293,6 +323,8 @@ def main():
     ap.add_argument("--items", type=int, default=512, help="circuits of the uneven QAOA batch (0: skip)")
     ap.add_argument("--no-large", action="store_true", help="skip the 32q c64 / 31q c128 single-instance gradients")
     ap.add_argument("--large-checkpoints", type=int, default=4)
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="run only the device copy probe (counter calibration for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -306,6 +338,9 @@ def main():
     if _native.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe_only:
+        print(json.dumps({"copy_probe": copy_probe()}), flush=True)
+        return
     # per-circuit compiled passes, compiled before the first launch (inside the warmup steps)
     _native.set_option("jit", 0 if args.interpreted else 2)
 
@@ -317,7 +352,9 @@ def main():
     per_launch = byts / max(launches, 1)
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic() if not args.interpreted and (args.qubits, args.batch, args.layers) ==
+                (20, 1024, 8) else None,
                 "kernel": "k_tile_pass<float, adjoint>" if args.interpreted else "mqa_jit_tile_pass_adj (complex64)",
                 "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": per_launch, "launches": launches}
@@ -325,6 +362,8 @@ def main():
 
     extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3}
     if not args.no_extra:
+        extra["copy_probe"] = copy_probe()
+        roofline["achievable_copy_gbs"] = extra["copy_probe"]["gbs"]
         v128, ms128, _, _, _ = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, 1, 1,
                                            args.seed, profile=False)
         extra["c128_grads_per_s"] = round(v128, 2)

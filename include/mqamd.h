@@ -115,6 +115,11 @@ int mqa_get_option(const char* key, int64_t* value);
  * No counterpart in the reference (it has no device timing). */
 int mqa_profile_get(const char* kernel, int64_t* launches, double* total_ms, double* bytes);
 int mqa_profile_reset(void);
+/* Streaming-copy probe: times `reps` launches of a plain element-wise copy kernel (one read and
+ * one write of `bytes` each) on two fresh device buffers and returns the average launch time.
+ * The achievable HBM rate of this GPU, and the kernel of known traffic that calibrates the HBM
+ * byte counters of a profiler.  No counterpart in the reference. */
+int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy);
 
 /* ---- circuit (reference: mindquantum/core/circuit/circuit.py:1108-1130 get_cpp_obj) ----
  * Host-side object: validated gate list plus the cached tile-pass schedules.

@@ -1651,6 +1651,32 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
     });
 }
 
+int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy) {
+    return guarded([&] {
+        require(ms_per_copy != nullptr && reps >= 1 && bytes >= 16, "bad argument");
+        if (mqa_device_count() <= 0) throw Error(MQA_ERR_HIP, "no HIP device is visible");
+        StreamOwner so;
+        const uint64_t n = bytes / 16;  // complex128 elements
+        DevBuf a, b;
+        a.ensure(n * 16);
+        b.ensure(n * 16);
+        hip_check(hipMemsetAsync(a.p, 0, n * 16, so.s), "memset");
+        hip_check(launch_scale(1, a.p, b.p, n, 1.0, 0.0, so.s), "copy");  // first touch
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, so.s), "event");
+        for (int r = 0; r < reps; r++) hip_check(launch_scale(1, a.p, b.p, n, 1.0, 0.0, so.s), "copy");
+        hip_check(hipEventRecord(e1, so.s), "event");
+        hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms_per_copy = double(ms) / reps;
+    });
+}
+
 int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out) {
     return guarded([&] {
         require(sim != nullptr && out != nullptr, "NULL argument");

@@ -281,6 +281,26 @@ def test_qaoa_maxcut_small():
         assert_grad(g, g_ref, dtype)
 
 
+def test_hea16_c128_workload():
+    """The CPU-runnable workload: 16-qubit HEA in complex128, 10 layers of RY, RZ on every qubit and
+    a CNOT ladder (320 parameters ~ U[0, 2pi) from seed 0), H = sum_i Z_i Z_{i+1} (open chain) +
+    sum_i 0.5 X_i; one parameter vector.  f and all 320 gradients against the oracle."""
+    n = 16
+    circ = hardware_efficient_ansatz(n, 10)
+    op = mq.QubitOperator("", 0)
+    for i in range(n - 1):
+        op += mq.QubitOperator(f"Z{i} Z{i + 1}")
+    for i in range(n):
+        op += 0.5 * mq.QubitOperator(f"X{i}")
+    vals = np.random.default_rng(0).uniform(0, 2 * np.pi, size=len(circ.params_name))
+    assert len(vals) == 320
+    sim = mq.Simulator("mqvector_amd", n, dtype=mq.complex128)
+    f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, mq.complex128), circ)(vals)
+    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals, mq.complex128)
+    assert abs(f[0, 0] - f_ref[0, 0]) < 1e-10
+    assert_grad(g, g_ref, mq.complex128)
+
+
 def test_torch_autograd_function():
     import torch
 
