@@ -67,16 +67,25 @@ def adjoint_workload(n, layers, batch, seed):
     return circ, op, enc, ans
 
 
-def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile):
+def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile, tune=False):
     import torch
 
     import mindquantum_amd as mq
-    from mindquantum_amd import _native
+    from mindquantum_amd import _native, autotune
     from mindquantum_amd.framework import MQOps
 
     circ, op, enc, ans = adjoint_workload(n, layers, batch, seed)
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
-    ops = MQOps(sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ))
+    grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
+    tiles = None
+    if tune:  # untimed: tile widths of the forward and back-sweep passes, chosen on this workload
+        c64 = dtype == mq.complex64
+        times = autotune.tune_grad(grad_ops, enc, ans, fwd_candidates=(12, 13) if c64 else (11, 12),
+                                   adj_candidates=(11, 12) if c64 else (10, 11))
+        tiles = {"forward_back_sweep": list(min(times, key=times.get)),
+                 "seconds": {f"{f}/{a}": round(t, 4) for (f, a), t in times.items()}}
+        log(f"autotune ({np.dtype(dtype).name}): {tiles}")
+    ops = MQOps(grad_ops)
     x = torch.tensor(enc, dtype=torch.float32, requires_grad=True)
     a = torch.nn.Parameter(torch.tensor(ans, dtype=torch.float32))
 
@@ -102,16 +111,23 @@ def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile):
         for k in ("tile_pass_fwd", "tile_pass_adj", "hamiltonian"):
             stats[k] = _native.profile(k)
     n_params = len(circ.params_name)
+    if tune:
+        stats["tiles"] = tiles
     return batch * steps / dt, dt / steps * 1e3, stats, n_params, len(circ)
 
 
-def run_gate_apply(dtype, n, depth, seed, unfused_depth):
+def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     import mindquantum_amd as mq
-    from mindquantum_amd import _native, ir
+    from mindquantum_amd import _native, autotune, ir
     from mindquantum_amd.algorithm import layered_random_circuit
 
     amp = 8 if dtype == mq.complex64 else 16
     circ = layered_random_circuit(n, depth, seed)
+    tuned = None
+    if tune:  # untimed: the tile width of the passes, chosen on this circuit
+        times = autotune.tune_apply(circ, n, dtype)
+        tuned = {"tile_qubits": min(times, key=times.get), "seconds": {k: round(v, 4) for k, v in times.items()}}
+        log(f"autotune apply {n}q {np.dtype(dtype).name}: {tuned}")
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     code = 0 if dtype == mq.complex64 else 1
     passes = len(ir.NativeCircuit(circ, []).schedule(n, code))
@@ -135,6 +151,7 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth):
         "gate_equivalent_gbs": round(len(circ) * 2 * state / dt / 1e9, 1),
         "kernel_ms": round(ms, 3), "launches": launches,
         "compiled_passes": _native.get_option("jit") != 0, "compile_seconds_untimed": round(compile_s, 1),
+        "autotune": tuned,
     }
     log(f"gate apply {n}q {np.dtype(dtype).name}: {res}")
     # unfused: one gate per pass, on a shorter prefix of the same circuit
@@ -323,6 +340,7 @@ def main():
     ap.add_argument("--items", type=int, default=512, help="circuits of the uneven QAOA batch (0: skip)")
     ap.add_argument("--no-large", action="store_true", help="skip the 32q c64 / 31q c128 single-instance gradients")
     ap.add_argument("--large-checkpoints", type=int, default=4)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile widths (no autotuning)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the device copy probe (counter calibration for profiling runs)")
     args = ap.parse_args()
@@ -344,8 +362,11 @@ def main():
     # per-circuit compiled passes, compiled before the first launch (inside the warmup steps)
     _native.set_option("jit", 0 if args.interpreted else 2)
 
+    tune = not args.no_tune and not args.interpreted
     value, ms_step, stats, n_params, n_gates = run_adjoint(mq.complex64, args.qubits, args.layers, args.batch,
-                                                           args.steps, args.warmup, args.seed, profile=True)
+                                                           args.steps, args.warmup, args.seed, profile=True,
+                                                           tune=tune)
+    tiles = stats.pop("tiles", None)
     log(f"c64: {value:.1f} grads/s, {ms_step:.1f} ms/step")
     launches, ms, byts = stats["tile_pass_adj"]
     avg_ms = ms / max(launches, 1)
@@ -360,18 +381,21 @@ def main():
                 "algorithmic_bytes_per_launch": per_launch, "launches": launches}
     share = {k: round(v[1], 2) for k, v in stats.items()}
 
-    extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3}
+    extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3,
+             "autotune": tiles}
     if not args.no_extra:
         extra["copy_probe"] = copy_probe()
         roofline["achievable_copy_gbs"] = extra["copy_probe"]["gbs"]
-        v128, ms128, _, _, _ = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, 1, 1,
-                                           args.seed, profile=False)
+        v128, ms128, st128, _, _ = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, 1, 1,
+                                               args.seed, profile=False, tune=tune)
+        extra["c128_autotune"] = st128.get("tiles")
         extra["c128_grads_per_s"] = round(v128, 2)
         extra["c128_ms_per_step"] = round(ms128, 2)
         log(f"c128: {v128:.1f} grads/s")
-        extra["gate_apply_28q_c64"] = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 10)
+        extra["gate_apply_28q_c64"] = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 10,
+                                                     tune=tune)
         extra["gate_apply_28q_c128"] = run_gate_apply(mq.complex128, args.apply_qubits, args.apply_depth, args.seed,
-                                                      10)
+                                                      10, tune=tune)
         if args.items > 0:
             extra["uneven_qaoa_c64"] = run_uneven_items(args.items, args.seed, not args.no_cpu)
     if not args.no_large and not args.no_extra:

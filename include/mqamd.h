@@ -131,6 +131,13 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
                        int32_t n_terms, const double* mats, int64_t n_mat_entries, int32_t n_params,
                        const uint8_t* param_requires_grad, mqa_circuit** out);
 int mqa_circuit_destroy(mqa_circuit* circ);
+/* Tile width (qubits per LDS tile) of this circuit's forward (adjoint == 0) or back-sweep passes
+ * in one precision; tile_qubits < 0 returns to the global option ("tile_qubits_c64" ...).  Set by
+ * the autotuner (mindquantum_amd.autotune), which times candidate widths on the device.  The get
+ * call returns the width in effect.  No counterpart in the reference (its CPU kernels have no
+ * tiles). */
+int mqa_circuit_set_tile_qubits(mqa_circuit* circ, int32_t dtype, int32_t adjoint, int32_t tile_qubits);
+int mqa_circuit_get_tile_qubits(const mqa_circuit* circ, int32_t dtype, int32_t adjoint, int32_t* tile_qubits);
 /* Export the tile-pass schedule (host only, no GPU needed) for inspection and tests.
  * Layout of out (int64 words): [n_passes, then per pass: local_mask, n_ops, gate ids...].
  * adjoint != 0 exports the reverse (back-sweep) schedule.  *len gets the words needed. */

@@ -66,6 +66,8 @@ _SIGS = {
     "mqa_get_option": ([ctypes.c_char_p, ctypes.POINTER(_I64)], ctypes.c_int),
     "mqa_circuit_create": ([_P, _I32, _P, _P, _I32, _P, _I64, _I32, _P, _PP], ctypes.c_int),
     "mqa_circuit_destroy": ([_P], ctypes.c_int),
+    "mqa_circuit_set_tile_qubits": ([_P, _I32, _I32, _I32], ctypes.c_int),
+    "mqa_circuit_get_tile_qubits": ([_P, _I32, _I32, ctypes.POINTER(_I32)], ctypes.c_int),
     "mqa_circuit_schedule": ([_P, _I32, _I32, _I32, _P, _I64, ctypes.POINTER(_I64)], ctypes.c_int),
     "mqa_circuit_jit_compile": ([_P, _I32, _I32, _I32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "mqa_ham_create": ([_P, _I32, _PP], ctypes.c_int),

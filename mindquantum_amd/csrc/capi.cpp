@@ -231,8 +231,18 @@ struct mqa_circuit {
     std::map<std::tuple<int, int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
     std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev_slots;  // per device: 6 buffers
 
-    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev) {
+    // per-circuit tile widths (autotuner), [dtype][adjoint]; < 0: the global option
+    std::atomic<int> tile_override[2][2] = {{-1, -1}, {-1, -1}};
+
+    mqa::SchedOptions options(int dtype, bool adjoint) const {
         auto o = mqa::sched_options(dtype, adjoint);
+        const int k = tile_override[dtype == MQA_C64 ? 0 : 1][adjoint ? 1 : 0].load();
+        if (k > 0) o.tile_qubits = k;
+        return o;
+    }
+
+    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev) {
+        auto o = options(dtype, adjoint);
         int dev = upload_dev ? mqa::current_device() : -1;
         auto key = std::make_tuple(n_eff, dtype, int(adjoint), o.tile_qubits, o.low_run, o.group_width,
                                    o.max_ops_per_pass, o.taps_per_group, dev);
@@ -281,7 +291,7 @@ struct mqa_circuit {
     std::map<std::tuple<const mqa::DevPlan*, int, int, int, int, int>, std::unique_ptr<Checkpointed>> ckpt_plans;
 
     Checkpointed* checkpointed(int n_eff, int dtype, int n_ck, const mqa::DevPlan* adj, hipStream_t s) {
-        auto o = mqa::sched_options(dtype, false);
+        auto o = options(dtype, false);
         std::lock_guard<std::mutex> lk(mu);
         auto key = std::make_tuple(adj, n_ck, o.tile_qubits, o.low_run, o.group_width, o.max_ops_per_pass);
         auto it = ckpt_plans.find(key);
@@ -793,6 +803,25 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
     });
 }
 
+int mqa_circuit_set_tile_qubits(mqa_circuit* circ, int32_t dtype, int32_t adjoint, int32_t tile_qubits) {
+    return guarded([&] {
+        require(circ != nullptr, "circuit is NULL");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
+        require(tile_qubits < 0 || (tile_qubits >= kMinTileQubits && tile_qubits <= kMaxTileQubits),
+                "tile qubits must be in [" + std::to_string(kMinTileQubits) + ", " + std::to_string(kMaxTileQubits) +
+                    "] (or negative for the global option)");
+        circ->tile_override[dtype == MQA_C64 ? 0 : 1][adjoint ? 1 : 0].store(tile_qubits < 0 ? -1 : tile_qubits);
+    });
+}
+
+int mqa_circuit_get_tile_qubits(const mqa_circuit* circ, int32_t dtype, int32_t adjoint, int32_t* tile_qubits) {
+    return guarded([&] {
+        require(circ != nullptr && tile_qubits != nullptr, "NULL argument");
+        require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
+        *tile_qubits = circ->options(dtype, adjoint != 0).tile_qubits;
+    });
+}
+
 int mqa_circuit_destroy(mqa_circuit* circ) {
     return guarded([&] { delete circ; });
 }
@@ -828,7 +857,7 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adjoint != 0, sched_options(dtype, adjoint != 0));
+        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adjoint != 0, circ->options(dtype, adjoint != 0));
         JitPlan jp(pl, dtype);
         jp.wait();
         if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());

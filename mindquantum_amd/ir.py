@@ -74,6 +74,21 @@ class NativeCircuit:
             _native.ptr(tc) if len(tc) else None, len(tp), _native.ptr(mats), n_mats, len(self.names), rgp,
             ctypes.byref(h)))
         self.handle = h
+        # tile widths chosen by the autotuner for this circuit: {(dtype code, adjoint): qubits}
+        for (code, adjoint), k in getattr(circ, "_mqa_tiles", {}).items():
+            self.set_tile_qubits(code, adjoint, k)
+
+    def set_tile_qubits(self, dtype_code, adjoint, tile_qubits):
+        """Per-circuit tile width of the forward (adjoint False) or back-sweep passes; negative
+        restores the global option."""
+        _native.check(_native.lib().mqa_circuit_set_tile_qubits(self.handle, int(dtype_code), int(bool(adjoint)),
+                                                                 int(tile_qubits)))
+
+    def tile_qubits(self, dtype_code, adjoint):
+        k = ctypes.c_int32(0)
+        _native.check(_native.lib().mqa_circuit_get_tile_qubits(self.handle, int(dtype_code), int(bool(adjoint)),
+                                                                 ctypes.byref(k)))
+        return k.value
 
     def schedule(self, n_qubits, dtype_code, adjoint=False):
         """Tile-pass schedule: list of (local qubit mask, [gate indices])."""

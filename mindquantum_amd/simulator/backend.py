@@ -299,6 +299,7 @@ class MQVectorAMD:
                                  parallel_worker, self)
         s = f"{self.n_qubits} qubit" + ("" if self.n_qubits == 1 else "s") + f" {self.name} VQA Operator"
         wrapper.set_str(s)
+        wrapper.native_circuits = [nc] + ([ncl] if ncl is not None else [])  # handles the autotuner adjusts
         return wrapper
 
 

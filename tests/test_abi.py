@@ -126,3 +126,26 @@ def test_scheduled_order_gives_the_same_state(seed):
     a = dense.state(circ, values, n)
     b = dense.state(reordered, values, n)
     assert np.allclose(a, b, atol=1e-12)
+
+
+def test_per_circuit_tile_width():
+    """Per-circuit tile widths (set by the autotuner) reach the scheduler and travel with the
+    Circuit object to every handle lowered from it; out-of-range widths are rejected."""
+    from mindquantum_amd.algorithm import hardware_efficient_ansatz
+
+    circ = hardware_efficient_ansatz(16, 2)
+    nc = ir.NativeCircuit(circ, list(circ.params_name))
+    default = nc.tile_qubits(_native.MQA_C64, False)
+    assert default == _native.get_option("tile_qubits_c64")
+    wide = len(nc.schedule(16, _native.MQA_C64))
+    nc.set_tile_qubits(_native.MQA_C64, False, 8)
+    assert nc.tile_qubits(_native.MQA_C64, False) == 8
+    assert len(nc.schedule(16, _native.MQA_C64)) > wide  # narrower tiles: more passes
+    assert nc.tile_qubits(_native.MQA_C128, False) == _native.get_option("tile_qubits_c128")
+    nc.set_tile_qubits(_native.MQA_C64, False, -1)
+    assert nc.tile_qubits(_native.MQA_C64, False) == default
+    with pytest.raises(ValueError):
+        nc.set_tile_qubits(_native.MQA_C64, True, 14)
+    circ._mqa_tiles = {(_native.MQA_C64, True): 10}
+    nc2 = ir.NativeCircuit(circ, list(circ.params_name))
+    assert nc2.tile_qubits(_native.MQA_C64, True) == 10
