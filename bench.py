@@ -154,27 +154,38 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
         "autotune": tuned,
     }
     log(f"gate apply {n}q {np.dtype(dtype).name}: {res}")
-    # unfused: one gate per pass, on a shorter prefix of the same circuit
+    # unfused: one gate per pass, on a shorter prefix of the same circuit.  Single-gate passes
+    # stream memory; they run on the interpreted kernel, with the tile width that streams fastest
+    # (timed here over the widths 10..12; the reference's schedule of one sweep per gate)
     short = circ[: unfused_depth * (n + n // 2)]
-    # single-gate passes stream memory; they run on the interpreted kernel
     _native.set_option("max_ops_per_pass", 1)
     jit = _native.get_option("jit")
     _native.set_option("jit", 0)
+    key = "tile_qubits_c64" if dtype == mq.complex64 else "tile_qubits_c128"
+    per_k = {}
     try:
         sim2 = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
-        sim2.apply_circuit(short[:4])
-        _native.profile_reset()
-        _native.set_option("profile", 1)
-        t0 = time.perf_counter()
-        sim2.apply_circuit(short)
-        dt2 = time.perf_counter() - t0
-        _native.set_option("profile", 0)
-        launches2, ms2, byts2 = _native.profile("tile_pass_fwd")
+        for k in ((10, 11, 12) if tune else (12,)):
+            _native.set_option(key, k)
+            sim2.apply_circuit(short[:4])
+            _native.profile_reset()
+            _native.set_option("profile", 1)
+            t0 = time.perf_counter()
+            sim2.apply_circuit(short)
+            dt2 = time.perf_counter() - t0
+            _native.set_option("profile", 0)
+            launches2, ms2, byts2 = _native.profile("tile_pass_fwd")
+            per_k[k] = (dt2, launches2, ms2, byts2)
     finally:
         _native.set_option("max_ops_per_pass", -1)
         _native.set_option("jit", jit)
-    res["unfused"] = {"gates": len(short), "seconds": round(dt2, 4), "hbm_gbs": round(byts2 / (ms2 * 1e-3) / 1e9, 1),
-                      "launches": launches2}
+        _native.set_option(key, -1)
+    k_best = min(per_k, key=lambda k: per_k[k][2])
+    dt2, launches2, ms2, byts2 = per_k[k_best]
+    gbs2 = byts2 / (ms2 * 1e-3) / 1e9
+    res["unfused"] = {"gates": len(short), "seconds": round(dt2, 4), "hbm_gbs": round(gbs2, 1),
+                      "frac_of_peak": round(gbs2 / HBM_PEAK_GBS, 4), "launches": launches2, "tile_qubits": k_best,
+                      "hbm_gbs_by_tile_qubits": {k: round(v[3] / (v[2] * 1e-3) / 1e9, 1) for k, v in per_k.items()}}
     del sim, sim2
     return res
 

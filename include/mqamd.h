@@ -216,6 +216,14 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim, const mqa_sim* left, const mqa_ham*
 int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
                    const int32_t* n_qubits, const double* params, const int64_t* param_offset, int32_t dtype,
                    double* f_out, double* g_out);
+/* sampling_measure_ending_without_noise (vector_state.tpp:1326-1373): applies circ (the circuit
+ * without its measurements; NULL: none) to a copy of the state and returns, for each of the `shots`
+ * sorted uniform samples u in [0, 1), the amplitude index i with C(i-1) < u * C(dim-1) <= C(i),
+ * C the cumulative probability (lower bound).  The caller draws the uniforms (the reference draws
+ * them from its seeded mt19937; results agree in distribution, not sample by sample) and reads the
+ * measured qubits' bits from the indices.  out_index: [shots]. */
+int mqa_sim_sample(const mqa_sim* sim, const mqa_circuit* circ, const double* params, int32_t n_params,
+                   const double* uniforms, uint64_t shots, uint64_t* out_index);
 /* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
  * state cast to dtype. */
 int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out);

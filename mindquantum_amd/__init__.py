@@ -6,7 +6,7 @@ in the HIP library ``libmqamd.so`` (C ABI in include/mqamd.h).
 """
 from . import dtype
 from .core import (CNOT, FSIM, H, I, ISWAP, RX, RY, RZ, SWAP, SX, U3, UN, BarrierGate, Circuit, FSim, Givens,
-                   GlobalPhase, GroupedPauli, Hamiltonian, ParameterResolver, PhaseShift, Power, QubitOperator, Rn,
+                   GlobalPhase, GroupedPauli, Hamiltonian, Measure, MeasureResult, ParameterResolver, PhaseShift, Power, QubitOperator, Rn,
                    RotPauliString, Rxx, Rxy, Rxz, Ryy, Ryz, Rzz, S, SWAPalpha, T, UnivMathGate, X, Y, Z)
 from .core import gates
 from .dtype import complex64, complex128

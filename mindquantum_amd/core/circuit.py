@@ -120,8 +120,45 @@ class Circuit(list):
         return any(getattr(g, "parameterized", False) for g in self)
 
     @property
+    def all_measures(self):
+        """Measurement gates in circuit order."""
+        from .measurement import Measure
+
+        return [g for g in self if isinstance(g, Measure)]
+
+    @property
     def has_measure_gate(self):
-        return False
+        return bool(self.all_measures)
+
+    @property
+    def is_measure_end(self):
+        """Every measurement comes after the last other gate acting on its qubit (and each qubit is
+        measured at most once)."""
+        from .measurement import Measure
+
+        done = set()
+        for g in self:
+            qs = set(g.obj_qubits) | set(g.ctrl_qubits)
+            if qs & done:
+                return False
+            if isinstance(g, Measure):
+                done |= qs
+        return True
+
+    def remove_measure(self):
+        from .measurement import Measure
+
+        return Circuit([g for g in self if not isinstance(g, Measure)])
+
+    def measure(self, key, obj=None):
+        """Append ``Measure(key).on(obj)`` (``measure(q)``: key ``q<q>``)."""
+        from .measurement import Measure
+
+        if obj is None:
+            self.append(Measure().on(key))
+        else:
+            self.append(Measure(key).on(obj))
+        return self
 
     @property
     def is_noise_circuit(self):

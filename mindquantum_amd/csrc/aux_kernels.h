@@ -10,4 +10,17 @@ namespace mqa {
 // 1 = complex128.
 hipError_t launch_scale(int dtype, const void* src, void* dst, uint64_t n_amps, double re, double im, hipStream_t s);
 
+// Sampling (measure-ending circuits).  Amplitudes are split into blocks of kSampleBlock (or the
+// whole state when smaller).
+constexpr uint32_t kSampleBlock = 4096;
+// out[b] = sum of |psi[j]|^2 over block b (double), for n_blocks = dim / block blocks.
+hipError_t launch_prob_block_sums(int dtype, const void* psi, uint64_t dim, uint32_t block, double* out,
+                                  hipStream_t s);
+// One workgroup per touched block: the block's inclusive prefix of |psi|^2 in LDS, then for every
+// shot s in [shot_begin[g], shot_end[g]) the first index j of the block with prefix[j] >= residual[s]
+// (lower bound; clamped to the block) -> out[s] = block_id[g] * block + j.
+hipError_t launch_sample_blocks(int dtype, const void* psi, uint32_t block, const uint64_t* block_id,
+                                const uint64_t* shot_begin, const uint64_t* shot_end, uint64_t n_groups,
+                                const double* residual, uint64_t* out, hipStream_t s);
+
 }  // namespace mqa

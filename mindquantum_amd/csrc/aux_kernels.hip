@@ -1,4 +1,5 @@
-// aux_kernels.hip - small element-wise kernels for gfx950: complex scaling of a state.
+// aux_kernels.hip - small kernels for gfx950: complex scaling of a state, probability block sums
+// and the per-block search of measurement sampling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +33,105 @@ __global__ void __launch_bounds__(256) k_scale(const typename Cplx<T>::V* src, t
     }
 }
 
+// Block sums of |psi|^2: one workgroup of 256 threads per block, double accumulation, fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prob_block_sums(const typename Cplx<T>::V* psi, uint32_t block, double* out) {
+    __shared__ double part[256];
+    const typename Cplx<T>::V* p = psi + uint64_t(blockIdx.x) * block;
+    double acc = 0;
+    for (uint32_t j = threadIdx.x; j < block; j += blockDim.x) {
+        const typename Cplx<T>::V v = p[j];
+        acc += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+// Per touched block: inclusive prefix of |psi|^2 over the block in LDS (each thread scans a
+// contiguous chunk, a block-wide scan of the chunk totals adds the offsets), then a binary search per
+// shot.  block <= kSampleBlock, a multiple of 256 or smaller than 256 (one amplitude per thread).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sample_blocks(const typename Cplx<T>::V* psi, uint32_t block,
+                                                       const uint64_t* block_id, const uint64_t* shot_begin,
+                                                       const uint64_t* shot_end, const double* residual,
+                                                       uint64_t* out) {
+    __shared__ double pre[kSampleBlock];
+    __shared__ double tot[256];
+    const uint64_t b = block_id[blockIdx.x];
+    const typename Cplx<T>::V* p = psi + b * block;
+    const uint32_t per = (block + 255u) / 256u;  // amplitudes per thread
+    const uint32_t j0 = threadIdx.x * per;
+    double run = 0;
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t j = j0 + k;
+        if (j < block) {
+            const typename Cplx<T>::V v = p[j];
+            run += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+            pre[j] = run;
+        }
+    }
+    tot[threadIdx.x] = run;
+    __syncthreads();
+    // inclusive scan of the 256 chunk totals (Hillis-Steele, two buffers' worth of syncs)
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const double add = threadIdx.x >= off ? tot[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        tot[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const double base = threadIdx.x > 0 ? tot[threadIdx.x - 1] : 0.0;
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t j = j0 + k;
+        if (j < block) pre[j] += base;
+    }
+    __syncthreads();
+    for (uint64_t sh = shot_begin[blockIdx.x] + threadIdx.x; sh < shot_end[blockIdx.x]; sh += blockDim.x) {
+        const double r = residual[sh];
+        uint32_t lo = 0, hi = block - 1;  // first j with pre[j] >= r, at most block - 1
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] >= r) hi = mid;
+            else lo = mid + 1;
+        }
+        out[sh] = b * block + lo;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_prob_block_sums(int dtype, const void* psi, uint64_t dim, uint32_t block, double* out,
+                                  hipStream_t s) {
+    if (block == 0 || dim % block != 0) return hipErrorInvalidValue;
+    const uint64_t nb = dim / block;
+    if (nb == 0 || nb > 0x7fffffffull) return hipErrorInvalidValue;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_prob_block_sums<float>, dim3(unsigned(nb)), dim3(256), 0, s,
+                           static_cast<const float2*>(psi), block, out);
+    else
+        hipLaunchKernelGGL(k_prob_block_sums<double>, dim3(unsigned(nb)), dim3(256), 0, s,
+                           static_cast<const double2*>(psi), block, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_blocks(int dtype, const void* psi, uint32_t block, const uint64_t* block_id,
+                                const uint64_t* shot_begin, const uint64_t* shot_end, uint64_t n_groups,
+                                const double* residual, uint64_t* out, hipStream_t s) {
+    if (n_groups == 0) return hipSuccess;
+    if (block == 0 || block > kSampleBlock || (block > 256 && block % 256 != 0) || n_groups > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_sample_blocks<float>, dim3(unsigned(n_groups)), dim3(256), 0, s,
+                           static_cast<const float2*>(psi), block, block_id, shot_begin, shot_end, residual, out);
+    else
+        hipLaunchKernelGGL(k_sample_blocks<double>, dim3(unsigned(n_groups)), dim3(256), 0, s,
+                           static_cast<const double2*>(psi), block, block_id, shot_begin, shot_end, residual, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_scale(int dtype, const void* src, void* dst, uint64_t n_amps, double re, double im, hipStream_t s) {
     if (n_amps == 0) return hipSuccess;

@@ -1680,6 +1680,76 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
     });
 }
 
+int mqa_sim_sample(const mqa_sim* sim_c, const mqa_circuit* circ_c, const double* params, int32_t n_params,
+                   const double* uniforms, uint64_t shots, uint64_t* out_index) {
+    return guarded([&] {
+        require(sim_c != nullptr && out_index != nullptr, "NULL argument");
+        require(shots == 0 || uniforms != nullptr, "uniforms is NULL");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        if (circ_c) check_circuit(sim, circ_c, n_params);
+        for (uint64_t i = 0; i < shots; i++) {
+            require(uniforms[i] >= 0.0 && uniforms[i] < 1.0, "uniform samples must lie in [0, 1)");
+            require(i == 0 || uniforms[i] >= uniforms[i - 1], "uniform samples must be sorted");
+        }
+        if (shots == 0) return;
+        DeviceGuard dg(sim->device);
+        hipStream_t s = sim->stream;
+        const size_t sb = sim->state_bytes();
+        // the circuit runs on a copy: the simulator's own state is left unchanged
+        sim->psi.ensure(sb);
+        hip_check(hipMemcpyAsync(sim->psi.p, sim->state.p, sb, hipMemcpyDeviceToDevice, s), "copy");
+        auto* circ = const_cast<mqa_circuit*>(circ_c);
+        if (circ && !circ->gates.empty()) {
+            DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
+            const double* ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
+            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, circ->n_slots, nullptr, 0);
+        }
+        // cumulative probability by blocks (device sums, host prefix), then the search inside the
+        // blocks the shots land in (GetCumulativeProbs / LowerBound, vector_state.tpp:1357-1361)
+        const uint32_t block = uint32_t(std::min<uint64_t>(kSampleBlock, sim->dim_eff));
+        const uint64_t nb = sim->dim_eff / block;
+        sim->fpart.ensure(nb * sizeof(double));
+        hip_check(launch_prob_block_sums(sim->dtype, sim->psi.p, sim->dim_eff, block, sim->fpart.as<double>(), s),
+                  "probability sums");
+        std::vector<double> bsum(nb);
+        hip_check(hipMemcpyAsync(bsum.data(), sim->fpart.p, nb * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "sync");
+        std::vector<double> cum(nb + 1, 0.0);
+        for (uint64_t b = 0; b < nb; b++) cum[b + 1] = cum[b] + bsum[b];
+        const double total = cum[nb];
+        if (!(total > 0)) throw Error(MQA_ERR_RUNTIME, "the state has zero norm");
+        std::vector<uint64_t> gid, gbeg, gend;
+        std::vector<double> resid(shots);
+        for (uint64_t i = 0; i < shots; i++) {
+            const double t = uniforms[i] * total;
+            // block b with cum[b] <= t < cum[b + 1]; blocks of zero mass are never chosen
+            uint64_t b = uint64_t(std::upper_bound(cum.begin(), cum.end(), t) - cum.begin());
+            b = (b == 0) ? 0 : b - 1;
+            if (b >= nb) b = nb - 1;
+            while (b > 0 && bsum[b] <= 0) b--;
+            resid[i] = t - cum[b];
+            if (gid.empty() || gid.back() != b) {
+                if (!gid.empty()) gend.push_back(i);
+                gid.push_back(b);
+                gbeg.push_back(i);
+            }
+        }
+        gend.push_back(shots);
+        DevBuf d_gid, d_gbeg, d_gend, d_res, d_out;
+        upload(d_gid, gid, s);
+        upload(d_gbeg, gbeg, s);
+        upload(d_gend, gend, s);
+        upload(d_res, resid, s);
+        d_out.ensure(shots * sizeof(uint64_t));
+        hip_check(launch_sample_blocks(sim->dtype, sim->psi.p, block, d_gid.as<uint64_t>(), d_gbeg.as<uint64_t>(),
+                                       d_gend.as<uint64_t>(), uint64_t(gid.size()), d_res.as<double>(),
+                                       d_out.as<uint64_t>(), s),
+                  "sample");
+        hip_check(hipMemcpyAsync(out_index, d_out.p, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "d2h");
+        sync(sim, "sample");
+    });
+}
+
 int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy) {
     return guarded([&] {
         require(ms_per_copy != nullptr && reps >= 1 && bytes >= 16, "bad argument");

@@ -124,10 +124,56 @@ class MQVectorAMD:
         self._check_circ(circuit)
         if len(circuit) == 0:
             return None
+        if circuit.has_measure_gate:
+            raise ValueError("apply_circuit here runs unitary circuits; sample measurements with sampling()")
         names, vals = resolve(circuit, pr)
         nc = self._native_circuit(circuit, names)
         _native.check(_native.lib().mqa_sim_apply_circuit(self._h, nc.handle, _native.ptr(vals), len(vals)))
         return None
+
+    def sampling(self, circuit, pr=None, shots=1, seed=None):
+        """Sample a measurement-ending circuit (mqsim.py:456-491, vector_state.tpp:1326-1373): the
+        non-measurement gates run on a copy of the state, the device draws amplitude indices by
+        cumulative probability for sorted uniforms from numpy's generator seeded with ``seed``, and
+        the measured qubits' bits are collected per key.  Agrees with the reference in distribution
+        (its mt19937 stream is not reproduced)."""
+        from ..core.measurement import MeasureResult
+
+        self._check_circ(circuit)
+        measures = circuit.all_measures
+        if not measures:
+            raise ValueError("circuit must have at least one measurement gate.")
+        if circuit.n_qubits > self.n_qubits:
+            raise ValueError(f"Circuit has {circuit.n_qubits} qubits, which is more than simulator qubits.")
+        if not isinstance(shots, (int, np.integer)) or isinstance(shots, bool):
+            raise TypeError(f"sampling shots requires a int, but get {type(shots)}")
+        if shots < 1:
+            raise ValueError(f"sampling shots should be not less than 1, but get {shots}")
+        if not circuit.is_measure_end:
+            raise ValueError("only measurement-ending circuits are sampled here (every measurement after the last "
+                             "gate on its qubit)")
+        keys = [m.key for m in measures]
+        if len(set(keys)) != len(keys):
+            raise ValueError("measure keys must be unique")
+        body = circuit.remove_measure()
+        if seed is None:
+            seed = int(np.random.randint(1, 2 << 20))
+        u = np.sort(np.random.default_rng(seed).random(int(shots)))
+        idx = np.zeros(int(shots), dtype=np.uint64)
+        handle, vals = None, np.zeros(0)
+        if len(body):
+            names, vals = resolve(body, pr)
+            handle = self._native_circuit(body, names).handle
+        _native.check(_native.lib().mqa_sim_sample(self._h, handle, _native.ptr(vals) if len(vals) else None,
+                                                    len(vals), _native.ptr(u), int(shots), _native.ptr(idx)))
+        res = MeasureResult()
+        res.add_measure(measures)
+        km = res.keys_map
+        samples = np.zeros((int(shots), len(keys)), dtype=np.int64)
+        for m in measures:
+            samples[:, km[m.key]] = ((idx >> np.uint64(m.obj_qubits[0])) & np.uint64(1)).astype(np.int64)
+        res.collect_data(samples)
+        return res
 
     def apply_hamiltonian(self, hamiltonian):
         self._check_ham(hamiltonian)
