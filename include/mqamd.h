@@ -153,6 +153,14 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
 
 /* ---- Hamiltonian (reference: ccsrc/include/ops/hamiltonian.h:30-55, how_to_ == ORIGIN) ---- */
 int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out);
+/* Sparse Hamiltonian from a dim x dim CSR matrix (hamiltonian.py:59-76, how_to_ == FRONTEND;
+ * Csr_Dot_Vec / ExpectationOfCsr, cpu_vector_core_dot_like.cpp:132-225): indptr [dim + 1],
+ * indices [nnz] (< dim), data [nnz] interleaved complex, in the little-endian amplitude order.  dim
+ * must equal 2^n of the simulators it is used with.  The operator need not be hermitian: the
+ * non-hermitian gradient applies its conjugate transpose where the reference uses herm_hams.
+ * Not accepted by mqa_grad_items. */
+int mqa_ham_create_csr(uint64_t dim, uint64_t nnz, const int64_t* indptr, const int64_t* indices, const double* data,
+                       mqa_ham** out);
 int mqa_ham_destroy(mqa_ham* ham);
 
 /* ---- simulator (reference: ccsrc/include/python/vector/bind_vec_state.h:46-84, BindSim) ---- */

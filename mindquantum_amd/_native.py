@@ -71,6 +71,7 @@ _SIGS = {
     "mqa_circuit_schedule": ([_P, _I32, _I32, _I32, _P, _I64, ctypes.POINTER(_I64)], ctypes.c_int),
     "mqa_circuit_jit_compile": ([_P, _I32, _I32, _I32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "mqa_ham_create": ([_P, _I32, _PP], ctypes.c_int),
+    "mqa_ham_create_csr": ([_U64, _U64, _P, _P, _P, _PP], ctypes.c_int),
     "mqa_ham_destroy": ([_P], ctypes.c_int),
     "mqa_sim_create": ([_I32, _I32, ctypes.c_uint32, _PP], ctypes.c_int),
     "mqa_sim_destroy": ([_P], ctypes.c_int),

@@ -1,9 +1,9 @@
 """QubitOperator and Hamiltonian (Pauli-term form).
 
-Host-side mirror of mindquantum/core/operators/qubit_operator.py and hamiltonian.py for the
-``how_to == ORIGIN`` path: a Hamiltonian is a list of Pauli strings with real coefficients
-(hamiltonian.py:90-101, 214-244).  ``terms_to_ir`` turns it into the (x, y, z, coeff) masks
-that the kernels consume.
+Host-side mirror of mindquantum/core/operators/qubit_operator.py and hamiltonian.py: a Hamiltonian
+is a list of Pauli strings with real coefficients (hamiltonian.py:90-101, 214-244; ``terms_ir``
+turns it into the (x, y, z, coeff) masks that the kernels consume), or a CSR sparse matrix
+(``how_to == FRONTEND``).
 """
 import numbers
 import re
@@ -136,33 +136,88 @@ class QubitOperator:
         return " +\n".join(parts) if parts else "0"
 
 
+ORIGIN, BACKEND, FRONTEND = 0, 1, 2  # HowTo of hamiltonian.py
+
+
 class Hamiltonian:
-    """Pauli-term Hamiltonian with real coefficients (hamiltonian.py:35-244)."""
+    """Hamiltonian (hamiltonian.py:35-244): Pauli terms with real coefficients, or a
+    ``scipy.sparse.csr_matrix`` of size 2^n x 2^n (``how_to == FRONTEND``: applied as a sparse
+    matrix-vector product on the device).  ``sparse(n)`` on a Pauli Hamiltonian marks it
+    ``BACKEND`` as the reference does; it keeps running on the Pauli-term engine, which computes the
+    same operator."""
 
     def __init__(self, hamiltonian, dtype=None):
+        import scipy.sparse as sp
+
         from ..dtype import to_mq_type  # local import: dtype imports nothing from here
 
+        self._sparse = None
         if isinstance(hamiltonian, Hamiltonian):
-            hamiltonian = hamiltonian.hamiltonian
-        if not isinstance(hamiltonian, QubitOperator):
-            raise TypeError(f"Hamiltonian requires a QubitOperator, but get {type(hamiltonian)}")
-        for v in hamiltonian.terms.values():
-            if abs(complex(v).imag) > 1e-12:
-                raise ValueError("Hamiltonian requires real coefficients (a hermitian operator).")
-        self.hamiltonian = hamiltonian
+            hamiltonian = hamiltonian._sparse if hamiltonian.how_to == FRONTEND else hamiltonian.hamiltonian
+        if sp.issparse(hamiltonian):
+            if not isinstance(hamiltonian, (sp.csr_matrix, sp.csr_array)):
+                raise TypeError(f"Require a QubitOperator or a csr_matrix, but get {type(hamiltonian)}!")
+            if len(hamiltonian.shape) != 2 or hamiltonian.shape[0] != hamiltonian.shape[1]:
+                raise ValueError(f"Hamiltonian requires a two dimension square csr_matrix, but get shape "
+                                 f"{hamiltonian.shape}")
+            dim = hamiltonian.shape[0]
+            if dim < 1 or dim & (dim - 1):
+                raise ValueError(f"size of hamiltonian sparse matrix should be power of 2, but get {hamiltonian.shape}")
+            self._sparse = sp.csr_matrix(hamiltonian, dtype=np.complex128)
+            self._sparse.sort_indices()
+            self.hamiltonian = None
+            self.how_to = FRONTEND
+            self.n_qubits = int(dim).bit_length() - 1
+        else:
+            if not isinstance(hamiltonian, QubitOperator):
+                raise TypeError(f"Hamiltonian requires a QubitOperator, but get {type(hamiltonian)}")
+            for v in hamiltonian.terms.values():
+                if abs(complex(v).imag) > 1e-12:
+                    raise ValueError("Hamiltonian requires real coefficients (a hermitian operator).")
+            self.hamiltonian = hamiltonian
+            self.how_to = ORIGIN
+            self.n_qubits = hamiltonian.count_qubits
         self.dtype = to_mq_type(dtype if dtype is not None else "complex128")
-        self.how_to = 0  # ORIGIN
-        self.n_qubits = hamiltonian.count_qubits
         self._native = None
 
     def astype(self, dtype):
-        return Hamiltonian(self.hamiltonian, dtype)
+        out = Hamiltonian(self._sparse if self.how_to == FRONTEND else self.hamiltonian, dtype)
+        if self.how_to == BACKEND:
+            out.how_to, out.n_qubits = BACKEND, self.n_qubits
+        return out
 
-    def sparse(self, n_qubits=1):  # pragma: no cover - CSR Hamiltonians are not on this path
-        raise NotImplementedError("sparse (CSR) Hamiltonians are not supported by the HIP backend yet")
+    def sparse(self, n_qubits=1):
+        """Mark the Pauli Hamiltonian for the sparse backend form (hamiltonian.py:137-151)."""
+        if self.how_to != ORIGIN:
+            raise ValueError("Already a sparse hamiltonian.")
+        if n_qubits < self.n_qubits:
+            raise ValueError(f"Can not sparse a {self.n_qubits} qubits hamiltonian to {n_qubits} hamiltonian.")
+        self.n_qubits = n_qubits
+        self.how_to = BACKEND
+        return self
+
+    def sparse_matrix(self):
+        """The Hamiltonian as a csr_matrix (little-endian qubit order)."""
+        import scipy.sparse as sp
+
+        if self.how_to == FRONTEND:
+            return self._sparse.copy()
+        n = max(self.n_qubits, 1)
+        pauli = {"X": sp.csr_matrix([[0, 1], [1, 0]], dtype=complex), "Y": sp.csr_matrix([[0, -1j], [1j, 0]]),
+                 "Z": sp.csr_matrix([[1, 0], [0, -1]], dtype=complex)}
+        out = sp.csr_matrix((1 << n, 1 << n), dtype=np.complex128)
+        for key, coeff in self.hamiltonian.terms.items():
+            ops = dict((q, p) for q, p in key)
+            m = sp.identity(1, dtype=np.complex128, format="csr")
+            for q in range(n - 1, -1, -1):
+                m = sp.kron(m, pauli[ops[q]] if q in ops else sp.identity(2, format="csr"), format="csr")
+            out = out + complex(coeff) * m
+        return out.tocsr()
 
     def terms_ir(self):
         """(x, y, z, coeff) masks of every term, as the C struct array of include/mqamd.h."""
+        if self.how_to == FRONTEND:
+            raise ValueError("this Hamiltonian was constructed from a sparse matrix; it has no Pauli terms")
         arr = np.zeros(len(self.hamiltonian.terms), dtype=_native.TERM_DTYPE)
         for i, (k, v) in enumerate(self.hamiltonian.terms.items()):
             for q, p in k:
@@ -176,9 +231,19 @@ class Hamiltonian:
         if self._native is None:
             import ctypes
 
-            terms = self.terms_ir()
             h = ctypes.c_void_p()
-            _native.check(_native.lib().mqa_ham_create(_native.ptr(terms), len(terms), ctypes.byref(h)))
+            if self.how_to == FRONTEND:
+                m = self._sparse
+                ptr = np.ascontiguousarray(m.indptr, dtype=np.int64)
+                idx = np.ascontiguousarray(m.indices, dtype=np.int64)
+                val = np.ascontiguousarray(m.data, dtype=np.complex128).view(np.float64)
+                _native.check(_native.lib().mqa_ham_create_csr(m.shape[0], m.nnz, _native.ptr(ptr),
+                                                                _native.ptr(idx) if m.nnz else None,
+                                                                _native.ptr(val) if m.nnz else None,
+                                                                ctypes.byref(h)))
+            else:
+                terms = self.terms_ir()
+                _native.check(_native.lib().mqa_ham_create(_native.ptr(terms), len(terms), ctypes.byref(h)))
             self._native = h
         return self._native
 
@@ -194,4 +259,4 @@ class Hamiltonian:
         return d
 
     def __repr__(self):
-        return repr(self.hamiltonian)
+        return repr(self._sparse) if self.how_to == FRONTEND else repr(self.hamiltonian)

@@ -10,6 +10,14 @@ namespace mqa {
 // 1 = complex128.
 hipError_t launch_scale(int dtype, const void* src, void* dst, uint64_t n_amps, double re, double im, hipStream_t s);
 
+// out[b][r] = sum_k data[k] * ket[b][indices[k]] over indptr[r] <= k < indptr[r + 1] for r < rows, and 0
+// for rows <= r < out_rows (padding amplitudes), for every sample b < batch (CSR matrix times a
+// batch of states; data holds interleaved complex values in the state's precision).  Column
+// indices must be < rows (checked where the matrix is created).
+hipError_t launch_csr_spmv(int dtype, const int64_t* indptr, const int64_t* indices, const void* data, uint64_t rows,
+                           const void* ket, uint64_t ket_stride, void* out, uint64_t out_stride, uint64_t out_rows,
+                           uint64_t batch, hipStream_t s);
+
 // Sampling (measure-ending circuits).  Amplitudes are split into blocks of kSampleBlock (or the
 // whole state when smaller).
 constexpr uint32_t kSampleBlock = 4096;

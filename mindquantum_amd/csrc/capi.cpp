@@ -380,6 +380,37 @@ struct mqa_ham {
         return raw;
     }
 
+    // sparse form (a CSR matrix given by the caller): [0] the operator, [1] its conjugate transpose
+    bool csr = false;
+    uint64_t csr_dim = 0;
+    std::vector<int64_t> csr_ptr[2], csr_idx[2];
+    std::vector<double> csr_val[2];  // interleaved complex
+    std::map<std::pair<int, int>, std::unique_ptr<mqa::DevBuf[]>> csr_dev;  // (device, dtype) -> 6 buffers
+
+    mqa::DevBuf* csr_on_device(int dtype, hipStream_t s) {
+        const int d = mqa::current_device();
+        std::lock_guard<std::mutex> lk(mu);
+        auto key = std::make_pair(d, dtype);
+        auto it = csr_dev.find(key);
+        if (it != csr_dev.end()) return it->second.get();
+        std::unique_ptr<mqa::DevBuf[]> b(new mqa::DevBuf[6]);
+        std::vector<float> vf[2];
+        for (int w = 0; w < 2; w++) {
+            mqa::upload(b[3 * w], csr_ptr[w], s);
+            mqa::upload(b[3 * w + 1], csr_idx[w], s);
+            if (dtype == MQA_C64) {
+                vf[w].assign(csr_val[w].begin(), csr_val[w].end());
+                mqa::upload(b[3 * w + 2], vf[w], s);
+            } else {
+                mqa::upload(b[3 * w + 2], csr_val[w], s);
+            }
+        }
+        mqa::hip_check(hipStreamSynchronize(s), "csr upload");
+        auto* raw = b.get();
+        csr_dev[key] = std::move(b);
+        return raw;
+    }
+
     mqa::DevBuf* on_device(hipStream_t s) {
         int d = mqa::current_device();
         std::lock_guard<std::mutex> lk(mu);
@@ -574,8 +605,46 @@ bool ham_apply_tiles(mqa_sim* sim, mqa_ham* ham, const void* ket, void* out, boo
     return true;
 }
 
+// Sparse Hamiltonian: out = H ket (or H^dagger ket) by CSR products, then <bra|out> per sample.
+void csr_apply(mqa_sim* sim, mqa_ham* ham, const void* ket, const void* bra, void* out, uint64_t batch, bool dagger) {
+    DevBuf* d = ham->csr_on_device(sim->dtype, sim->stream);
+    const int w = dagger ? 1 : 0;
+    void* target = out;
+    if (!target) {
+        sim->scratch_state.ensure(sim->state_bytes() * batch);
+        target = sim->scratch_state.p;
+    }
+    timed(sim, "hamiltonian", double(batch) * double(sim->state_bytes()) * 2.0, [&] {
+        hip_check(launch_csr_spmv(sim->dtype, d[3 * w].as<int64_t>(), d[3 * w + 1].as<int64_t>(), d[3 * w + 2].p,
+                                  ham->csr_dim, ket, sim->dim_eff, target, sim->dim_eff, sim->dim_eff, batch,
+                                  sim->stream),
+                  "sparse hamiltonian");
+    });
+    if (!bra) return;
+    VdotArgs v{};
+    v.a = bra;
+    v.a_stride = sim->dim_eff;
+    v.b = target;
+    v.b_stride = sim->dim_eff;
+    v.dim = sim->dim_eff;
+    v.amps_per_block = uint32_t(std::min<uint64_t>(sim->dim_eff, 4096));
+    v.n_blocks = uint32_t((sim->dim_eff + v.amps_per_block - 1) / v.amps_per_block);
+    sim->fpart.ensure(batch * v.n_blocks * 2 * sizeof(double));
+    sim->fsum.ensure(batch * 2 * sizeof(double));
+    v.partial = sim->fpart.as<double>();
+    hip_check(launch_vdot(sim->dtype, v, batch, sim->stream), "vdot");
+    hip_check(launch_reduce_rows(sim->fpart.as<double>(), v.n_blocks, sim->fsum.as<double>(), batch, sim->stream),
+              "reduce");
+}
+
 // <bra|H|ket> or H ket over a batch; returns per-sample sums in sim->fsum (device, [batch][2]).
-void ham_apply(mqa_sim* sim, mqa_ham* ham, const void* ket, const void* bra, void* out, uint64_t batch) {
+// dagger: apply H^dagger (differs from H only for a sparse Hamiltonian given as a matrix).
+void ham_apply(mqa_sim* sim, mqa_ham* ham, const void* ket, const void* bra, void* out, uint64_t batch,
+               bool dagger = false) {
+    if (ham->csr) {
+        csr_apply(sim, ham, ket, bra, out, batch, dagger);
+        return;
+    }
     if (out && (bra == nullptr || bra == ket) && ham_apply_tiles(sim, ham, ket, out, bra != nullptr, batch)) return;
     DevBuf* hd = ham->on_device(sim->stream);
     HamArgs h{};
@@ -631,6 +700,10 @@ void check_circuit(const mqa_sim* sim, const mqa_circuit* c, int n_params) {
 
 void check_ham(const mqa_sim* sim, const mqa_ham* h) {
     require(h != nullptr, "hamiltonian is NULL");
+    if (h->csr) {  // cpu_vector_core_dot_like.cpp:134-137
+        require(h->csr_dim == sim->dim, "Sparse hamiltonian size not match with quantum state size.");
+        return;
+    }
     require((h->qubit_mask >> sim->n) == 0 || sim->n >= 64, "hamiltonian acts on more qubits than the simulator has");
 }
 
@@ -904,6 +977,47 @@ int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out) 
             g.n_terms = 0;
             h->groups.push_back(g);
         }
+        *out = h.release();
+    });
+}
+
+int mqa_ham_create_csr(uint64_t dim, uint64_t nnz, const int64_t* indptr, const int64_t* indices, const double* data,
+                       mqa_ham** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(dim >= 1 && (dim & (dim - 1)) == 0 && dim <= (uint64_t(1) << 36),
+                "size of hamiltonian sparse matrix should be power of 2");
+        require(indptr != nullptr && (nnz == 0 || (indices != nullptr && data != nullptr)), "NULL argument");
+        require(indptr[0] == 0 && uint64_t(indptr[dim]) == nnz, "bad CSR row pointer");
+        for (uint64_t r = 0; r < dim; r++) require(indptr[r + 1] >= indptr[r], "bad CSR row pointer");
+        for (uint64_t e = 0; e < nnz; e++)
+            require(indices[e] >= 0 && uint64_t(indices[e]) < dim, "CSR column index out of range");
+        auto h = std::make_unique<mqa_ham>();
+        h->csr = true;
+        h->csr_dim = dim;
+        h->qubit_mask = dim - 1;
+        h->csr_ptr[0].assign(indptr, indptr + dim + 1);
+        h->csr_idx[0].assign(indices, indices + nnz);
+        h->csr_val[0].assign(data, data + 2 * nnz);
+        // conjugate transpose (counting sort by column)
+        auto& tp = h->csr_ptr[1];
+        auto& ti = h->csr_idx[1];
+        auto& tv = h->csr_val[1];
+        tp.assign(dim + 1, 0);
+        for (uint64_t e = 0; e < nnz; e++) tp[size_t(indices[e]) + 1]++;
+        for (uint64_t r = 0; r < dim; r++) tp[r + 1] += tp[r];
+        ti.assign(nnz, 0);
+        tv.assign(2 * nnz, 0.0);
+        std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
+        for (uint64_t r = 0; r < dim; r++)
+            for (int64_t e = indptr[r]; e < indptr[r + 1]; e++) {
+                const int64_t at = fill[size_t(indices[e])]++;
+                ti[size_t(at)] = int64_t(r);
+                tv[2 * size_t(at)] = data[2 * e];
+                tv[2 * size_t(at) + 1] = -data[2 * e + 1];
+            }
+        DevHamGroup g{};  // the Pauli tables stay empty
+        h->groups.push_back(g);
         *out = h.release();
     });
 }
@@ -1521,6 +1635,7 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             const int n = n_qubits[i];
             require(n >= 1 && n <= 36, "item " + std::to_string(i) + ": bad qubit count");
             require(c->max_qubit < n, "item " + std::to_string(i) + ": circuit is wider than its qubit count");
+            require(!h->csr, "item " + std::to_string(i) + ": sparse (CSR) hamiltonians are not supported in item batches");
             require((h->qubit_mask >> n) == 0, "item " + std::to_string(i) + ": hamiltonian is wider than its qubit count");
             require(c->n_params == 0 || params != nullptr, "params is NULL");
             GradItem it;
@@ -1631,11 +1746,11 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
                     run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0);
                 }
                 auto* ham = const_cast<mqa_ham*>(hams[h]);
-                ham_apply(sim, ham, P + 2 * sb, nullptr, L, 1);  // H phi_l (H is hermitian: real coefficients)
-                ham_apply(sim, ham, P, nullptr, L + 2 * sb, 1);  // H psi_r
+                ham_apply(sim, ham, P + 2 * sb, nullptr, L, 1, true);  // H^dagger phi_l
+                ham_apply(sim, ham, P, nullptr, L + 2 * sb, 1);        // H psi_r
                 hip_check(launch_scale(sim->dtype, L, L + sb, dim, 0.0, 1.0, s), "scale");
                 hip_check(launch_scale(sim->dtype, L + 2 * sb, L + 3 * sb, dim, 0.0, 1.0, s), "scale");
-                vdot(sim, L, P, f_h);  // <H phi_l|psi_r> = <phi_l|H|psi_r> (synchronises)
+                vdot(sim, L, P, f_h);  // <H^dagger phi_l|psi_r> = <phi_l|H|psi_r> (synchronises)
                 for (int i = 0; i < 2; i++) {  // the back-sweeps (they consume the forward pairs)
                     if (side[i].c->n_tap_ids == 0) {
                         taps_h[i].clear();

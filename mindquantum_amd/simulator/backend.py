@@ -109,7 +109,7 @@ class MQVectorAMD:
             raise TypeError(
                 f"Data type of {self.name} simulator is {mqdtype.precision_str(self.dtype)} ({self.dtype.__name__}), "
                 f"but {i}th hamiltonian is {mqdtype.precision_str(ham.dtype)} ({ham.dtype.__name__}).")
-        if ham.n_qubits > self.n_qubits:
+        if ham.n_qubits > self.n_qubits or (ham.how_to == 2 and ham.n_qubits != self.n_qubits):
             raise ValueError(f"Hamiltonian qubits is {ham.n_qubits}, not match with simulator qubits number "
                              f"{self.n_qubits}")
 

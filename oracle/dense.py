@@ -84,7 +84,11 @@ def state(circ, values, n, init=None):
 
 
 def hamiltonian_matrix(qubit_operator, n):
-    """Sum of coeff * (kron of Pauli matrices), qubit 0 as the least significant factor."""
+    """Sum of coeff * (kron of Pauli matrices), qubit 0 as the least significant factor; a matrix
+    (numpy or scipy sparse, 2^n x 2^n) is returned as a dense array."""
+    if not hasattr(qubit_operator, "terms"):
+        m = qubit_operator.toarray() if hasattr(qubit_operator, "toarray") else np.asarray(qubit_operator)
+        return np.asarray(m, dtype=np.complex128)
     dim = 1 << n
     h = np.zeros((dim, dim), dtype=np.complex128)
     for key, coeff in qubit_operator.terms.items():
