@@ -232,6 +232,13 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
  * measured qubits' bits from the indices.  out_index: [shots]. */
 int mqa_sim_sample(const mqa_sim* sim, const mqa_circuit* circ, const double* params, int32_t n_params,
                    const double* uniforms, uint64_t shots, uint64_t* out_index);
+/* Expectations of a batch of angle settings of one circuit (the evaluations of
+ * get_expectation_with_grad_parameter_shift_multi_multi, vector_state.tpp:1033-1300): sample b
+ * runs circ from the simulator's state with gate angle slot j set to angles[b * n_slots + j] (slots
+ * in the circuit's order of parameterised gates, the angle as the gate's matrix takes it), and
+ * f_out[b][h] = <psi_b|H_h|psi_b> (re, im).  Samples run in chunks that fit the HBM budget. */
+int mqa_sim_expectation_angles(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams,
+                               const mqa_circuit* circ, const double* angles, int64_t batch, double* f_out);
 /* complex64() / complex128() (vector_state.tpp:770-775 astype): a new simulator holding the
  * state cast to dtype. */
 int mqa_sim_astype(const mqa_sim* sim, int32_t dtype, mqa_sim** out);

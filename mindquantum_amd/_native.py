@@ -87,6 +87,7 @@ _SIGS = {
     "mqa_sim_grad_batch": ([_P, _P, _I32, _P, _P, _I64, _I32, _P, _I32, _P, _P], ctypes.c_int),
     "mqa_sim_grad_nonherm": ([_P, _P, _P, _I32, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P], ctypes.c_int),
     "mqa_sim_sample": ([_P, _P, _P, _I32, _P, _U64, _P], ctypes.c_int),
+    "mqa_sim_expectation_angles": ([_P, _P, _I32, _P, _P, _I64, _P], ctypes.c_int),
     "mqa_sim_astype": ([_P, _I32, _PP], ctypes.c_int),
     "mqa_grad_items": ([_I32, _P, _P, _P, _P, _P, _I32, _P, _P], ctypes.c_int),
     "mqa_inner_product": ([_P, _P, _P], ctypes.c_int),

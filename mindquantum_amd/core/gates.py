@@ -377,6 +377,19 @@ class GlobalPhase(ParameterGate):
 _PAULI_OF = {"X": _PX, "Y": _PY, "Z": _PZ, "I": np.eye(2, dtype=np.complex128)}
 
 
+def _tag(gates, k, factor):
+    """Record on lowered gates that their angle is ``factor`` times parameter expression k of the
+    gate they came from (nested lowerings multiply factors); untagged parameterised gates carry
+    constant angles.  Used to shift one expression of a composite gate (parameter-shift rule)."""
+    out = []
+    for g in gates:
+        if isinstance(g, ParameterGate) and not g.coeff.is_const():
+            inner = getattr(g, "_origin", (0, 1.0))
+            g._origin = (k, factor * inner[1])
+        out.append(g)
+    return out
+
+
 def _kron_string(paulis):
     """Matrix of a Pauli string, paulis[i] acting on matrix bit i (little-endian)."""
     m = np.ones((1, 1), dtype=np.complex128)
@@ -401,18 +414,18 @@ def _pauli_rotation(paulis, objs, ctrl, pr):
     rotation carries the controls: V^dag C(R) V is C(V^dag R V) for any V."""
     act = [(p, q) for p, q in zip(paulis, objs) if p != "I"]
     if not act:
-        return [GlobalPhase(pr * 0.5).on(objs[0], ctrl)]
+        return _tag([GlobalPhase(pr * 0.5).on(objs[0], ctrl)], 0, 0.5)
     if len(act) == 1:
         p, q = act[0]
-        return [{"X": RX, "Y": RY, "Z": RZ}[p](pr).on(q, ctrl)]
+        return _tag([{"X": RX, "Y": RY, "Z": RZ}[p](pr).on(q, ctrl)], 0, 1.0)
     pre = [g for g in (_basis_to_z(p, q, False) for p, q in act) if g is not None]
     post = [g for g in (_basis_to_z(p, q, True) for p, q in act) if g is not None]
     qs = [q for _, q in act]
     if len(qs) == 2:
-        core = [Rzz(pr).on(qs, ctrl)]
+        core = _tag([Rzz(pr).on(qs, ctrl)], 0, 1.0)
     else:
         ladder = [X.on(qs[i + 1], qs[i]) for i in range(len(qs) - 2)]
-        core = ladder + [Rzz(pr).on(qs[-2:], ctrl)] + ladder[::-1]
+        core = ladder + _tag([Rzz(pr).on(qs[-2:], ctrl)], 0, 1.0) + ladder[::-1]
     return pre + core + post
 
 
@@ -490,8 +503,8 @@ class Givens(ParameterGate):
 
     def lowered(self):
         a, b = self.obj_qubits
-        return Rxy(self.coeff).on([a, b], self.ctrl_qubits).lowered() + \
-            Rxy(-self.coeff).on([b, a], self.ctrl_qubits).lowered()
+        return _tag(Rxy(self.coeff).on([a, b], self.ctrl_qubits).lowered(), 0, 1.0) + \
+            _tag(Rxy(-self.coeff).on([b, a], self.ctrl_qubits).lowered(), 0, -1.0)
 
 
 class SWAPalpha(ParameterGate):
@@ -515,8 +528,8 @@ class SWAPalpha(ParameterGate):
     def lowered(self):
         half = self.coeff * (np.pi / 2)
         q, c = self.obj_qubits, self.ctrl_qubits
-        return [GlobalPhase(self.coeff * (-np.pi / 4)).on(q[0], c), Rxx(half).on(q, c), Ryy(half).on(q, c),
-                Rzz(half).on(q, c)]
+        return _tag([GlobalPhase(self.coeff * (-np.pi / 4)).on(q[0], c)], 0, -np.pi / 4) + \
+            _tag([Rxx(half).on(q, c), Ryy(half).on(q, c), Rzz(half).on(q, c)], 0, np.pi / 2)
 
 
 class MultiParamsGate(BasicGate):
@@ -618,7 +631,8 @@ class U3(MultiParamsGate):
 
     def lowered(self):
         q, c = self.obj_qubits, self.ctrl_qubits
-        return [PhaseShift(self.prs[2]).on(q, c), RY(self.prs[0]).on(q, c), PhaseShift(self.prs[1]).on(q, c)]
+        return _tag([PhaseShift(self.prs[2]).on(q, c)], 2, 1.0) + _tag([RY(self.prs[0]).on(q, c)], 0, 1.0) + \
+            _tag([PhaseShift(self.prs[1]).on(q, c)], 1, 1.0)
 
 
 class FSim(MultiParamsGate):
@@ -655,7 +669,8 @@ class FSim(MultiParamsGate):
 
     def lowered(self):
         q, c = self.obj_qubits, list(self.ctrl_qubits)
-        return [Rxx(self.prs[0]).on(q, c), Ryy(self.prs[0]).on(q, c), PhaseShift(-self.prs[1]).on(q[1], [q[0]] + c)]
+        return _tag([Rxx(self.prs[0]).on(q, c), Ryy(self.prs[0]).on(q, c)], 0, 1.0) + \
+            _tag([PhaseShift(-self.prs[1]).on(q[1], [q[0]] + c)], 1, -1.0)
 
 
 class Rn(MultiParamsGate):

@@ -1865,6 +1865,62 @@ int mqa_sim_sample(const mqa_sim* sim_c, const mqa_circuit* circ_c, const double
     });
 }
 
+int mqa_sim_expectation_angles(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t n_hams,
+                               const mqa_circuit* circ_c, const double* angles, int64_t batch, double* f_out) {
+    return guarded([&] {
+        require(sim_c != nullptr && circ_c != nullptr && f_out != nullptr, "NULL argument");
+        require(n_hams >= 1 && hams != nullptr, "need at least one hamiltonian");
+        require(batch >= 1, "batch must be positive");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        auto* circ = const_cast<mqa_circuit*>(circ_c);
+        require(circ->max_qubit < sim->n, "circuit is wider than the simulator");
+        require(circ->n_slots == 0 || angles != nullptr, "angles is NULL");
+        for (int h = 0; h < n_hams; h++) check_ham(sim, hams[h]);
+        DeviceGuard dg(sim->device);
+        hipStream_t s = sim->stream;
+        const size_t sb = sim->state_bytes();
+        DevPlan* fwd = circ->gates.empty() ? nullptr : circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
+        const size_t budget = hbm_budget();
+        int64_t chunk = int64_t(std::max<size_t>(budget / (2 * sb + 4096), 1));
+        chunk = std::min<int64_t>({chunk, batch, 65535});
+        sim->psi.ensure(sb * chunk);
+        sim->lam.ensure(sb * chunk);
+        std::vector<double> tab(size_t(chunk) * std::max(circ->n_slots, 1) * 2), f_host(size_t(chunk) * 2);
+        for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+            const int64_t nb = std::min<int64_t>(chunk, batch - b0);
+            // (cos, sin) of the half angle (rotations) or of the angle (PhaseShift, GlobalPhase)
+            for (int64_t b = 0; b < nb; b++)
+                for (int sl = 0; sl < circ->n_slots; sl++) {
+                    double th = angles[(b0 + b) * circ->n_slots + sl];
+                    if (circ->slot_half[size_t(sl)]) th *= 0.5;
+                    tab[size_t(b * circ->n_slots + sl) * 2] = std::cos(th);
+                    tab[size_t(b * circ->n_slots + sl) * 2 + 1] = std::sin(th);
+                }
+            if (circ->n_slots > 0) {
+                sim->angles.ensure(tab.size() * sizeof(double));
+                hip_check(hipMemcpyAsync(sim->angles.p, tab.data(), size_t(nb) * circ->n_slots * 2 * sizeof(double),
+                                         hipMemcpyHostToDevice, s),
+                          "angles");
+            }
+            hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff, nb, s),
+                      "fill");
+            if (fwd)
+                run_plan(sim, fwd, sim->psi.p, nullptr, nb, circ->n_slots ? sim->angles.as<double>() : nullptr,
+                         circ->n_slots, nullptr, 0);
+            for (int h = 0; h < n_hams; h++) {
+                ham_apply(sim, const_cast<mqa_ham*>(hams[h]), sim->psi.p, sim->psi.p, sim->lam.p, nb);
+                hip_check(hipMemcpyAsync(f_host.data(), sim->fsum.p, nb * 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+                          "d2h f");
+                sync(sim, "expectation batch");
+                for (int64_t b = 0; b < nb; b++) {
+                    f_out[((b0 + b) * n_hams + h) * 2] = f_host[2 * b];
+                    f_out[((b0 + b) * n_hams + h) * 2 + 1] = f_host[2 * b + 1];
+                }
+            }
+        }
+    });
+}
+
 int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy) {
     return guarded([&] {
         require(ms_per_copy != nullptr && reps >= 1 && bytes >= 16, "bad argument");

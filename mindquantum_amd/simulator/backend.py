@@ -274,8 +274,8 @@ class MQVectorAMD:
             simulator_left = self
         if circ_left is None:
             circ_left = circ_right
-        if pr_shift:
-            raise NotImplementedError("parameter-shift gradients are not on this path; the adjoint is used")
+        if pr_shift and non_hermitian:  # mqsim.py:372-388: the non-hermitian gradient takes precedence
+            pr_shift = False
         if circ_right.has_measure_gate:
             raise ValueError("circuit for variational algorithm cannot have measure gate")
         if parallel_worker is not None and not isinstance(parallel_worker, (int, np.integer)):
@@ -322,6 +322,14 @@ class MQVectorAMD:
                 ans = np.ascontiguousarray(inputs[0], dtype=np.float64)
             batch = enc.shape[0]
             _thread_balance(batch, len(hams), parallel_worker)  # validates like the reference
+            if pr_shift:
+                from .param_shift import gradient as shift_gradient
+
+                fc, gc = shift_gradient(self, hams, circ_right, encoder_params_name, ansatz_params_name, enc, ans)
+                fc, gc = fc.astype(self.dtype), gc.astype(self.dtype)
+                if version == "both":
+                    return fc, gc[:, :, :n_enc], gc[:, :, n_enc:]
+                return fc, gc
             f = np.zeros((batch, len(hams), 2), dtype=np.float64)
             if non_hermitian:
                 g = np.zeros((batch, len(hams), n_enc + n_ans, 2), dtype=np.float64)
