@@ -287,9 +287,13 @@ def copy_probe(gib=4, reps=10):
     from mindquantum_amd import _native
 
     ms = ctypes.c_double(0)
-    _native.check(_native.lib().mqa_copy_probe(gib << 30, reps, ctypes.byref(ms)))
+    _native.check(_native.lib().mqa_copy_probe(gib << 30, reps, 0, ctypes.byref(ms)))
     byts = 2 * (gib << 30)
     res = {"bytes_per_copy": byts, "avg_ms": round(ms.value, 4), "gbs": round(byts / (ms.value * 1e-3) / 1e9, 1)}
+    _native.check(_native.lib().mqa_copy_probe(gib << 30, reps, 1, ctypes.byref(ms)))
+    res["in_place_gbs"] = round(byts / (ms.value * 1e-3) / 1e9, 1)
+    _native.check(_native.lib().mqa_copy_probe(gib << 30, reps, 2, ctypes.byref(ms)))
+    res["in_place_nontemporal_gbs"] = round(byts / (ms.value * 1e-3) / 1e9, 1)
     log(f"copy probe: {res}")
     return res
 

@@ -116,10 +116,12 @@ int mqa_get_option(const char* key, int64_t* value);
 int mqa_profile_get(const char* kernel, int64_t* launches, double* total_ms, double* bytes);
 int mqa_profile_reset(void);
 /* Streaming-copy probe: times `reps` launches of a plain element-wise copy kernel (one read and
- * one write of `bytes` each) on two fresh device buffers and returns the average launch time.
- * The achievable HBM rate of this GPU, and the kernel of known traffic that calibrates the HBM
- * byte counters of a profiler.  No counterpart in the reference. */
-int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy);
+ * one write of `bytes` each) on fresh device buffers and returns the average launch time;
+ * in_place = 1 writes every element back where it was read (the access pattern of a gate pass), 2
+ * does so with non-temporal accesses (as the tile passes stream the state).  The achievable HBM
+ * rate of this GPU, and the kernel of known traffic that calibrates the HBM byte counters of a
+ * profiler.  No counterpart in the reference. */
+int mqa_copy_probe(uint64_t bytes, int32_t reps, int32_t in_place, double* ms_per_copy);
 
 /* ---- circuit (reference: mindquantum/core/circuit/circuit.py:1108-1130 get_cpp_obj) ----
  * Host-side object: validated gate list plus the cached tile-pass schedules.

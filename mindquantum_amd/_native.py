@@ -93,7 +93,7 @@ _SIGS = {
     "mqa_inner_product": ([_P, _P, _P], ctypes.c_int),
     "mqa_profile_get": ([ctypes.c_char_p, ctypes.POINTER(_I64), _DP, _DP], ctypes.c_int),
     "mqa_profile_reset": ([], ctypes.c_int),
-    "mqa_copy_probe": ([_U64, _I32, _DP], ctypes.c_int),
+    "mqa_copy_probe": ([_U64, _I32, _I32, _DP], ctypes.c_int),
 }
 
 

@@ -6,6 +6,8 @@
 
 namespace mqa {
 
+
+
 // dst[j] = (re + i im) * src[j] for j < n_amps (dst may equal src).  dtype: 0 = complex64,
 // 1 = complex128.
 hipError_t launch_scale(int dtype, const void* src, void* dst, uint64_t n_amps, double re, double im, hipStream_t s);

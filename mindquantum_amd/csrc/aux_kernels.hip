@@ -1,5 +1,6 @@
-// aux_kernels.hip - small kernels for gfx950: complex scaling of a state, CSR matrix times states,
-// probability block sums and the per-block search of measurement sampling.
+// aux_kernels.hip - kernels for gfx950 outside the tile engine: complex scaling of a state (also the
+// copy probe), CSR matrix times states, probability block sums and the per-block search of
+// measurement sampling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,16 +21,28 @@ struct Cplx<double> {
     using V = double2;
 };
 
-// Grid-stride over amplitudes; one 8- or 16-byte load and store per amplitude.
-template <typename T>
+// Grid-stride over amplitudes; one 8- or 16-byte load and store per amplitude.  NT: non-temporal
+// accesses (streamed data that is not read again).
+template <typename T, bool NT = false>
 __global__ void __launch_bounds__(256) k_scale(const typename Cplx<T>::V* src, typename Cplx<T>::V* dst, uint64_t n,
                                                T re, T im) {
     for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += uint64_t(gridDim.x) * blockDim.x) {
-        const typename Cplx<T>::V v = src[j];
+        typename Cplx<T>::V v;
+        if constexpr (NT) {
+            v.x = __builtin_nontemporal_load(&src[j].x);
+            v.y = __builtin_nontemporal_load(&src[j].y);
+        } else {
+            v = src[j];
+        }
         typename Cplx<T>::V o;
         o.x = re * v.x - im * v.y;
         o.y = re * v.y + im * v.x;
-        dst[j] = o;
+        if constexpr (NT) {
+            __builtin_nontemporal_store(o.x, &dst[j].x);
+            __builtin_nontemporal_store(o.y, &dst[j].y);
+        } else {
+            dst[j] = o;
+        }
     }
 }
 
@@ -180,6 +193,11 @@ hipError_t launch_sample_blocks(int dtype, const void* psi, uint32_t block, cons
 hipError_t launch_scale(int dtype, const void* src, void* dst, uint64_t n_amps, double re, double im, hipStream_t s) {
     if (n_amps == 0) return hipSuccess;
     const uint64_t blocks = std::min<uint64_t>((n_amps + 255) / 256, uint64_t(1) << 20);
+    if (dtype == 2) {  // complex128 with non-temporal accesses (probe)
+        hipLaunchKernelGGL((k_scale<double, true>), dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const double2*>(src), static_cast<double2*>(dst), n_amps, re, im);
+        return hipGetLastError();
+    }
     if (dtype == 0) {
         hipLaunchKernelGGL(k_scale<float>, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<const float2*>(src),
                            static_cast<float2*>(dst), n_amps, float(re), float(im));

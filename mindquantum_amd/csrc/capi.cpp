@@ -1921,7 +1921,7 @@ int mqa_sim_expectation_angles(const mqa_sim* sim_c, const mqa_ham* const* hams,
     });
 }
 
-int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy) {
+int mqa_copy_probe(uint64_t bytes, int32_t reps, int32_t in_place, double* ms_per_copy) {
     return guarded([&] {
         require(ms_per_copy != nullptr && reps >= 1 && bytes >= 16, "bad argument");
         if (mqa_device_count() <= 0) throw Error(MQA_ERR_HIP, "no HIP device is visible");
@@ -1936,7 +1936,9 @@ int mqa_copy_probe(uint64_t bytes, int32_t reps, double* ms_per_copy) {
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
         hip_check(hipEventRecord(e0, so.s), "event");
-        for (int r = 0; r < reps; r++) hip_check(launch_scale(1, a.p, b.p, n, 1.0, 0.0, so.s), "copy");
+        void* dst = in_place ? a.p : b.p;  // in place: every element read and written back (a gate pass's pattern)
+        const int kind = in_place == 2 ? 2 : 1;  // 2: in place with non-temporal accesses
+        for (int r = 0; r < reps; r++) hip_check(launch_scale(kind, a.p, dst, n, 1.0, 0.0, so.s), "copy");
         hip_check(hipEventRecord(e1, so.s), "event");
         hip_check(hipEventSynchronize(e1), "sync");
         float ms = 0;
