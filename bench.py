@@ -82,7 +82,9 @@ def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile, tune=Fals
         c64 = dtype == mq.complex64
         times = autotune.tune_grad(grad_ops, enc, ans, fwd_candidates=(12, 13) if c64 else (11, 12),
                                    adj_candidates=(11, 12) if c64 else (10, 11))
-        tiles = {"forward_back_sweep": list(min(times, key=times.get)),
+        code = mq.dtype.native_code(dtype)
+        nc = grad_ops.native_circuits[0]
+        tiles = {"forward_back_sweep": [nc.tile_qubits(code, False), nc.tile_qubits(code, True)],
                  "seconds": {f"{f}/{a}": round(t, 4) for (f, a), t in times.items()}}
         log(f"autotune ({np.dtype(dtype).name}): {tiles}")
     ops = MQOps(grad_ops)
@@ -126,7 +128,7 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     tuned = None
     if tune:  # untimed: the tile width of the passes, chosen on this circuit
         times = autotune.tune_apply(circ, n, dtype)
-        tuned = {"tile_qubits": min(times, key=times.get), "seconds": {k: round(v, 4) for k, v in times.items()}}
+        tuned = {"tile_qubits": circ._mqa_tiles[(mq.dtype.native_code(dtype), False)], "seconds": {k: round(v, 4) for k, v in times.items()}}
         log(f"autotune apply {n}q {np.dtype(dtype).name}: {tuned}")
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     code = 0 if dtype == mq.complex64 else 1
