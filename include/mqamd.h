@@ -102,8 +102,10 @@ int mqa_device_count(void);
  * "max_ops_per_pass" (1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
  * MQAMD_HBM_BUDGET_BYTES; 0 = 85% of free memory), "profile", "ham_tile_qubits_c64",
  * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
- * the first launch), "jit_min_amps", "checkpoints".  mqa_get_option also reads the counters
- * "jit_launches", "jit_compile_ms", "grad_recomputes" and "checkpoints_used".  No counterpart
+ * the first launch), "jit_min_amps", "checkpoints", "phase_free" (1, the default: mqa_sim_grad_batch
+ * runs plans in which uncontrolled RX / RY / RZ / GP drop their global phase, see there; 0: exact
+ * plans).  mqa_get_option also reads the counters
+ * "jit_launches", "jit_compile_ms", "grad_recomputes", "checkpoints_used" and "phase_free_used".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
 int mqa_get_option(const char* key, int64_t* value);
@@ -146,8 +148,9 @@ int mqa_circuit_get_tile_qubits(const mqa_circuit* circ, int32_t dtype, int32_t 
 int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint, int64_t* out,
                          int64_t cap, int64_t* len);
 /* Generate and compile (hipRTC, host only, no GPU needed) the per-circuit kernels of the
- * forward (adjoint == 0) or back-sweep schedule, as the library does in the background for
- * large launches (option "jit").  *seconds gets the compile wall time.  No counterpart in the
+ * forward (adjoint == 0) or back-sweep (1) schedule, or of their phase-free forms (2, 3: those
+ * the hermitian gradient runs), as the library does in the background for large launches
+ * (option "jit").  *seconds gets the compile wall time.  No counterpart in the
  * reference: a build check of the code generator for tests and for callers that want to pay
  * the compile cost up front. */
 int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint,
@@ -202,7 +205,14 @@ int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ha
  * also stores psi at up to c back-sweep pass boundaries (as many as the budget holds for one
  * sample, one state each; read back with the counter "checkpoints_used") and the back-sweep
  * continues from each stored state instead of the uncomputed one, bounding the rounding drift of
- * the uncompute; the forward sweep is then recomputed per hamiltonian. */
+ * the uncompute; the forward sweep is then recomputed per hamiltonian.  With the option
+ * "phase_free" (default 1) the forward sweep, the back-sweep and the checkpoints drop the global
+ * phases of uncontrolled RX / RY / RZ / GP gates alike (neither <psi|H|psi> nor the taps
+ * <lambda|dG|psi> see a phase common to psi and lambda): RZ becomes a phase on one amplitude,
+ * RX / RY rotate as three shears, and the RZ and GP taps use the fact that <lambda|psi> =
+ * <psi|H|psi> is real, so they run only when every observable is hermitian (Pauli terms always
+ * are; a CSR matrix must equal its conjugate transpose); the counter "phase_free_used" tells
+ * whether the last call ran them. */
 int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ,
                        const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
                        double* g_out);

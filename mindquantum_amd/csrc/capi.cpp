@@ -75,6 +75,8 @@ struct Options {
     std::atomic<int64_t> jit{1}, jit_min_amps{int64_t(1) << 24};
     // forward-state checkpoints of the batched gradient (0: none; the back-sweep uncomputes psi)
     std::atomic<int64_t> checkpoints{0};
+    // 1: hermitian gradients run phase-free plans (uncontrolled rotations drop global phases)
+    std::atomic<int64_t> phase_free{1};
 };
 Options g_opt;
 
@@ -91,7 +93,8 @@ std::vector<OptEntry> opt_table() {
             {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0},
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
-            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0}};
+            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
+            {"phase_free", &g_opt.phase_free, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -228,24 +231,27 @@ struct mqa_circuit {
     int n_tap_ids = 0;
 
     std::mutex mu;
-    std::map<std::tuple<int, int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, std::unique_ptr<mqa::DevPlan>> plans;
     std::map<int, std::unique_ptr<mqa::DevBuf[]>> dev_slots;  // per device: 6 buffers
 
     // per-circuit tile widths (autotuner), [dtype][adjoint]; < 0: the global option
     std::atomic<int> tile_override[2][2] = {{-1, -1}, {-1, -1}};
 
-    mqa::SchedOptions options(int dtype, bool adjoint) const {
+    // phase_free: the plan may drop global phases of uncontrolled gates (OPK_PF_* kinds); only for
+    // states whose results do not see them (the hermitian gradient's forward and back-sweep)
+    mqa::SchedOptions options(int dtype, bool adjoint, bool phase_free = false) const {
         auto o = mqa::sched_options(dtype, adjoint);
         const int k = tile_override[dtype == MQA_C64 ? 0 : 1][adjoint ? 1 : 0].load();
         if (k > 0) o.tile_qubits = k;
+        o.phase_free = phase_free;
         return o;
     }
 
-    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev) {
-        auto o = options(dtype, adjoint);
+    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev, bool phase_free = false) {
+        auto o = options(dtype, adjoint, phase_free);
         int dev = upload_dev ? mqa::current_device() : -1;
         auto key = std::make_tuple(n_eff, dtype, int(adjoint), o.tile_qubits, o.low_run, o.group_width,
-                                   o.max_ops_per_pass, o.taps_per_group, dev);
+                                   o.max_ops_per_pass, o.taps_per_group, int(o.phase_free), dev);
         std::lock_guard<std::mutex> lk(mu);
         auto it = plans.find(key);
         if (it != plans.end()) return it->second.get();
@@ -290,8 +296,10 @@ struct mqa_circuit {
     };
     std::map<std::tuple<const mqa::DevPlan*, int, int, int, int, int>, std::unique_ptr<Checkpointed>> ckpt_plans;
 
+    // The segments drop the same phases as the back-sweep plan adj (a phase-free back-sweep
+    // continues from phase-free checkpoints, so psi and lambda keep one common phase).
     Checkpointed* checkpointed(int n_eff, int dtype, int n_ck, const mqa::DevPlan* adj, hipStream_t s) {
-        auto o = options(dtype, false);
+        auto o = options(dtype, false, adj->plan.opts.phase_free);
         std::lock_guard<std::mutex> lk(mu);
         auto key = std::make_tuple(adj, n_ck, o.tile_qubits, o.low_run, o.group_width, o.max_ops_per_pass);
         auto it = ckpt_plans.find(key);
@@ -382,6 +390,7 @@ struct mqa_ham {
 
     // sparse form (a CSR matrix given by the caller): [0] the operator, [1] its conjugate transpose
     bool csr = false;
+    bool hermitian = true;  // Pauli terms have real coefficients; a CSR matrix is compared with H^dagger
     uint64_t csr_dim = 0;
     std::vector<int64_t> csr_ptr[2], csr_idx[2];
     std::vector<double> csr_val[2];  // interleaved complex
@@ -480,6 +489,7 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 std::atomic<int64_t> g_jit_launches{0};
 std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
 std::atomic<int64_t> g_ckpt_used{0};    // checkpoints the last batched gradient call held
+std::atomic<int64_t> g_pf_used{0};      // 1: the last batched gradient call ran phase-free plans
 void count_jit_launch() { g_jit_launches.fetch_add(1); }
 
 void sync(mqa_sim* sim, const char* what) {
@@ -764,6 +774,10 @@ int mqa_get_option(const char* key, int64_t* value) {
             *value = g_ckpt_used.load();
             return;
         }
+        if (std::strcmp(key, "phase_free_used") == 0) {  // read-only: see mqa_sim_grad_batch
+            *value = g_pf_used.load();
+            return;
+        }
         if (std::strcmp(key, "jit_compile_ms") == 0) {  // read-only: time spent compiling generated passes
             *value = int64_t(jit_compile_seconds_total() * 1e3);
             return;
@@ -930,7 +944,9 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adjoint != 0, circ->options(dtype, adjoint != 0));
+        require(adjoint >= 0 && adjoint <= 3, "adjoint must be 0..3");
+        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0;
+        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf));
         JitPlan jp(pl, dtype);
         jp.wait();
         if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());
@@ -1016,6 +1032,28 @@ int mqa_ham_create_csr(uint64_t dim, uint64_t nnz, const int64_t* indptr, const 
                 tv[2 * size_t(at)] = data[2 * e];
                 tv[2 * size_t(at) + 1] = -data[2 * e + 1];
             }
+        // hermitian iff every row equals the same row of the conjugate transpose (entries sorted by
+        // column, duplicates summed, zeros dropped); exact comparison
+        auto row_of = [](const int64_t* ptr, const int64_t* idx, const double* val, uint64_t r) {
+            std::vector<std::tuple<int64_t, double, double>> e;
+            for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) e.emplace_back(idx[k], val[2 * k], val[2 * k + 1]);
+            std::sort(e.begin(), e.end(), [](auto& a, auto& b) { return std::get<0>(a) < std::get<0>(b); });
+            std::vector<std::tuple<int64_t, double, double>> m;
+            for (auto& [c, re, im] : e) {
+                if (!m.empty() && std::get<0>(m.back()) == c) {
+                    std::get<1>(m.back()) += re;
+                    std::get<2>(m.back()) += im;
+                } else {
+                    m.emplace_back(c, re, im);
+                }
+            }
+            m.erase(std::remove_if(m.begin(), m.end(),
+                                   [](auto& x) { return std::get<1>(x) == 0.0 && std::get<2>(x) == 0.0; }),
+                    m.end());
+            return m;
+        };
+        for (uint64_t r = 0; r < dim && h->hermitian; r++)
+            h->hermitian = row_of(indptr, indices, data, r) == row_of(tp.data(), ti.data(), tv.data(), r);
         DevHamGroup g{};  // the Pauli tables stay empty
         h->groups.push_back(g);
         *out = h.release();
@@ -1215,8 +1253,14 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         DeviceGuard dg(sim->device);
         hipStream_t s = sim->stream;
 
-        DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
-        DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true);
+        // <psi|H|psi> and the taps <lambda|dG|psi> (lambda = H psi) do not see a global phase
+        // common to psi and lambda: phase-free plans; their RZ / GP taps also need <lambda|psi>
+        // real, i.e. every observable hermitian
+        bool pf = g_opt.phase_free.load() != 0;
+        for (int h = 0; h < n_hams; h++) pf = pf && hams[h]->hermitian;
+        g_pf_used.store(pf ? 1 : 0);
+        DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true, pf);
+        DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true, pf);
         DevBuf* sl = circ->slots_on_device(s);
         const int n_tap_ids = circ->n_tap_ids;
         const uint64_t n_tiles = uint64_t(1) << (adj->plan.n_eff - adj->plan.K);

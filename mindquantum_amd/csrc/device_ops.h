@@ -430,37 +430,49 @@ __device__ __forceinline__ void apply_body(Regs<T>& a, const GroupCtx& g, uint32
 
 // One op of a group.  ADJ: Re(tap) of the derivative, then G^dagger on psi and lambda.  Lean
 // passes (FULL = false) never carry local control predicates: the host picks FULL when they do.
+// The phase-free kinds run as the op they equal up to that phase: PF_RX / PF_RY as RX / RY with
+// the (cos, sin) pair negated when cos < 0, PF_RZ as PS of the full angle, PF_GP as nothing.
 template <typename T, bool ADJ, bool FULL>
 __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g, uint64_t base,
                                        const T* __restrict__ pool, const double* __restrict__ ang_row,
                                        TapAcc<T>& acc) {
-    if ((base & op.ctrl_global) != op.ctrl_global) return;  // a non-local control is off: no-op, zero tap
-    const uint32_t cl = op.ctrl_local;
-    const int kind = op.kind;
-    T c = 0, s = 0;
-    if (kind >= OPK_RX) {  // (cos, sin) of the half angle, or of the angle for PS / GP
-        c = T(ang_row[2 * op.slot]);
-        s = T(ang_row[2 * op.slot + 1]);
-        if (ADJ) s = -s;  // G^dagger = G(-angle)
-    }
-    const T* m = (op.mat >= 0) ? pool + op.mat : pool;
-    T tr = 0;  // Re(tap)
-    dispatch_op<FULL>(kind, op.q0, op.q1, [&]<int KD, int Q0, int Q1>() {
-        auto body = [&]<bool CT>() {
-            if constexpr (ADJ && KD >= OPK_RX) {
-                if constexpr (kTwoTarget<KD>) tr = two_target_tap<T, KD, Q0, Q1, CT>(p, l, g, cl);
-                else tr = one_target_tap<T, KD, Q0, CT>(p, l, g, cl);
-            }
-            apply_body<T, KD, Q0, Q1, CT>(p, g, cl, c, s, m);
-            if constexpr (ADJ) apply_body<T, KD, Q0, Q1, CT>(l, g, cl, c, s, m);
-        };
-        if constexpr (FULL) {
-            if (cl) body.template operator()<true>();
-            else body.template operator()<false>();
-        } else {
-            body.template operator()<false>();
+    T tr = 0;  // Re(tap); zero when a non-local control is off (the op is then a no-op)
+    int kind = op.kind;
+    if ((base & op.ctrl_global) == op.ctrl_global && kind != OPK_PF_GP) {
+        const uint32_t cl = op.ctrl_local;
+        T c = 0, s = 0;
+        if (kind >= OPK_RX) {  // (cos, sin) of the half angle, or of the angle for PS / GP
+            c = T(ang_row[2 * op.slot]);
+            s = T(ang_row[2 * op.slot + 1]);
+            if (ADJ) s = -s;  // G^dagger = G(-angle)
         }
-    });
+        if (kind == OPK_PF_RZ) {
+            const T c2 = c * c - s * s;
+            s = T(2) * c * s;
+            c = c2;
+            kind = OPK_PS;
+        } else if (kind == OPK_PF_RX || kind == OPK_PF_RY) {
+            if (c < T(0)) c = -c, s = -s;
+            kind = kind == OPK_PF_RX ? OPK_RX : OPK_RY;
+        }
+        const T* m = (op.mat >= 0) ? pool + op.mat : pool;
+        dispatch_op<FULL>(kind, op.q0, op.q1, [&]<int KD, int Q0, int Q1>() {
+            auto body = [&]<bool CT>() {
+                if constexpr (ADJ && KD >= OPK_RX) {
+                    if constexpr (kTwoTarget<KD>) tr = two_target_tap<T, KD, Q0, Q1, CT>(p, l, g, cl);
+                    else tr = one_target_tap<T, KD, Q0, CT>(p, l, g, cl);
+                }
+                apply_body<T, KD, Q0, Q1, CT>(p, g, cl, c, s, m);
+                if constexpr (ADJ) apply_body<T, KD, Q0, Q1, CT>(l, g, cl, c, s, m);
+            };
+            if constexpr (FULL) {
+                if (cl) body.template operator()<true>();
+                else body.template operator()<false>();
+            } else {
+                body.template operator()<false>();
+            }
+        });
+    }
     if constexpr (ADJ) {
         if (op.tap >= 0) {
             // the group's taps arrive in order: shift them through the register file so that
@@ -470,6 +482,66 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
             acc.r[0] = tr;
         }
     }
+}
+
+// X on register position Q when on (a tile-uniform predicate: a non-local control): selects, no
+// branch, so the register state has no control-flow merge (a merge in the middle of a group can
+// make the compiler keep both register sets alive).
+template <typename T, int Q>
+__device__ __forceinline__ void x_if(Regs<T>& a, bool on) {
+    for_pairs<Q>([&]<int V, int W>() {
+        const T ar = a.r[V], ai = a.i[V], br = a.r[W], bi = a.i[W];
+        a.r[V] = on ? br : ar, a.i[V] = on ? bi : ai;
+        a.r[W] = on ? ar : br, a.i[W] = on ? ai : bi;
+    });
+}
+
+// ---------------------------------------------------------------- phase-free rotations
+// A plane rotation [[c, -s], [s, c]] as three shears (x -= t y; y += s x; x -= t y) with
+// t = tan(angle / 2) = s / (1 + c): three FMAs per real pair instead of four products.  The
+// coefficients come from the (cos, sin) pair with its sign normalised to cos >= 0 (the OPK_PF_*
+// convention), so |t| <= 1.
+template <typename T>
+struct Shear {
+    T t, s;
+};
+template <typename T>
+__device__ __forceinline__ Shear<T> shear_of(T c, T s) {
+    if (c < T(0)) c = -c, s = -s;
+    Shear<T> h;
+    if constexpr (sizeof(T) == 4) h.t = s * __builtin_amdgcn_rcpf(T(1) + c);
+    else h.t = s / (T(1) + c);
+    h.s = s;
+    return h;
+}
+template <typename T>
+__device__ __forceinline__ void shear_rotate(T& x, T& y, T t, T s) {
+    x = fma(-t, y, x);
+    y = fma(s, x, y);
+    x = fma(-t, y, x);
+}
+
+// PF_RY: [[c, -s], [s, c]] on the real and on the imaginary parts of each pair.
+// PF_RX: [[c, -is], [-is, c]] rotates (re a, im b) by -angle and (im a, re b) by +angle.
+template <typename T, int KIND, int Q>
+__device__ __forceinline__ void pf_rotation(Regs<T>& a, Shear<T> h) {
+    for_pairs<Q>([&]<int V, int W>() {
+        if constexpr (KIND == OPK_PF_RY) {
+            shear_rotate(a.r[V], a.r[W], h.t, h.s);
+            shear_rotate(a.i[V], a.i[W], h.t, h.s);
+        } else {
+            shear_rotate(a.r[V], a.i[W], -h.t, -h.s);
+            shear_rotate(a.i[V], a.r[W], h.t, h.s);
+        }
+    });
+}
+
+// PF_RZ: cos / sin of the full angle from those of the half angle.
+template <typename T>
+__device__ __forceinline__ void double_angle(T& c, T& s) {
+    const T c2 = c * c - s * s;
+    s = T(2) * c * s;
+    c = c2;
 }
 
 // ---------------------------------------------------------------- streaming accesses
@@ -508,13 +580,11 @@ __device__ __forceinline__ void st_stream(double2 v, double2* p) {
 // ---------------------------------------------------------------- tile I/O
 // HBM <-> LDS transfer of one tile.  Every access of a thread is issued before the first is
 // waited on (full unroll) so that enough bytes are in flight per CU to stream at HBM rate.
-// complex64 moves two adjacent amplitudes (16 B) per access: for u even, u and u + 1 are
-// adjacent in HBM (the low run C >= 1 is contiguous) and share the 16-byte LDS slot at
-// swz(u) & ~1, since swz(u ^ 1) == swz(u) ^ 1; when the swizzle sets bit 0 of swz(u) the
-// two amplitudes sit in that slot in swapped order.
-__device__ __forceinline__ float4 swap_halves_if(float4 v, bool odd) {
-    return odd ? make_float4(v.z, v.w, v.x, v.y) : v;
-}
+// complex64 moves two adjacent amplitudes (16 B) per HBM access: for u even, u and u + 1 are
+// adjacent in HBM (the low run C >= 1 is contiguous); in LDS they sit at swz(u) and
+// swz(u ^ 1) == swz(u) ^ 1, written and read as two 8-byte accesses (the swizzle may swap the
+// pair inside its 16-byte slot, and a per-lane swap in registers costs more than the second
+// LDS access).
 
 template <typename T>
 __device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* __restrict__ g,
@@ -530,7 +600,8 @@ __device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* 
 #pragma unroll
         for (int j = 0; j < kAmpsPerThread / 2; j++) {
             const uint32_t a = swz<T>(2u * (tid + NT * j));
-            *reinterpret_cast<float4*>(lds + (a & ~1u)) = swap_halves_if(buf[j], a & 1u);
+            lds[a] = make_float2(buf[j].x, buf[j].y);
+            lds[a ^ 1u] = make_float2(buf[j].z, buf[j].w);
         }
     } else {
         // complex128: two batches of 8 x 16 B, the same bytes in flight as complex64
@@ -558,8 +629,8 @@ __device__ __forceinline__ void tile_store(const V2<T>* __restrict__ lds, V2<T>*
         for (int j = 0; j < kAmpsPerThread / 2; j++) {
             const uint32_t u = 2u * (tid + NT * j);
             const uint32_t a = swz<T>(u);
-            st_stream(swap_halves_if(*reinterpret_cast<const float4*>(lds + (a & ~1u)), a & 1u),
-                      reinterpret_cast<float4*>(g + ((u & cmask) | offtab[u >> C])));
+            const V2<T> x = lds[a], y = lds[a ^ 1u];
+            st_stream(make_float4(x.x, x.y, y.x, y.y), reinterpret_cast<float4*>(g + ((u & cmask) | offtab[u >> C])));
         }
     } else {
 #pragma unroll
@@ -589,13 +660,15 @@ struct TileFrame {
 };
 
 // Build the offset table and stage the tile (psi, and lambda for the back-sweep) in LDS.
-template <typename T, bool ADJ>
+// KC > 0: the tile width is the compile-time constant KC (generated passes), so the LDS layout
+// and the thread count (2^(KC - 4), what every launch uses) fold into immediates.
+template <typename T, bool ADJ, int KC = 0>
 __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass, unsigned char* smem,
                                                    uint64_t tile, uint64_t sample) {
     using V = V2<T>;
     TileFrame<T> f;
-    const int K = args.K;
-    f.NT = blockDim.x;
+    const int K = KC > 0 ? KC : args.K;
+    f.NT = KC > 0 ? (1u << (KC - kGroupWidth)) : blockDim.x;
     f.tid = threadIdx.x;
     f.C = pass.low_run;
     const int H = K - f.C;
@@ -621,10 +694,10 @@ __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, con
 }
 
 // The usual mapping: blockIdx.x = tile, blockIdx.y = sample.
-template <typename T, bool ADJ>
+template <typename T, bool ADJ, int KC = 0>
 __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, const DevPass& pass,
                                                    unsigned char* smem) {
-    return tile_begin<T, ADJ>(args, pass, smem, blockIdx.x, blockIdx.y);
+    return tile_begin<T, ADJ, KC>(args, pass, smem, blockIdx.x, blockIdx.y);
 }
 
 // Launches over the tiles of several independent states (items of different sizes and

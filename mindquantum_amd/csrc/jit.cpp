@@ -71,6 +71,39 @@ bool read_file(const std::string& path, std::string& out) {
     return true;
 }
 
+// Body of a phase-free op (OPK_PF_*; see program.h), closing the op's brace.  Taps are taken
+// before the op, as for every kind: PF_RX / PF_RY with the RX / RY tap, PF_RZ with the PS tap of
+// the full angle, PF_GP with a zero tap (its variable keeps its initial zero).
+void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, std::vector<std::pair<int, int>>& taps) {
+    const int q = int(op.q0);
+    int var = -1;
+    if (adj && op.tap >= 0) {
+        var = int(taps.size());
+        taps.push_back({var, op.tap});
+    }
+    if (op.kind == OPK_PF_GP) {
+        b << "    }\n";
+        return;
+    }
+    b << "      T c = T(ang_row[" << 2 * op.slot << "]);\n"
+      << "      T s = " << (adj ? "-" : "") << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
+    if (op.kind == OPK_PF_RZ) {
+        b << "      double_angle(c, s);\n";
+        if (var >= 0) b << "      tap" << var << " = one_target_tap<T, OPK_PS, " << q << ", false>(p, l, g, 0u);\n";
+        b << "      apply_body<T, OPK_PS, " << q << ", 0, false>(p, g, 0u, c, s, pool);\n";
+        if (adj) b << "      apply_body<T, OPK_PS, " << q << ", 0, false>(l, g, 0u, c, s, pool);\n";
+    } else {
+        const char* base = op.kind == OPK_PF_RX ? "OPK_RX" : "OPK_RY";
+        const char* pf = op.kind == OPK_PF_RX ? "OPK_PF_RX" : "OPK_PF_RY";
+        if (var >= 0)
+            b << "      tap" << var << " = one_target_tap<T, " << base << ", " << q << ", false>(p, l, g, 0u);\n";
+        b << "      const Shear<T> h = shear_of(c, s);\n"
+          << "      pf_rotation<T, " << pf << ", " << q << ">(p, h);\n";
+        if (adj) b << "      pf_rotation<T, " << pf << ", " << q << ">(l, h);\n";
+    }
+    b << "    }\n";
+}
+
 }  // namespace
 
 const char* jit_kernel_name(bool adjoint) { return adjoint ? "mqa_jit_tile_pass_adj" : "mqa_jit_tile_pass_fwd"; }
@@ -100,7 +133,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
       << "  pass.tap_begin = " << ps.tap_begin << ";\n"
       << "  pass.n_taps = " << ps.n_taps << ";\n"
       << "  pass.low_run = " << ps.low_run << ";\n"
-      << "  TileFrame<T> f = tile_begin<T, " << A << ">(args, pass, smem);\n"
+      << "  TileFrame<T> f = tile_begin<T, " << A << ", " << pl.K << ">(args, pass, smem);\n"
       << "  const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;\n"
       << "  const T* pool = static_cast<const T*>(args.pool);\n"
       << "  (void)ang_row; (void)pool;\n";
@@ -124,6 +157,15 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
             const DevOp& op = pl.dev_ops.at(size_t(g.op_begin + oi));
             const bool gated = op.ctrl_global != 0;
             const char* ctrl = op.ctrl_local ? "true" : "false";
+            // CNOT across the tile in a back-sweep pass: selects (a branch there can make the
+            // compiler keep both register sets of psi and lambda alive); forward passes branch,
+            // which costs register moves only on tiles where the control is on
+            if (adj && gated && op.kind == OPK_X && op.ctrl_local == 0) {
+                b << "    {\n      const bool on = (f.base & 0x" << std::hex << op.ctrl_global << "ull) == 0x"
+                  << op.ctrl_global << "ull;\n" << std::dec << "      x_if<T, " << int(op.q0) << ">(p, on);\n"
+                  << "      x_if<T, " << int(op.q0) << ">(l, on);\n    }\n";
+                continue;
+            }
             b << "    ";
             if (gated) b << "if ((f.base & 0x" << std::hex << op.ctrl_global << "ull) == 0x" << op.ctrl_global << "ull) " << std::dec;
             b << "{\n";
@@ -135,6 +177,10 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
                     b << "      dense_from_lds<T, " << m << ">(l, g, " << op.ctrl_local << "u, pool + " << op.mat
                       << ", f.t_lam);\n";
                 b << "    }\n";
+                continue;
+            }
+            if (op.kind >= OPK_PF_RX) {  // phase-free forms: never controlled
+                emit_phase_free(b, op, adj, taps);
                 continue;
             }
             const char* kn = kind_name(op.kind);

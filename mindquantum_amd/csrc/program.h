@@ -42,6 +42,21 @@ enum OpKind : int32_t {
     OPK_RXX = 21,
     OPK_RYY = 22,
     OPK_RZZ = 23,
+    // Phase-free forms of uncontrolled rotations, for plans whose states only enter quantities
+    // that do not see a global phase (the hermitian gradient: <psi|H|psi>, and taps of psi and
+    // lambda, which carry the same phase).  Each drops a global phase that the forward op and
+    // its back-sweep inverse drop alike:
+    //   PF_RX, PF_RY: RX / RY with (cos, sin) of the half angle negated when cos < 0 (a factor
+    //                 -1), so the rotation runs as three shears with |tan| <= 1;
+    //   PF_RZ:        diag(1, e^{i theta}) = e^{i theta/2} RZ(theta), a phase on one amplitude;
+    //   PF_GP:        the identity.
+    // Back-sweep taps: PF_RX / PF_RY as RX / RY; PF_RZ and PF_GP rely on <lambda|psi> being
+    // real (lambda = H psi with H hermitian), so Re(tap) of RZ is -Im(sum over |1> of
+    // conj(lambda) psi), the PS tap, and that of GP is zero.
+    OPK_PF_RX = 24,
+    OPK_PF_RY = 25,
+    OPK_PF_RZ = 26,
+    OPK_PF_GP = 27,
 };
 
 inline bool opk_is_param(int k) { return k >= OPK_RX; }

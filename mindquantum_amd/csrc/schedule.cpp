@@ -102,6 +102,7 @@ struct PassBuilder {
     bool adjoint;
     int W;
     int taps_per_group;
+    bool phase_free;
     std::vector<int> tap_id_of_slot;  // angle slot -> global tap id (-1: no tap)
 
     // Split the pass's gates (dependency-ordered) into register groups of <= W target positions.
@@ -167,6 +168,12 @@ struct PassBuilder {
                 HostOp op;
                 op.gate = gi;
                 DevOp& d = op.dev;
+                if (phase_free && g.ctrl == 0) {
+                    if (kind == OPK_RX) kind = OPK_PF_RX;
+                    else if (kind == OPK_RY) kind = OPK_PF_RY;
+                    else if (kind == OPK_RZ) kind = OPK_PF_RZ;
+                    else if (kind == OPK_GP) kind = OPK_PF_GP;
+                }
                 d.kind = kind;
                 d.slot = opk_is_param(kind) ? g.slot : -1;
                 d.tap = -1;
@@ -236,7 +243,8 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     }
     if (adjoint) std::reverse(remaining.begin(), remaining.end());
 
-    PassBuilder pb{gates, adjoint, W, std::min(std::max(opts.taps_per_group, 1), kMaxTapsPerGroup), {}};
+    PassBuilder pb{gates, adjoint, W, std::min(std::max(opts.taps_per_group, 1), kMaxTapsPerGroup), opts.phase_free,
+                   {}};
     int n_slots = 0;
     for (auto& g : gates) n_slots = std::max(n_slots, g.slot + 1);
     pb.tap_id_of_slot.assign(n_slots, -1);

@@ -32,10 +32,11 @@ struct SchedOptions {
     int max_ops_per_pass = 0;  // 0: unlimited; 1 gives the unfused one-gate-per-pass schedule
     int taps_per_group = 8;    // derivative taps a register group may hold (<= kMaxTapsPerGroup)
     int amp_bytes = 8;         // 8 (complex64) or 16 (complex128): selects the LDS swizzle
+    bool phase_free = false;   // uncontrolled RX / RY / RZ / GP lower to their OPK_PF_* forms
     bool operator==(const SchedOptions& o) const {
         return tile_qubits == o.tile_qubits && low_run == o.low_run && group_width == o.group_width &&
                max_ops_per_pass == o.max_ops_per_pass && taps_per_group == o.taps_per_group &&
-               amp_bytes == o.amp_bytes;
+               amp_bytes == o.amp_bytes && phase_free == o.phase_free;
     }
 };
 

@@ -6,7 +6,7 @@ import pytest
 import scipy.sparse as sp
 
 import mindquantum_amd as mq
-from mindquantum_amd import ir
+from mindquantum_amd import _native, ir
 from oracle import cpu, dense
 from tests.conftest import hea, random_hamiltonian
 from tests.test_gate_families import family_circuit
@@ -58,6 +58,7 @@ def test_csr_copy_of_pauli_hamiltonian_matches_pauli_path(dtype):
     ans = rng.uniform(0, 2 * np.pi, size=len(circ.ansatz_params_name))
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f1, ge1, ga1 = sim.get_expectation_with_grad([h_pauli, h_csr], circ)(enc, ans)
+    assert _native.get_option("phase_free_used") == _native.get_option("phase_free")  # both hermitian
     names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
     f_ref, g_ref = cpu.grad_batch(n, dtype, ir.lower(circ, names), [h_pauli.terms_ir()], enc, ans, batch_threads=4)
     g1 = np.concatenate([ge1, ga1], axis=2)
@@ -78,6 +79,7 @@ def test_hermitian_gradient_of_a_general_matrix(dtype):
     vals = np.random.default_rng(1).uniform(0, 6, size=len(names))
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g = sim.get_expectation_with_grad(mq.Hamiltonian(m, dtype), circ)(vals)
+    assert _native.get_option("phase_free_used") == 0  # <H psi|psi> is not real: exact plans
     f_ref, g_ref = dense.gradient(circ, dict(zip(names, vals)), names, n, m)
     scale = float(np.abs(m).sum(axis=1).max())
     assert abs(f[0, 0] - f_ref) < tol(dtype, scale)

@@ -36,6 +36,18 @@ def test_every_gate_kind_compiles(dtype_code, adjoint):
     compile_circuit(circ, n, dtype_code, adjoint)
 
 
+@pytest.mark.parametrize("dtype_code", [_native.MQA_C64, _native.MQA_C128])
+@pytest.mark.parametrize("mode", [2, 3])
+def test_phase_free_forms_compile(dtype_code, mode):
+    """The phase-free forward (2) and back-sweep (3) kernels of the hermitian gradient: shear
+    rotations, one-amplitude RZ phases, dropped GP, CNOTs across the tile as selects."""
+    n = 14
+    circ = hea(n, 2, encoder=True)
+    circ += G.GlobalPhase("g").on(3)
+    circ += G.RZ("c").on(2, 13)
+    compile_circuit(circ, n, dtype_code, mode)
+
+
 def test_hea_compiles_both_directions():
     circ = hea(14, 3, encoder=True)
     assert compile_circuit(circ, 14, _native.MQA_C64, False) >= 0
