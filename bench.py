@@ -360,6 +360,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile widths (no autotuning)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the device copy probe (counter calibration for profiling runs)")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a library option (mqa_set_option) before the workloads; repeatable")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -378,6 +380,9 @@ def main():
         return
     # per-circuit compiled passes, compiled before the first launch (inside the warmup steps)
     _native.set_option("jit", 0 if args.interpreted else 2)
+    for kv in args.option:
+        key, _, val = kv.partition("=")
+        _native.set_option(key.strip(), int(val))
 
     tune = not args.no_tune and not args.interpreted
     value, ms_step, stats, n_params, n_gates = run_adjoint(mq.complex64, args.qubits, args.layers, args.batch,

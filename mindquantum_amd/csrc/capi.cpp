@@ -77,6 +77,9 @@ struct Options {
     std::atomic<int64_t> checkpoints{0};
     // 1: hermitian gradients run phase-free plans (uncontrolled rotations drop global phases)
     std::atomic<int64_t> phase_free{1};
+    // 1: compiled complex64 back-sweep passes run persistent, prefetching the next tile;
+    // jit_persistent_blocks > 0 caps their grid (tests: many units per block)
+    std::atomic<int64_t> jit_persistent{0}, jit_persist_blocks{0};
 };
 Options g_opt;
 
@@ -94,7 +97,8 @@ std::vector<OptEntry> opt_table() {
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
-            {"phase_free", &g_opt.phase_free, 1}};
+            {"phase_free", &g_opt.phase_free, 1}, {"jit_persistent", &g_opt.jit_persistent, 0},
+            {"jit_persistent_blocks", &g_opt.jit_persist_blocks, 0}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -154,7 +158,8 @@ struct DevPlan {
     DevBuf passes, groups, ops, pool, tap_ids;
     bool full = false;
     std::mutex jit_mu;
-    std::unique_ptr<JitPlan> jit;  // compiled passes (created on first large launch)
+    std::unique_ptr<JitPlan> jit;          // compiled passes (created on first large launch)
+    std::unique_ptr<JitPlan> jit_persist;  // the same as persistent kernels (option "jit_persistent")
 };
 
 // ---------------------------------------------------------------- kernel timing
@@ -487,6 +492,7 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 }
 
 std::atomic<int64_t> g_jit_launches{0};
+std::atomic<int64_t> g_jit_persist_launches{0};  // of those, persistent kernels
 std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
 std::atomic<int64_t> g_ckpt_used{0};    // checkpoints the last batched gradient call held
 std::atomic<int64_t> g_pf_used{0};      // 1: the last batched gradient call ran phase-free plans
@@ -519,20 +525,24 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     a.n_tap_ids = n_tap_ids;
     a.tap_ids = dp->tap_ids.as<int32_t>();
     a.partials = partials;
+    a.batch = batch;
     const double bytes = double(batch) * double(sim->state_bytes()) * (pl.adjoint ? 4.0 : 2.0);
     JitPlan* jp = nullptr;
     if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
         std::lock_guard<std::mutex> lk(dp->jit_mu);
-        if (!dp->jit) dp->jit = std::make_unique<JitPlan>(pl, sim->dtype);
-        if (g_opt.jit == 2) dp->jit->wait();
-        if (dp->jit->ready()) jp = dp->jit.get();  // until then (or if it failed): the interpreted kernel
+        const bool persist = g_opt.jit_persistent != 0 && jit_can_persist(pl, sim->dtype);
+        auto& slot = persist ? dp->jit_persist : dp->jit;
+        if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, persist);
+        if (g_opt.jit == 2) slot->wait();
+        if (slot->ready()) jp = slot.get();  // until then (or if it failed): the interpreted kernel
     }
     const size_t lds = tile_pass_lds_bytes(sim->dtype, pl.adjoint, pl.K, pl.C, std::max(pl.max_taps, 1));
     for (int i = p0; i < p1; i++) {
         a.pass_index = i;
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
             if (jp) {
-                hip_check(jp->launch(i, a, batch, lds, sim->stream), "compiled tile pass");
+                hip_check(jp->launch(i, a, batch, lds, sim->stream, unsigned(std::max<int64_t>(0, g_opt.jit_persist_blocks))),
+                          "compiled tile pass");
             } else {
                 hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C,
                                            std::max(pl.max_taps, 1), sim->stream),
@@ -540,6 +550,7 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
             }
         });
         if (jp) count_jit_launch();
+        if (jp && jp->persistent()) g_jit_persist_launches.fetch_add(1);
     }
 }
 
@@ -766,6 +777,10 @@ int mqa_get_option(const char* key, int64_t* value) {
             *value = g_jit_launches.load();
             return;
         }
+        if (std::strcmp(key, "jit_persistent_launches") == 0) {  // read-only: of those, persistent ones
+            *value = g_jit_persist_launches.load();
+            return;
+        }
         if (std::strcmp(key, "grad_recomputes") == 0) {  // read-only: see mqa_sim_grad_batch
             *value = g_recomputes.load();
             return;
@@ -944,10 +959,11 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        require(adjoint >= 0 && adjoint <= 3, "adjoint must be 0..3");
-        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0;
+        require(adjoint >= 0 && adjoint <= 7, "adjoint must be 0..7");
+        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, persist = (adjoint & 4) != 0;
         const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf));
-        JitPlan jp(pl, dtype);
+        require(!persist || jit_can_persist(pl, dtype), "persistent passes are complex64 back-sweeps only");
+        JitPlan jp(pl, dtype, persist);
         jp.wait();
         if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());
         if (seconds) *seconds = jp.compile_seconds();

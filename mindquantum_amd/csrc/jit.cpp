@@ -113,7 +113,9 @@ std::atomic<int64_t> g_compile_us{0};
 }
 double jit_compile_seconds_total() { return double(g_compile_us.load()) * 1e-6; }
 
-std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
+std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool persistent) {
+    if (persistent && !jit_can_persist(pl, dtype))
+        throw std::invalid_argument("code generation: persistent passes are complex64 back-sweeps only");
     const DevPass& ps = pl.dev_passes.at(size_t(pass_index));
     const bool adj = pl.adjoint;
     const char* T = dtype == 0 ? "float" : "double";
@@ -133,10 +135,28 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
       << "  pass.tap_begin = " << ps.tap_begin << ";\n"
       << "  pass.n_taps = " << ps.n_taps << ";\n"
       << "  pass.low_run = " << ps.low_run << ";\n"
-      << "  TileFrame<T> f = tile_begin<T, " << A << ", " << pl.K << ">(args, pass, smem);\n"
-      << "  const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;\n"
-      << "  const T* pool = static_cast<const T*>(args.pool);\n"
-      << "  (void)ang_row; (void)pool;\n";
+      << "  const T* pool = (const T*)as_const(static_cast<const T*>(args.pool));\n"
+      << "  (void)pool;\n";
+    if (persistent) {
+        o << "  TileFrame<T> f = stream_setup<" << pl.K << ">(pass, smem);\n"
+          << "  const uint64_t n_units = args.n_tiles * args.batch;\n"
+          << "  uint64_t unit = blockIdx.x;\n"
+          << "  Prefetch64 pre;\n"
+          << "  if (unit < n_units) {\n    stream_unit(f, args, pass, unit);\n    stream_fetch(pre, f);\n  }\n"
+          << "  for (; unit < n_units; unit += gridDim.x) {\n"
+          << "  stream_unit(f, args, pass, unit);\n"
+          << "  __syncthreads();  // the previous unit's LDS reads are done\n"
+          << "  stream_stage(pre, f);\n"
+          << "  __syncthreads();\n"
+          << "  if (unit + gridDim.x < n_units) {  // next unit's tiles: in flight during this unit's groups\n"
+          << "    TileFrame<T> fn = f;\n    stream_unit(fn, args, pass, unit + gridDim.x);\n"
+          << "    stream_fetch(pre, fn);\n  }\n";
+    } else {
+        o << "  TileFrame<T> f = tile_begin<T, " << A << ", " << pl.K << ">(args, pass, smem);\n";
+    }
+    o << "  const_ptr<double> ang_row = args.angles ? as_const(args.angles + 2 * f.sample * uint64_t(args.n_slots)) "
+         ": nullptr;\n"
+      << "  (void)ang_row;\n";
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
         o << "  {  // group " << gi << "\n"
@@ -223,8 +243,9 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype) {
           << "    __syncthreads();\n"
           << "  }\n";
     }
-    o << "  tile_end<T, " << A << ">(args, pass, f);\n"
-      << "}\n"
+    o << "  tile_end<T, " << A << ">(args, pass, f);\n";
+    if (persistent) o << "  }\n";
+    o << "}\n"
       << "}  // namespace mqa\n";
     return o.str();
 }
@@ -258,7 +279,8 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log) {
     return ok;
 }
 
-JitPlan::JitPlan(const Plan& pl, int dtype) : plan_(pl), dtype_(dtype) {
+JitPlan::JitPlan(const Plan& pl, int dtype, bool persistent)
+    : plan_(pl), dtype_(dtype), persistent_(persistent && jit_can_persist(pl, dtype)) {
     code_.resize(plan_.dev_passes.size());
     worker_ = std::thread([this] { compile_all(); });
 }
@@ -281,7 +303,7 @@ void JitPlan::compile_all() {
     std::vector<std::string> src(static_cast<size_t>(n)), err(static_cast<size_t>(n));
     std::vector<char> done(static_cast<size_t>(n), 0);
     try {
-        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_);
+        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, persistent_);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
         error_ = e.what();
@@ -358,12 +380,14 @@ std::string JitPlan::error() const {
     return error_;
 }
 
-hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
+hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s,
+                           unsigned max_blocks) {
     if (!ready()) return hipErrorNotReady;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     hipFunction_t fn = nullptr;
+    unsigned resident = 0;
     {
         std::lock_guard<std::mutex> lk(mu_);
         auto& fns = functions_[dev];
@@ -381,10 +405,44 @@ hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size
             }
         }
         fn = fns.at(size_t(pass));
+        if (persistent_) {
+            // one block per resident slot: blocks that could not start at once would run their
+            // whole share of units after the others
+            auto& grid = grid_[dev];
+            if (grid.empty()) grid.assign(code_.size(), 0u);
+            if (grid[size_t(pass)] == 0) {
+                const int nt = int(1u << (plan_.K - kGroupWidth));
+                int per_cu = 0, cus = 0;
+                e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+                if (e != hipSuccess) return e;
+                if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds) != hipSuccess) {
+                    (void)hipGetLastError();
+                    // from the kernel's registers and LDS: 512 VGPRs per SIMD lane, 4 SIMDs, 160 KiB
+                    int regs = 0;
+                    const int waves = std::max(1, nt / 64);
+                    if (hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) == hipSuccess && regs > 0) {
+                        const int waves_per_simd = 512 / ((regs + 7) / 8 * 8);
+                        per_cu = std::min(waves_per_simd * 4 / waves, int((160u << 10) / std::max<size_t>(lds, 1)));
+                    } else {
+                        (void)hipGetLastError();
+                        per_cu = 1;
+                    }
+                }
+                grid[size_t(pass)] = unsigned(std::max(1, per_cu) * std::max(1, cus));
+            }
+            resident = grid[size_t(pass)];
+        }
     }
     TilePassArgs args = a;
+    args.batch = batch;
     void* params[] = {&args};
     const unsigned nt = 1u << (plan_.K - kGroupWidth);
+    if (persistent_) {
+        const uint64_t units = a.n_tiles * batch;
+        unsigned blocks = unsigned(std::min<uint64_t>(units, resident));
+        if (max_blocks > 0) blocks = std::min(blocks, max_blocks);
+        return hipModuleLaunchKernel(fn, blocks, 1, 1, nt, 1, 1, unsigned(lds), s, params, nullptr);
+    }
     e = hipModuleLaunchKernel(fn, unsigned(a.n_tiles), unsigned(batch), 1, nt, 1, 1, unsigned(lds), s, params,
                               nullptr);
     return e;

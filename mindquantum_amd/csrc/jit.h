@@ -27,7 +27,12 @@ const char* jit_kernel_name(bool adjoint);
 
 // HIP source of pass `pass` of a plan (dtype 0: complex64, 1: complex128).  It includes
 // "device_ops.h" and defines one extern "C" kernel (jit_kernel_name) taking TilePassArgs.
-std::string jit_pass_source(const Plan& pl, int pass, int dtype);
+// persistent (complex64 back-sweep plans only): the kernel walks (tile, sample) units with a
+// stride of gridDim.x and prefetches the next unit's tiles into registers (device_ops.h).
+std::string jit_pass_source(const Plan& pl, int pass, int dtype, bool persistent = false);
+
+// Whether a plan's compiled passes may run persistent.
+inline bool jit_can_persist(const Plan& pl, int dtype) { return pl.adjoint && dtype == 0; }
 
 // Wall time spent compiling generated passes in this process.
 double jit_compile_seconds_total();
@@ -38,7 +43,7 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log);
 class JitPlan {
   public:
     // Copies the plan and starts compiling every pass on a background thread.
-    JitPlan(const Plan& pl, int dtype);
+    JitPlan(const Plan& pl, int dtype, bool persistent = false);
     ~JitPlan();
     JitPlan(const JitPlan&) = delete;
     JitPlan& operator=(const JitPlan&) = delete;
@@ -48,9 +53,12 @@ class JitPlan {
     void wait();                       // block until compiled (or failed)
     std::string error() const;
     double compile_seconds() const { return seconds_; }
+    bool persistent() const { return persistent_; }
 
     // Launch pass `pass` on the current device and stream (modules load on first use).
-    hipError_t launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s);
+    // max_blocks > 0 caps the grid of a persistent pass.
+    hipError_t launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s,
+                      unsigned max_blocks = 0);
 
   private:
     enum { kCompiling = 0, kReady = 1, kFailed = 2 };
@@ -58,6 +66,8 @@ class JitPlan {
 
     Plan plan_;
     int dtype_;
+    bool persistent_;
+    std::map<int, std::vector<unsigned>> grid_;  // per device: resident blocks of each persistent pass
     std::atomic<int> state_{kCompiling};
     std::vector<std::string> code_;
     std::string error_;

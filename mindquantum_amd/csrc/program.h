@@ -161,6 +161,7 @@ struct TilePassArgs {
     int n_tap_ids;
     const int32_t* tap_ids;
     double* partials;      // [batch][n_tap_ids][n_tiles] Re(tap)
+    uint64_t batch;        // states (persistent passes walk n_tiles * batch units)
 };
 
 // Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are

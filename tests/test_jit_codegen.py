@@ -59,3 +59,18 @@ def test_bad_arguments_are_reported():
     nc = ir.NativeCircuit(circ, [])
     st = _native.lib().mqa_circuit_jit_compile(nc.handle, 2, _native.MQA_C64, 0, None)
     assert st == _native.MQA_ERR_ARG
+
+
+@pytest.mark.parametrize("mode", [5, 7])
+def test_persistent_back_sweep_compiles(mode):
+    """Persistent complex64 back-sweep kernels (exact 5 and phase-free 7)."""
+    circ = hea(14, 2, encoder=True)
+    circ += G.RY("c").on(2, 13)
+    compile_circuit(circ, 14, _native.MQA_C64, mode)
+
+
+def test_persistent_is_c64_back_sweep_only():
+    circ = hea(8, 1)
+    nc = ir.NativeCircuit(circ, list(circ.params_name))
+    for dtype_code, mode in ((_native.MQA_C128, 5), (_native.MQA_C64, 4)):
+        assert _native.lib().mqa_circuit_jit_compile(nc.handle, 8, dtype_code, mode, None) == _native.MQA_ERR_ARG
