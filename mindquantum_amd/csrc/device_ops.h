@@ -73,15 +73,46 @@ __device__ __forceinline__ double dpp_mov(double v) {
     const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
+// Cross-row steps with gfx950's lane swaps (VALU, no LDS crossbar): permlane16_swap exchanges
+// odd rows of its first operand with even rows of its second, permlane32_swap the upper half of
+// the first with the lower half of the second; with v in both, the two results sum to the
+// xor-16 / xor-32 partner sum in every lane.
+template <bool ROWS32>
+__device__ __forceinline__ float swap_sum(float v) {
+    const unsigned u = __float_as_uint(v);
+    if constexpr (ROWS32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+}
+template <bool ROWS32>
+__device__ __forceinline__ double swap_sum(double v) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(b), hi = unsigned(b >> 32);
+    unsigned a0, a1, b0, b1;
+    if constexpr (ROWS32) {
+        const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        a0 = rl[0], a1 = rl[1], b0 = rh[0], b1 = rh[1];
+    } else {
+        const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        a0 = rl[0], a1 = rl[1], b0 = rh[0], b1 = rh[1];
+    }
+    return __longlong_as_double((static_cast<long long>(b0) << 32) | a0) +
+           __longlong_as_double((static_cast<long long>(b1) << 32) | a1);
+}
 template <typename T>
 __device__ __forceinline__ T wave_sum64(T v) {
     v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
     v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
     v += dpp_mov<0x141>(v);  // row_half_mirror
     v += dpp_mov<0x140>(v);  // row_mirror
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+    v = swap_sum<false>(v);  // rows 0 + 1, 2 + 3
+    return swap_sum<true>(v);
 }
 
 // Per-thread derivative-tap accumulators of one register group.  Only Re(tap) is kept: every
@@ -858,6 +889,54 @@ template <typename T>
 __device__ __forceinline__ void tap_to_lds(const TileFrame<T>& f, T v, int tap) {
     v = (f.wave_width == 64) ? wave_sum64(v) : wave_sum(v, f.wave_width);
     if ((f.tid & 63) == 0) f.lds_taps[tap * f.n_waves + f.wave] = double(v);
+}
+
+// Lane swaps of two different values (no copies): permlane16_swap(a, b) leaves a's rows 0 / 2
+// with b's rows 0 / 2 and the reverse, so a + b holds a's xor-16 sums in rows 0 / 2 and b's in
+// rows 1 / 3; permlane32_swap(c, d) then gives c's halves summed in lanes 0-31 and d's in 32-63.
+template <bool ROWS32>
+__device__ __forceinline__ float swap_pair_sum(float a, float b) {
+    const auto r = ROWS32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                          : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+template <bool ROWS32>
+__device__ __forceinline__ double swap_pair_sum(double a, double b) {
+    const unsigned long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const auto rl = ROWS32 ? __builtin_amdgcn_permlane32_swap(unsigned(x), unsigned(y), false, false)
+                           : __builtin_amdgcn_permlane16_swap(unsigned(x), unsigned(y), false, false);
+    const auto rh = ROWS32 ? __builtin_amdgcn_permlane32_swap(unsigned(x >> 32), unsigned(y >> 32), false, false)
+                           : __builtin_amdgcn_permlane16_swap(unsigned(x >> 32), unsigned(y >> 32), false, false);
+    return __longlong_as_double((static_cast<long long>(rh[0]) << 32) | rl[0]) +
+           __longlong_as_double((static_cast<long long>(rh[1]) << 32) | rl[1]);
+}
+
+// Four Re(taps) of a full 64-lane wave reduced together: row sums by DPP, then two pairwise
+// swaps (rows 0+1 / 2+3 for (t0, t1) and (t2, t3)) and one across halves.  Lanes 0, 16, 32 and
+// 48 end with the totals of t0, t1, t2, t3 and write them to the LDS tap table (index < 0:
+// no tap, its value is ignored).
+template <typename T>
+__device__ __forceinline__ void taps4_to_lds(const TileFrame<T>& f, T t0, int i0, T t1, int i1, T t2, int i2, T t3,
+                                             int i3) {
+    if (f.wave_width != 64) {
+        if (i0 >= 0) tap_to_lds(f, t0, i0);
+        if (i1 >= 0) tap_to_lds(f, t1, i1);
+        if (i2 >= 0) tap_to_lds(f, t2, i2);
+        if (i3 >= 0) tap_to_lds(f, t3, i3);
+        return;
+    }
+    auto rows = [](T v) {
+        v += dpp_mov<0xB1>(v);
+        v += dpp_mov<0x4E>(v);
+        v += dpp_mov<0x141>(v);
+        return v + dpp_mov<0x140>(v);
+    };
+    const T c = swap_pair_sum<false>(rows(t0), rows(t1));
+    const T d = swap_pair_sum<false>(rows(t2), rows(t3));
+    const T e = swap_pair_sum<true>(c, d);
+    const uint32_t lane = f.tid & 63;
+    const int idx = lane == 0 ? i0 : lane == 16 ? i1 : lane == 32 ? i2 : lane == 48 ? i3 : -1;
+    if (idx >= 0) f.lds_taps[idx * f.n_waves + f.wave] = double(e);
 }
 
 }  // namespace mqa

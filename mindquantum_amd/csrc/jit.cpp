@@ -238,7 +238,14 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool pers
             o << ";\n";
         }
         o << b.str();
-        for (auto& [var, tap] : taps) o << "    tap_to_lds<T>(f, tap" << var << ", " << tap << ");\n";
+        for (size_t k = 0; k < taps.size(); k += 4) {  // four taps per batched cross-lane reduction
+            o << "    taps4_to_lds<T>(f";
+            for (size_t j = k; j < k + 4; j++) {
+                if (j < taps.size()) o << ", tap" << taps[j].first << ", " << taps[j].second;
+                else o << ", T(0), -1";
+            }
+            o << ");\n";
+        }
         o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
           << "    __syncthreads();\n"
           << "  }\n";
