@@ -301,6 +301,44 @@ __device__ __forceinline__ void ham_tile_body(const HamTileArgs& args, unsigned 
                 ar[J] += wr * x.x - wi * x.y;
                 ai[J] += wr * x.y + wi * x.x;
             }
+        } else if (grp.kind == HAMW_WHT) {
+            const uint32_t tsrc = ((2u * tid) ^ grp.flipl) & tmask;
+            // register part of the flip, in register-mask coordinates
+            const uint32_t freg = (grp.flipl & 1u) | (((grp.flipl >> JS) & 7u) << 1);
+            const_ptr<DevHamTileTerm> tc = as_const(tm);
+            T wr[NA], wi[NA];
+            int t = 0;
+#pragma unroll
+            for (int m = 0; m < NA; m++) {  // W[m]: terms of register mask m, tile and thread signs
+                T sr = 0, si = 0;
+                const int end = grp.seg_end[m];
+                for (; t < end; t++) {
+                    const uint32_t neg =
+                        (uint32_t(__popcll(base & tc[t].yz_nl)) + __popc(tsrc & tc[t].yzl)) & 1u;
+                    sr += flip_sign(T(tc[t].coef_re), neg);
+                    si += flip_sign(T(tc[t].coef_im), neg);
+                }
+                const uint32_t fneg = uint32_t(__popc(freg & uint32_t(m))) & 1u;
+                wr[m] = flip_sign(sr, fneg);
+                wi[m] = flip_sign(si, fneg);
+            }
+            // weight of register J = sum_m W[m] (-1)^{popcount(J & m)}: Walsh-Hadamard transform
+#pragma unroll
+            for (int b = 1; b < NA; b <<= 1) {
+#pragma unroll
+                for (int m = 0; m < NA; m++) {
+                    if (m & b) continue;
+                    const T xr = wr[m], xi = wi[m];
+                    wr[m] = xr + wr[m | b], wi[m] = xi + wi[m | b];
+                    wr[m | b] = xr - wr[m | b], wi[m | b] = xi - wi[m | b];
+                }
+            }
+#pragma unroll
+            for (int J = 0; J < NA; J++) {
+                const V x = t_ket[a0 ^ swz<T>(jb(J) ^ grp.flipl)];
+                ar[J] += wr[J] * x.x - wi[J] * x.y;
+                ai[J] += wr[J] * x.y + wi[J] * x.x;
+            }
         } else {
             const uint32_t tsrc = ((2u * tid) ^ grp.flipl) & tmask;
             // two halves of the owned amplitudes bound the weight registers

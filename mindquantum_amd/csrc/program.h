@@ -182,12 +182,18 @@ enum HamWeightKind : int32_t {
     HAMW_TILE = 0,    // no term has yz bits inside the tile: one weight per tile
     HAMW_THREAD = 1,  // yz bits inside the tile only on thread-index bits: one weight per thread
     HAMW_FULL = 2,    // a weight per amplitude
+    // a weight per amplitude through the register masks: the group's terms are sorted by the
+    // 4-bit mask m of their yz bits on register positions (tile bits 0, K-3, K-2, K-1) and
+    // seg_end[m] ends each mask's run; a thread sums each run's coefficients with its tile and
+    // thread signs, and a 16-point Walsh-Hadamard transform gives every register's weight
+    HAMW_WHT = 3,
 };
-struct DevHamTileGroup {     // 16 bytes
+struct DevHamTileGroup {     // 48 bytes
     uint32_t flipl;          // flip mask in tile-index coordinates
     int32_t term_begin;
     int32_t n_terms;
     int32_t kind;            // HamWeightKind
+    uint16_t seg_end[16];    // HAMW_WHT: end of the terms of register mask m (relative to term_begin)
 };
 struct DevHamTileTerm {      // 32 bytes
     uint64_t yz_nl;          // yz bits outside the tile (global index positions)

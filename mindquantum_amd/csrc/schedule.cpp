@@ -415,6 +415,20 @@ bool build_ham_plan(const std::vector<DevHamGroup>& groups, const std::vector<De
                 else if (tt.yzl && tg.kind == HAMW_TILE) tg.kind = HAMW_THREAD;
                 plan.terms.push_back(tt);
             }
+            if (tg.kind == HAMW_FULL && tg.n_terms <= 0xffff) {
+                // sort by register mask (bit 0 <- tile bit 0, bits 1..3 <- tile bits K-3..K-1)
+                auto reg_mask = [K](uint32_t yzl) { return (yzl & 1u) | (((yzl >> (K - 3)) & 7u) << 1); };
+                auto first = plan.terms.begin() + tg.term_begin;
+                std::stable_sort(first, plan.terms.end(), [&](const DevHamTileTerm& a, const DevHamTileTerm& b) {
+                    return reg_mask(a.yzl) < reg_mask(b.yzl);
+                });
+                int cnt = 0;
+                for (uint32_t m = 0; m < 16; m++) {
+                    for (auto it = first + cnt; it != plan.terms.end() && reg_mask(it->yzl) == m; ++it) cnt++;
+                    tg.seg_end[m] = uint16_t(cnt);
+                }
+                tg.kind = HAMW_WHT;
+            }
             plan.groups.push_back(tg);
         }
         plan.passes.push_back(dp);
