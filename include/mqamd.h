@@ -106,9 +106,11 @@ int mqa_device_count(void);
  * runs plans in which uncontrolled RX / RY / RZ / GP drop their global phase, see there; 0: exact
  * plans), "jit_persistent" (1: compiled complex64 back-sweep passes run one block per resident
  * slot, each walking tiles and loading the next tile while it computes the current one),
- * "jit_persistent_blocks" (> 0: at most that many blocks in a persistent pass).  mqa_get_option also reads the counters
- * "jit_launches", "jit_compile_ms", "grad_recomputes", "checkpoints_used", "phase_free_used" and
- * "jit_persistent_launches".  No counterpart
+ * "jit_persistent_blocks" (> 0: at most that many blocks in a persistent pass), "jit_single_buffer"
+ * (1, the default: compiled back-sweep passes keep psi and lambda in registers between register
+ * groups and share one LDS tile buffer, so twice as many blocks fit a CU; 0: two tile buffers).  mqa_get_option also reads the counters
+ * "jit_launches", "jit_compile_ms", "grad_recomputes", "checkpoints_used", "phase_free_used",
+ * "jit_persistent_launches" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
 int mqa_get_option(const char* key, int64_t* value);
@@ -153,7 +155,8 @@ int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtyp
 /* Generate and compile (hipRTC, host only, no GPU needed) the per-circuit kernels of the
  * forward (adjoint == 0) or back-sweep (1) schedule, or of their phase-free forms (2, 3: those
  * the hermitian gradient runs); + 4 for the persistent form of a complex64 back-sweep (option
- * "jit_persistent"); as the library does in the background for large launches (option "jit").  *seconds gets the compile wall time.  No counterpart in the
+ * "jit_persistent") or + 8 for its single-buffer form (option "jit_single_buffer"); as the library
+ * does in the background for large launches (option "jit").  *seconds gets the compile wall time.  No counterpart in the
  * reference: a build check of the code generator for tests and for callers that want to pay
  * the compile cost up front. */
 int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint,

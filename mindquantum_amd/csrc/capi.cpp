@@ -80,6 +80,9 @@ struct Options {
     // 1: compiled complex64 back-sweep passes run persistent, prefetching the next tile;
     // jit_persistent_blocks > 0 caps their grid (tests: many units per block)
     std::atomic<int64_t> jit_persistent{0}, jit_persist_blocks{0};
+    // 1: compiled back-sweep passes keep psi and lambda in registers and share one LDS tile
+    // buffer (twice the resident blocks per CU)
+    std::atomic<int64_t> jit_single_buffer{1};
 };
 Options g_opt;
 
@@ -98,7 +101,8 @@ std::vector<OptEntry> opt_table() {
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
             {"phase_free", &g_opt.phase_free, 1}, {"jit_persistent", &g_opt.jit_persistent, 0},
-            {"jit_persistent_blocks", &g_opt.jit_persist_blocks, 0}};
+            {"jit_persistent_blocks", &g_opt.jit_persist_blocks, 0},
+            {"jit_single_buffer", &g_opt.jit_single_buffer, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -160,6 +164,7 @@ struct DevPlan {
     std::mutex jit_mu;
     std::unique_ptr<JitPlan> jit;          // compiled passes (created on first large launch)
     std::unique_ptr<JitPlan> jit_persist;  // the same as persistent kernels (option "jit_persistent")
+    std::unique_ptr<JitPlan> jit_sb;       // the same with one LDS tile buffer (option "jit_single_buffer")
 };
 
 // ---------------------------------------------------------------- kernel timing
@@ -493,6 +498,7 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 
 std::atomic<int64_t> g_jit_launches{0};
 std::atomic<int64_t> g_jit_persist_launches{0};  // of those, persistent kernels
+std::atomic<int64_t> g_jit_sb_launches{0};       // of those, single-buffer kernels
 std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
 std::atomic<int64_t> g_ckpt_used{0};    // checkpoints the last batched gradient call held
 std::atomic<int64_t> g_pf_used{0};      // 1: the last batched gradient call ran phase-free plans
@@ -530,13 +536,18 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     JitPlan* jp = nullptr;
     if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
         std::lock_guard<std::mutex> lk(dp->jit_mu);
-        const bool persist = g_opt.jit_persistent != 0 && jit_can_persist(pl, sim->dtype);
-        auto& slot = persist ? dp->jit_persist : dp->jit;
-        if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, persist);
+        JitForm form = JitForm::kTile;
+        if (g_opt.jit_persistent != 0 && jit_can_persist(pl, sim->dtype)) form = JitForm::kPersistent;
+        else if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, sim->dtype)) form = JitForm::kSingleBuffer;
+        auto& slot = form == JitForm::kPersistent ? dp->jit_persist : form == JitForm::kSingleBuffer ? dp->jit_sb : dp->jit;
+        if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, form);
         if (g_opt.jit == 2) slot->wait();
         if (slot->ready()) jp = slot.get();  // until then (or if it failed): the interpreted kernel
     }
-    const size_t lds = tile_pass_lds_bytes(sim->dtype, pl.adjoint, pl.K, pl.C, std::max(pl.max_taps, 1));
+    // a single-buffer pass holds one tile of LDS (the forward layout) plus the tap table
+    const bool one_tile = jp && jp->form() == JitForm::kSingleBuffer;
+    const size_t lds =
+        tile_pass_lds_bytes(sim->dtype, pl.adjoint && !one_tile, pl.K, pl.C, std::max(pl.max_taps, 1));
     for (int i = p0; i < p1; i++) {
         a.pass_index = i;
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
@@ -551,6 +562,7 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         });
         if (jp) count_jit_launch();
         if (jp && jp->persistent()) g_jit_persist_launches.fetch_add(1);
+        if (one_tile) g_jit_sb_launches.fetch_add(1);
     }
 }
 
@@ -777,6 +789,10 @@ int mqa_get_option(const char* key, int64_t* value) {
             *value = g_jit_launches.load();
             return;
         }
+        if (std::strcmp(key, "jit_single_buffer_launches") == 0) {  // read-only: of those, single-buffer ones
+            *value = g_jit_sb_launches.load();
+            return;
+        }
         if (std::strcmp(key, "jit_persistent_launches") == 0) {  // read-only: of those, persistent ones
             *value = g_jit_persist_launches.load();
             return;
@@ -959,11 +975,14 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        require(adjoint >= 0 && adjoint <= 7, "adjoint must be 0..7");
-        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, persist = (adjoint & 4) != 0;
+        require(adjoint >= 0 && adjoint <= 11 && (adjoint & 12) != 12, "adjoint must be 0..3, + 4 or + 8");
+        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, persist = (adjoint & 4) != 0,
+                   one_tile = (adjoint & 8) != 0;
         const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf));
         require(!persist || jit_can_persist(pl, dtype), "persistent passes are complex64 back-sweeps only");
-        JitPlan jp(pl, dtype, persist);
+        require(!one_tile || jit_can_single_buffer(pl, dtype),
+                "single-buffer passes are back-sweeps without dense 3- or 4-qubit blocks");
+        JitPlan jp(pl, dtype, persist ? JitForm::kPersistent : one_tile ? JitForm::kSingleBuffer : JitForm::kTile);
         jp.wait();
         if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());
         if (seconds) *seconds = jp.compile_seconds();

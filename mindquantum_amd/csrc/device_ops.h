@@ -773,6 +773,64 @@ __device__ __forceinline__ void tile_end(const TilePassArgs& args, const DevPass
     }
 }
 
+// ---------------------------------------------------------------- single-buffer back-sweep passes
+// Compiled back-sweep passes keep psi and lambda in registers between register groups and re-lay
+// them out through ONE LDS tile buffer, one state after the other: a block then holds one tile of
+// LDS instead of two (36 instead of 66 KB for 12-qubit complex64 tiles), so twice as many blocks
+// share a CU, at the price of four barriers per group transition instead of one.
+template <typename T, int KC>
+__device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const DevPass& pass,
+                                                 unsigned char* smem) {
+    using V = V2<T>;
+    TileFrame<T> f;
+    f.NT = 1u << (KC - kGroupWidth);
+    f.tid = threadIdx.x;
+    f.C = pass.low_run;
+    const int H = KC - f.C;
+    const LdsLayout lay = lds_layout(KC, f.C, false, int(sizeof(V)), 0);
+    f.t_psi = reinterpret_cast<V*>(smem + lay.psi);
+    f.t_lam = f.t_psi;
+    f.offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
+    f.lds_taps = reinterpret_cast<double*>(smem + lay.taps);
+    f.tile = blockIdx.x;
+    f.sample = blockIdx.y;
+    f.base = pdep64(f.tile, pass.nonlocal_mask);
+    for (uint32_t h = f.tid; h < (1u << H); h += f.NT) f.offtab[h] = pdep64(h, pass.local_hi_mask);
+    __syncthreads();
+    f.g_psi = static_cast<V*>(args.psi) + f.sample * args.stride + f.base;
+    f.g_lam = static_cast<V*>(args.lam) + f.sample * args.stride + f.base;
+    f.wave = int(f.tid >> 6);
+    f.n_waves = int((f.NT + 63) >> 6);
+    f.wave_width = f.NT < 64 ? int(f.NT) : 64;
+    return f;
+}
+
+// One state's registers <-> the LDS tile buffer in a group's layout.
+template <typename T>
+__device__ __forceinline__ void group_load1(const V2<T>* buf, uint32_t a0, const uint16_t* off, Regs<T>& r) {
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) {
+        const V2<T> x = buf[a0 ^ off[v]];
+        r.r[v] = x.x, r.i[v] = x.y;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void group_store1(V2<T>* buf, uint32_t a0, const uint16_t* off, const Regs<T>& r) {
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) buf[a0 ^ off[v]] = V2<T>{r.r[v], r.i[v]};
+}
+
+// This tile's partial of every tap of the pass (after the barrier that follows the last group).
+template <typename T>
+__device__ __forceinline__ void taps_out(const TilePassArgs& args, const DevPass& pass, const TileFrame<T>& f) {
+    for (int t = int(f.tid); t < pass.n_taps; t += int(f.NT)) {
+        double sr = 0;
+        for (int w = 0; w < f.n_waves; w++) sr += f.lds_taps[t * f.n_waves + w];
+        const int tap_id = args.tap_ids[pass.tap_begin + t];
+        args.partials[(f.sample * uint64_t(args.n_tap_ids) + uint64_t(tap_id)) * args.n_tiles + f.tile] = sr;
+    }
+}
+
 // ---------------------------------------------------------------- persistent passes
 // complex64 back-sweep passes compiled per circuit can run persistent: a block walks the
 // (tile, sample) units blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next unit's psi and

@@ -113,9 +113,13 @@ std::atomic<int64_t> g_compile_us{0};
 }
 double jit_compile_seconds_total() { return double(g_compile_us.load()) * 1e-6; }
 
-std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool persistent) {
+std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm form) {
+    const bool persistent = form == JitForm::kPersistent, sb = form == JitForm::kSingleBuffer;
     if (persistent && !jit_can_persist(pl, dtype))
         throw std::invalid_argument("code generation: persistent passes are complex64 back-sweeps only");
+    if (sb && !jit_can_single_buffer(pl, dtype))
+        throw std::invalid_argument(
+            "code generation: single-buffer passes are back-sweeps without dense 3- or 4-qubit blocks");
     const DevPass& ps = pl.dev_passes.at(size_t(pass_index));
     const bool adj = pl.adjoint;
     const char* T = dtype == 0 ? "float" : "double";
@@ -151,14 +155,52 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool pers
           << "  if (unit + gridDim.x < n_units) {  // next unit's tiles: in flight during this unit's groups\n"
           << "    TileFrame<T> fn = f;\n    stream_unit(fn, args, pass, unit + gridDim.x);\n"
           << "    stream_fetch(pre, fn);\n  }\n";
+    } else if (sb) {
+        o << "  TileFrame<T> f = sb_begin<T, " << pl.K << ">(args, pass, smem);\n";
     } else {
         o << "  TileFrame<T> f = tile_begin<T, " << A << ", " << pl.K << ">(args, pass, smem);\n";
     }
     o << "  const_ptr<double> ang_row = args.angles ? as_const(args.angles + 2 * f.sample * uint64_t(args.n_slots)) "
          ": nullptr;\n"
       << "  (void)ang_row;\n";
+    // single buffer: every group's layout at function scope; psi, then lambda, into group 0's layout
+    auto sb_move = [&](const char* what, int gi, const char* reg) {
+        o << "  group_" << what << "1<T>(f.t_psi, swz<T>(group_ctx(f.tid, pos_" << gi << ", spos_" << gi
+          << ").ub), off_" << gi << ", " << reg << ");\n";
+    };
+    if (sb) {
+        for (int gi = 0; gi < ps.n_groups; gi++) {
+            const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
+            o << "  constexpr uint8_t pos_" << gi << "[4] = {" << int(g.pos[0]) << ", " << int(g.pos[1]) << ", "
+              << int(g.pos[2]) << ", " << int(g.pos[3]) << "};\n"
+              << "  constexpr uint8_t spos_" << gi << "[4] = {" << int(g.spos[0]) << ", " << int(g.spos[1]) << ", "
+              << int(g.spos[2]) << ", " << int(g.spos[3]) << "};\n"
+              << "  constexpr uint16_t off_" << gi << "[16] = {";
+            for (int v = 0; v < 16; v++) o << (v ? ", " : "") << g.off[v];
+            o << "};\n";
+        }
+        o << "  Regs<T> p, l;\n"
+          << "  tile_load<T>(f.t_psi, f.g_psi, f.offtab, f.C, f.tid, f.NT);\n  __syncthreads();\n";
+        sb_move("load", 0, "p");
+        o << "  __syncthreads();\n  tile_load<T>(f.t_psi, f.g_lam, f.offtab, f.C, f.tid, f.NT);\n  __syncthreads();\n";
+        sb_move("load", 0, "l");
+    }
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
+        if (sb) {
+            if (gi > 0) {  // re-lay psi, then lambda, from group gi - 1's layout into group gi's
+                o << "  __syncthreads();\n";
+                sb_move("store", gi - 1, "p");
+                o << "  __syncthreads();\n";
+                sb_move("load", gi, "p");
+                o << "  __syncthreads();\n";
+                sb_move("store", gi - 1, "l");
+                o << "  __syncthreads();\n";
+                sb_move("load", gi, "l");
+            }
+            o << "  {  // group " << gi << "\n"
+              << "    const GroupCtx g = group_ctx(f.tid, pos_" << gi << ", spos_" << gi << ");\n";
+        } else {
         o << "  {  // group " << gi << "\n"
           << "    constexpr uint8_t pos[4] = {" << int(g.pos[0]) << ", " << int(g.pos[1]) << ", " << int(g.pos[2])
           << ", " << int(g.pos[3]) << "};\n"
@@ -171,6 +213,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool pers
           << "    const uint32_t a0 = swz<T>(g.ub);\n"
           << "    Regs<T> p, l;\n"
           << "    group_load<T, " << A << ">(f, a0, off, p, l);\n";
+        }
         std::vector<std::pair<int, int>> taps;  // (variable, pass-level tap index)
         std::ostringstream b;                   // the group's ops
         for (int oi = 0; oi < g.n_ops; oi++) {
@@ -246,11 +289,26 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, bool pers
             }
             o << ");\n";
         }
+        if (sb) {
+            o << "  }\n";
+            continue;
+        }
         o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
           << "    __syncthreads();\n"
           << "  }\n";
     }
-    o << "  tile_end<T, " << A << ">(args, pass, f);\n";
+    if (sb) {  // psi, then lambda, back through the buffer to HBM; the tap partials
+        const int last = ps.n_groups - 1;
+        o << "  uint32_t tid_st = f.tid;\n  asm volatile(\"\" : \"+v\"(tid_st));\n  __syncthreads();\n";
+        sb_move("store", last, "p");
+        o << "  __syncthreads();\n  tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);\n"
+          << "  __syncthreads();\n";
+        sb_move("store", last, "l");
+        o << "  __syncthreads();\n  tile_store<T>(f.t_psi, f.g_lam, f.offtab, f.C, tid_st, f.NT);\n"
+          << "  taps_out<T>(args, pass, f);\n";
+    } else {
+        o << "  tile_end<T, " << A << ">(args, pass, f);\n";
+    }
     if (persistent) o << "  }\n";
     o << "}\n"
       << "}  // namespace mqa\n";
@@ -286,8 +344,11 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log) {
     return ok;
 }
 
-JitPlan::JitPlan(const Plan& pl, int dtype, bool persistent)
-    : plan_(pl), dtype_(dtype), persistent_(persistent && jit_can_persist(pl, dtype)) {
+JitPlan::JitPlan(const Plan& pl, int dtype, JitForm form) : plan_(pl), dtype_(dtype), form_(form) {
+    if ((form_ == JitForm::kPersistent && !jit_can_persist(pl, dtype)) ||
+        (form_ == JitForm::kSingleBuffer && !jit_can_single_buffer(pl, dtype)))
+        form_ = JitForm::kTile;
+    persistent_ = form_ == JitForm::kPersistent;
     code_.resize(plan_.dev_passes.size());
     worker_ = std::thread([this] { compile_all(); });
 }
@@ -310,7 +371,7 @@ void JitPlan::compile_all() {
     std::vector<std::string> src(static_cast<size_t>(n)), err(static_cast<size_t>(n));
     std::vector<char> done(static_cast<size_t>(n), 0);
     try {
-        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, persistent_);
+        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, form_);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
         error_ = e.what();

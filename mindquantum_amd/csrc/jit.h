@@ -27,12 +27,22 @@ const char* jit_kernel_name(bool adjoint);
 
 // HIP source of pass `pass` of a plan (dtype 0: complex64, 1: complex128).  It includes
 // "device_ops.h" and defines one extern "C" kernel (jit_kernel_name) taking TilePassArgs.
-// persistent (complex64 back-sweep plans only): the kernel walks (tile, sample) units with a
-// stride of gridDim.x and prefetches the next unit's tiles into registers (device_ops.h).
-std::string jit_pass_source(const Plan& pl, int pass, int dtype, bool persistent = false);
+// Forms of a compiled pass: kTile, one workgroup per (tile, sample) with the tiles in LDS (every
+// plan); kPersistent (complex64 back-sweeps), blocks walk (tile, sample) units with a stride of
+// gridDim.x and prefetch the next unit's tiles into registers; kSingleBuffer (back-sweeps), psi and
+// lambda stay in registers between register groups and share one LDS tile buffer (device_ops.h).
+enum class JitForm { kTile = 0, kPersistent = 1, kSingleBuffer = 2 };
+std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile);
 
-// Whether a plan's compiled passes may run persistent.
+// Whether a plan's compiled passes may take a form.
 inline bool jit_can_persist(const Plan& pl, int dtype) { return pl.adjoint && dtype == 0; }
+inline bool jit_can_single_buffer(const Plan& pl, int dtype) {
+    (void)dtype;
+    if (!pl.adjoint) return false;
+    for (const DevOp& op : pl.dev_ops)  // dense 3- / 4-qubit blocks re-read their inputs from the tile
+        if (op.kind == OPK_M3 || op.kind == OPK_M4) return false;
+    return true;
+}
 
 // Wall time spent compiling generated passes in this process.
 double jit_compile_seconds_total();
@@ -43,7 +53,7 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log);
 class JitPlan {
   public:
     // Copies the plan and starts compiling every pass on a background thread.
-    JitPlan(const Plan& pl, int dtype, bool persistent = false);
+    JitPlan(const Plan& pl, int dtype, JitForm form = JitForm::kTile);
     ~JitPlan();
     JitPlan(const JitPlan&) = delete;
     JitPlan& operator=(const JitPlan&) = delete;
@@ -54,6 +64,7 @@ class JitPlan {
     std::string error() const;
     double compile_seconds() const { return seconds_; }
     bool persistent() const { return persistent_; }
+    JitForm form() const { return form_; }
 
     // Launch pass `pass` on the current device and stream (modules load on first use).
     // max_blocks > 0 caps the grid of a persistent pass.
@@ -66,6 +77,7 @@ class JitPlan {
 
     Plan plan_;
     int dtype_;
+    JitForm form_;
     bool persistent_;
     std::map<int, std::vector<unsigned>> grid_;  // per device: resident blocks of each persistent pass
     std::atomic<int> state_{kCompiling};

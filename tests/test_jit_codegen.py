@@ -74,3 +74,29 @@ def test_persistent_is_c64_back_sweep_only():
     nc = ir.NativeCircuit(circ, list(circ.params_name))
     for dtype_code, mode in ((_native.MQA_C128, 5), (_native.MQA_C64, 4)):
         assert _native.lib().mqa_circuit_jit_compile(nc.handle, 8, dtype_code, mode, None) == _native.MQA_ERR_ARG
+
+
+@pytest.mark.parametrize("mode", [9, 11])
+def test_single_buffer_back_sweep_compiles(mode):
+    """Single-buffer complex64 back-sweep kernels (exact 9 and phase-free 11)."""
+    circ = hea(14, 2, encoder=True)
+    circ += G.RY("c").on(2, 13)
+    circ += G.X.on(12, [0, 13])
+    compile_circuit(circ, 14, _native.MQA_C64, mode)
+
+
+def test_single_buffer_rejects_dense_blocks_and_forward_plans():
+    rng = np.random.default_rng(4)
+    m = np.linalg.qr(rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8)))[0]
+    circ = hea(8, 1) + G.UnivMathGate("u3", m).on([0, 1, 2])
+    nc = ir.NativeCircuit(circ, list(circ.params_name))
+    assert _native.lib().mqa_circuit_jit_compile(nc.handle, 8, _native.MQA_C64, 9, None) == _native.MQA_ERR_ARG
+    plain = hea(8, 1)
+    nc2 = ir.NativeCircuit(plain, list(plain.params_name))
+    assert _native.lib().mqa_circuit_jit_compile(nc2.handle, 8, _native.MQA_C64, 8, None) == _native.MQA_ERR_ARG
+
+
+def test_single_buffer_back_sweep_compiles_c128():
+    circ = hea(12, 2, encoder=True)
+    circ += G.RY("c").on(2, 11)
+    compile_circuit(circ, 12, _native.MQA_C128, 11)
