@@ -81,7 +81,7 @@ def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile, tune=Fals
     if tune:  # untimed: tile widths of the forward and back-sweep passes, chosen on this workload
         c64 = dtype == mq.complex64
         times = autotune.tune_grad(grad_ops, enc, ans, fwd_candidates=(12, 13) if c64 else (11, 12),
-                                   adj_candidates=(11, 12) if c64 else (10, 11))
+                                   adj_candidates=(11, 12) if c64 else (10, 11, 12))
         code = mq.dtype.native_code(dtype)
         nc = grad_ops.native_circuits[0]
         tiles = {"forward_back_sweep": [nc.tile_qubits(code, False), nc.tile_qubits(code, True)],

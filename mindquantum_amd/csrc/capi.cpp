@@ -597,14 +597,26 @@ const double* build_angles(mqa_sim* sim, mqa_circuit* circ, const double* enc_ho
     return table.as<double>();
 }
 
+// Tile plan of a Hamiltonian: tiles of the option's width, or one qubit narrower when that fits in
+// no more passes (half the LDS per block, so twice the resident blocks per CU).  ok == false: no
+// tile holds some flip group.
+mqa_ham::TilePlan* ham_plan_for(mqa_ham* ham, int n_eff, int dtype, hipStream_t s) {
+    // complex128 tiles above 13 qubits exceed the LDS of a CU
+    const int K = int(dtype == MQA_C64 ? g_opt.ham_tile_c64.load() : std::min<int64_t>(g_opt.ham_tile_c128, 13));
+    const int C = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
+    mqa_ham::TilePlan* wide = ham->tile_plan(n_eff, K, C, s);
+    if (K - 1 - C >= kGroupWidth && n_eff > K - 1) {
+        mqa_ham::TilePlan* narrow = ham->tile_plan(n_eff, K - 1, C, s);
+        if (narrow->ok && (!wide->ok || narrow->plan.passes.size() <= wide->plan.passes.size())) return narrow;
+    }
+    return wide;
+}
+
 // out = H ket over a batch with the tile engine; with `expect`, per-sample <ket|H|ket> in
 // sim->fsum.  Returns false (nothing launched) when the Hamiltonian has no tile plan.
 bool ham_apply_tiles(mqa_sim* sim, mqa_ham* ham, const void* ket, void* out, bool expect, uint64_t batch) {
     if (!g_opt.ham_engine) return false;
-    // complex128 tiles above 13 qubits exceed the LDS of a CU
-    const int K = int(sim->dtype == MQA_C64 ? g_opt.ham_tile_c64.load() : std::min<int64_t>(g_opt.ham_tile_c128, 13));
-    const int C = int(sim->dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
-    mqa_ham::TilePlan* tp = ham->tile_plan(sim->n_eff, K, C, sim->stream);
+    mqa_ham::TilePlan* tp = ham_plan_for(ham, sim->n_eff, sim->dtype, sim->stream);
     if (!tp->ok) return false;
     const HamTilePlan& pl = tp->plan;
     const uint64_t n_tiles = uint64_t(1) << (sim->n_eff - pl.K);
@@ -1723,10 +1735,8 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             it.dim = uint64_t(1) << it.n_eff;
             it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true);
             it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true);
-            const int K = int(dtype == MQA_C64 ? g_opt.ham_tile_c64.load() : std::min<int64_t>(g_opt.ham_tile_c128, 13));
-            const int C = int(dtype == MQA_C64 ? g_opt.lowq_c64 : g_opt.lowq_c128);
             if (g_opt.ham_engine) {
-                auto* tp = const_cast<mqa_ham*>(h)->tile_plan(it.n_eff, K, C, so.s);
+                auto* tp = ham_plan_for(const_cast<mqa_ham*>(h), it.n_eff, dtype, so.s);
                 it.hp = tp->ok ? tp : nullptr;
             }
             const uint64_t adj_tiles = uint64_t(1) << (it.n_eff - it.adj->plan.K);

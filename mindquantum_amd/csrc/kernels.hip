@@ -312,7 +312,8 @@ __device__ __forceinline__ void ham_tile_body(const HamTileArgs& args, unsigned 
             for (int m = 0; m < NA; m++) {  // W[m]: terms of register mask m, tile and thread signs
                 T sr = 0, si = 0;
                 const int end = grp.seg_end[m];
-                for (; t < end; t++) {
+#pragma unroll 4
+                for (; t < end; t++) {  // unrolled: four terms' scalar loads share one wait
                     const uint32_t neg =
                         (uint32_t(__popcll(base & tc[t].yz_nl)) + __popc(tsrc & tc[t].yzl)) & 1u;
                     sr += flip_sign(T(tc[t].coef_re), neg);
