@@ -197,6 +197,10 @@ __device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32
         T xr, xi, yr, yi;
         if constexpr (KIND == OPK_X) {
             xr = br, xi = bi, yr = ar, yi = ai;
+        } else if constexpr (KIND == OPK_H) {  // [[1, 1], [1, -1]] / sqrt(2)
+            constexpr T r = T(0.70710678118654752440084436210484903928);
+            xr = (ar + br) * r, xi = (ai + bi) * r;
+            yr = (ar - br) * r, yi = (ai - bi) * r;
         } else if constexpr (KIND == OPK_RX) {  // [[c, -is], [-is, c]]
             xr = c * ar + s * bi, xi = c * ai - s * br;
             yr = c * br + s * ai, yi = c * bi - s * ar;
@@ -431,6 +435,7 @@ __device__ __forceinline__ void dispatch_op(int kind, int q0, int q1, F&& f) {
     }
     switch (kind) {
         case OPK_X: MQA_D1(OPK_X) break;
+        case OPK_H: MQA_D1(OPK_H) break;
         case OPK_RX: MQA_D1(OPK_RX) break;
         case OPK_RY: MQA_D1(OPK_RY) break;
         case OPK_RZ: MQA_D1(OPK_RZ) break;

@@ -30,6 +30,7 @@ namespace {
 const char* kind_name(int k) {
     switch (k) {
         case OPK_X: return "OPK_X";
+        case OPK_H: return "OPK_H";
         case OPK_M1: return "OPK_M1";
         case OPK_SWAP: return "OPK_SWAP";
         case OPK_M2: return "OPK_M2";

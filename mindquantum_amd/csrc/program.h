@@ -33,6 +33,7 @@ enum OpKind : int32_t {
     OPK_M4 = 9,  // dense 16x16 over group positions 0..3 (4-qubit matrices)
     OPK_CX = 10, // X on q0 controlled by register position q1 (CNOT), plus any other controls
     OPK_CZ = 11, // Z on q0 controlled by register position q1 (CZ), plus any other controls
+    OPK_H = 12,  // Hadamard: (a + b, a - b) / sqrt(2), half the operations of a dense 2x2
     // parameterised: the angle comes from the per-sample angle table
     OPK_RX = 16,
     OPK_RY = 17,

@@ -59,6 +59,7 @@ static int lower_kind(const CGate& g, bool adjoint, std::vector<cplx>* mat) {
     mat->clear();
     switch (g.kind) {
         case MQA_GATE_X: return OPK_X;
+        case MQA_GATE_H: return OPK_H;  // self-inverse: the same op in the back-sweep
         case MQA_GATE_SWAP: return OPK_SWAP;
         case MQA_GATE_Y:
         case MQA_GATE_Z:
@@ -66,7 +67,6 @@ static int lower_kind(const CGate& g, bool adjoint, std::vector<cplx>* mat) {
         case MQA_GATE_SDAG:
         case MQA_GATE_T:
         case MQA_GATE_TDAG:
-        case MQA_GATE_H:
         case MQA_GATE_SX:
         case MQA_GATE_SXDAG:
         case MQA_GATE_ISWAP:
