@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -612,6 +613,40 @@ mqa_ham::TilePlan* ham_plan_for(mqa_ham* ham, int n_eff, int dtype, hipStream_t 
     return wide;
 }
 
+// (cos, sin) table of a circuit's angle slots for one parameter vector (the half angle for RX /
+// RY / RZ and the two-qubit rotations, the angle for PS / GP), as k_angles computes it.
+std::vector<double> host_angles(const mqa_circuit* c, const double* pv) {
+    std::vector<double> tab(2 * size_t(c->n_slots));
+    for (int sl = 0; sl < c->n_slots; sl++) {
+        double th = c->slot_const[size_t(sl)];
+        for (int k = c->slot_term_begin[size_t(sl)]; k < c->slot_term_begin[size_t(sl) + 1]; k++)
+            th += c->slot_term_coeff[size_t(k)] * pv[c->slot_term_param[size_t(k)]];
+        if (c->slot_half[size_t(sl)]) th *= 0.5;
+        tab[2 * size_t(sl)] = std::cos(th);
+        tab[2 * size_t(sl) + 1] = std::sin(th);
+    }
+    return tab;
+}
+
+// The global phase a phase-free plan of circuit c drops (exact state = phase x phase-free state),
+// from the angle table the kernels read: -1 for an uncontrolled RX / RY whose half-angle cosine
+// is negative in the state's precision, e^{-i theta/2} for an uncontrolled RZ, e^{-i theta} for an
+// uncontrolled GP (the OPK_PF_* conventions of program.h).
+std::complex<double> dropped_phase(const mqa_circuit* c, const std::vector<double>& tab, int dtype) {
+    std::complex<double> ph(1.0, 0.0);
+    for (const auto& g : c->gates) {
+        if (g.ctrl != 0 || g.slot < 0) continue;
+        const double cs = tab[2 * size_t(g.slot)], sn = tab[2 * size_t(g.slot) + 1];
+        if (g.kind == MQA_GATE_RX || g.kind == MQA_GATE_RY) {
+            const bool neg = dtype == MQA_C64 ? (float(cs) < 0.0f) : (cs < 0.0);
+            if (neg) ph = -ph;
+        } else if (g.kind == MQA_GATE_RZ || g.kind == MQA_GATE_GP) {
+            ph *= std::complex<double>(cs, -sn);
+        }
+    }
+    return ph;
+}
+
 // out = H ket over a batch with the tile engine; with `expect`, per-sample <ket|H|ket> in
 // sim->fsum.  Returns false (nothing launched) when the Hamiltonian has no tile plan.
 bool ham_apply_tiles(mqa_sim* sim, mqa_ham* ham, const void* ket, void* out, bool expect, uint64_t batch) {
@@ -1207,9 +1242,30 @@ int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double*
         auto* circ = const_cast<mqa_circuit*>(circ_c);
         if (circ->gates.empty()) return;
         DeviceGuard dg(sim->device);
-        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
-        const double* ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
+        // phase-free plan (uncontrolled RX / RY / RZ / GP drop their global phase), then the dropped
+        // phase restored by one scaling pass; the angle table is computed here, so the phase is
+        // decided on exactly the values the kernels read
+        const bool pf = g_opt.phase_free.load() != 0 && circ->n_slots > 0;
+        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true, pf);
+        const double* ang = nullptr;
+        std::vector<double> tab;
+        std::complex<double> phase(1.0, 0.0);
+        if (pf) {
+            tab = host_angles(circ, params);
+            phase = dropped_phase(circ, tab, sim->dtype);
+            sim->angles.ensure(tab.size() * sizeof(double));
+            hip_check(hipMemcpyAsync(sim->angles.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     sim->stream),
+                      "angle upload");
+            ang = sim->angles.as<double>();
+        } else {
+            ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
+        }
         run_plan(sim, dp, sim->state.p, nullptr, 1, ang, circ->n_slots, nullptr, 0);
+        if (phase != std::complex<double>(1.0, 0.0))
+            hip_check(launch_scale(sim->dtype, sim->state.p, sim->state.p, sim->dim_eff, phase.real(), phase.imag(),
+                                   sim->stream),
+                      "global phase");
         sync(sim, "apply_circuit");
     });
 }

@@ -101,3 +101,45 @@ def test_phase_free_matches_exact_plans(options):
         out.append(sim.get_expectation_with_grad(mq.Hamiltonian(op, mq.complex128), circ)(enc, ans))
     for a, b in zip(out[0], out[1]):
         assert np.max(np.abs(a - b)) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("compiled", [False, True])
+def test_apply_circuit_restores_the_dropped_phase(options, dtype, compiled):
+    """apply_circuit runs the phase-free plan and multiplies the dropped global phase back in:
+    the state (its phase included) matches the oracle and the exact plan, with half-angle cosines
+    of both signs, global-phase gates, and controlled rotations (which stay exact)."""
+    from tests.test_gpu_parity import assert_state, oracle_state
+
+    n = 11
+    rng = np.random.default_rng(12)
+    circ = mq.Circuit()
+    for q in range(n):
+        circ.h(q)
+    for k in range(60):
+        q = int(rng.integers(n))
+        kind = k % 5
+        th = float(rng.uniform(-3 * np.pi, 3 * np.pi))
+        if kind == 0:
+            circ += G.RX(th).on(q)
+        elif kind == 1:
+            circ += G.RY(f"p{k}").on(q)
+        elif kind == 2:
+            circ += G.RZ(th).on(q)
+        elif kind == 3:
+            circ += G.GlobalPhase(f"p{k}").on(q)
+        else:
+            circ += G.RY(th).on(q, (q + 1) % n)
+        if k % 7 == 0:
+            circ += G.X.on((q + 2) % n, q)
+    names = list(circ.params_name)
+    vals = dict(zip(names, rng.uniform(-3 * np.pi, 3 * np.pi, size=len(names))))
+    states = []
+    for pf in (True, False):
+        set_mode(compiled, pf)
+        sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
+        sim.apply_circuit(circ, vals)
+        states.append(sim.get_qs())
+    assert_state(states[0], oracle_state(circ, n, vals, dtype), dtype)
+    tol = 1e-12 if dtype == mq.complex128 else 2e-6
+    assert np.max(np.abs(states[0] - states[1])) < tol
