@@ -1687,7 +1687,7 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
     auto run_tile_steps = [&](const std::vector<std::vector<StepGroup<TilePassArgs>>>& steps, bool adjoint) {
         for (auto& v : steps)
             for (auto& g : v) {
-                const size_t lds = tile_pass_lds_bytes(dtype, adjoint, g.K, g.C, g.max_taps);
+                const size_t lds = interpreted_pass_lds_bytes(dtype, adjoint, g.full, g.K, g.C, g.max_taps);
                 hip_check(launch_tile_pass_items(dtype, adjoint, g.full, targs_d + g.args_at, prefix_d + g.prefix_at,
                                                  int(g.args.size()), g.blocks, g.K, lds, s),
                           "tile pass (items)");

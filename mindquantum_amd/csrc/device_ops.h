@@ -783,22 +783,24 @@ __device__ __forceinline__ void tile_end(const TilePassArgs& args, const DevPass
 // them out through ONE LDS tile buffer, one state after the other: a block then holds one tile of
 // LDS instead of two (36 instead of 66 KB for 12-qubit complex64 tiles), so twice as many blocks
 // share a CU, at the price of four barriers per group transition instead of one.
+// KC > 0: compile-time tile width (generated passes); 0: args.K and blockDim.x (interpreted).
 template <typename T, int KC>
-__device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const DevPass& pass,
-                                                 unsigned char* smem) {
+__device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const DevPass& pass, unsigned char* smem,
+                                                 uint64_t tile, uint64_t sample) {
     using V = V2<T>;
     TileFrame<T> f;
-    f.NT = 1u << (KC - kGroupWidth);
+    const int K = KC > 0 ? KC : args.K;
+    f.NT = KC > 0 ? (1u << (KC - kGroupWidth)) : blockDim.x;
     f.tid = threadIdx.x;
     f.C = pass.low_run;
-    const int H = KC - f.C;
-    const LdsLayout lay = lds_layout(KC, f.C, false, int(sizeof(V)), 0);
+    const int H = K - f.C;
+    const LdsLayout lay = lds_layout(K, f.C, false, int(sizeof(V)), 0);
     f.t_psi = reinterpret_cast<V*>(smem + lay.psi);
     f.t_lam = f.t_psi;
     f.offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
     f.lds_taps = reinterpret_cast<double*>(smem + lay.taps);
-    f.tile = blockIdx.x;
-    f.sample = blockIdx.y;
+    f.tile = tile;
+    f.sample = sample;
     f.base = pdep64(f.tile, pass.nonlocal_mask);
     for (uint32_t h = f.tid; h < (1u << H); h += f.NT) f.offtab[h] = pdep64(h, pass.local_hi_mask);
     __syncthreads();
@@ -808,6 +810,11 @@ __device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const
     f.n_waves = int((f.NT + 63) >> 6);
     f.wave_width = f.NT < 64 ? int(f.NT) : 64;
     return f;
+}
+
+template <typename T, int KC>
+__device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const DevPass& pass, unsigned char* smem) {
+    return sb_begin<T, KC>(args, pass, smem, blockIdx.x, blockIdx.y);
 }
 
 // One state's registers <-> the LDS tile buffer in a group's layout.

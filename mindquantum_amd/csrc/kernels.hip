@@ -28,10 +28,91 @@ namespace mqa {
 // ---------------------------------------------------------------- the tile pass
 // The interpreted pass: walks the group and op tables of any plan.  Circuits that run often
 // are also compiled into straight-line kernels of the same structure (jit.cpp).
+// Back-sweep without dense blocks: psi and lambda stay in registers between groups and share one
+// LDS tile buffer (the single-buffer form of the generated passes, device_ops.h), so twice as many
+// blocks fit a CU.
+template <typename T>
+__device__ __forceinline__ void tile_pass_adj_one_buffer(const TilePassArgs& args, const DevPass& pass,
+                                                         unsigned char* smem, uint64_t tile, uint64_t sample) {
+    TileFrame<T> f = sb_begin<T, 0>(args, pass, smem, tile, sample);
+    const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;
+    Regs<T> p, l;
+    DevGroup prev = args.groups[pass.group_begin];
+    {
+        const uint32_t a0 = swz<T>(group_ctx(f.tid, prev.pos, prev.spos).ub);
+        tile_load<T>(f.t_psi, f.g_psi, f.offtab, f.C, f.tid, f.NT);
+        __syncthreads();
+        group_load1<T>(f.t_psi, a0, prev.off, p);
+        __syncthreads();
+        tile_load<T>(f.t_psi, f.g_lam, f.offtab, f.C, f.tid, f.NT);
+        __syncthreads();
+        group_load1<T>(f.t_psi, a0, prev.off, l);
+    }
+    for (int gi = 0; gi < pass.n_groups; gi++) {
+        const DevGroup grp = args.groups[pass.group_begin + gi];
+        const GroupCtx g = group_ctx(f.tid, grp.pos, grp.spos);
+        if (gi > 0) {  // psi, then lambda, from the previous group's layout into this one's
+            const uint32_t ap = swz<T>(group_ctx(f.tid, prev.pos, prev.spos).ub), an = swz<T>(g.ub);
+            __syncthreads();
+            group_store1<T>(f.t_psi, ap, prev.off, p);
+            __syncthreads();
+            group_load1<T>(f.t_psi, an, grp.off, p);
+            __syncthreads();
+            group_store1<T>(f.t_psi, ap, prev.off, l);
+            __syncthreads();
+            group_load1<T>(f.t_psi, an, grp.off, l);
+        }
+        TapAcc<T> acc;
+#pragma unroll
+        for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0);
+        for (int oi = 0; oi < grp.n_ops; oi++) {
+            const DevOp op = args.ops[grp.op_begin + oi];
+            run_op<T, true, false>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc);
+        }
+        if (grp.n_taps > 0) {  // one batched cross-lane reduction per group
+            if (f.wave_width == 64) {
+#pragma unroll
+                for (int j = 0; j < kTapSlots<T>; j++) {
+                    if (j < 4 || grp.n_taps > 4) acc.r[j] = wave_sum64(acc.r[j]);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = wave_sum(acc.r[j], f.wave_width);
+            }
+            if ((f.tid & 63) == 0) {
+#pragma unroll
+                for (int j = 0; j < kTapSlots<T>; j++) {
+                    if (j < grp.n_taps) {  // acc[j] holds the group's (n_taps - 1 - j)-th tap
+                        const int t = grp.tap_begin + grp.n_taps - 1 - j;
+                        f.lds_taps[t * f.n_waves + f.wave] = double(acc.r[j]);
+                    }
+                }
+            }
+        }
+        prev = grp;
+    }
+    uint32_t tid_st = f.tid;
+    asm volatile("" : "+v"(tid_st));
+    const uint32_t ap = swz<T>(group_ctx(tid_st, prev.pos, prev.spos).ub);
+    __syncthreads();
+    group_store1<T>(f.t_psi, ap, prev.off, p);
+    __syncthreads();
+    tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
+    __syncthreads();
+    group_store1<T>(f.t_psi, ap, prev.off, l);
+    __syncthreads();
+    tile_store<T>(f.t_psi, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    taps_out<T>(args, pass, f);
+}
+
 template <typename T, bool ADJ, bool FULL>
 __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigned char* smem, uint64_t tile,
                                                uint64_t sample) {
     const DevPass pass = args.passes[args.pass_index];
+    if constexpr (ADJ && !FULL) {
+        tile_pass_adj_one_buffer<T>(args, pass, smem, tile, sample);
+        return;
+    }
     TileFrame<T> f = tile_begin<T, ADJ>(args, pass, smem, tile, sample);
     const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;
 
@@ -548,6 +629,9 @@ static constexpr size_t kLdsLimit = 160 * 1024;
 size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps) {
     return lds_layout(K, C, adj, dtype == 0 ? 8 : 16, max_taps).total;
 }
+size_t interpreted_pass_lds_bytes(int dtype, bool adj, bool full, int K, int C, int max_taps) {
+    return tile_pass_lds_bytes(dtype, adj && full, K, C, max_taps);  // one tile buffer unless dense blocks
+}
 
 template <typename T, bool ADJ, bool FULL>
 static hipError_t launch_tp(const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
@@ -572,7 +656,7 @@ static hipError_t launch_tp_t(bool adj, bool full, const TilePassArgs& a, uint64
 
 hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
                             int max_taps, hipStream_t s) {
-    size_t lds = tile_pass_lds_bytes(dtype, adj, a.K, C, max_taps);
+    size_t lds = interpreted_pass_lds_bytes(dtype, adj, full, a.K, C, max_taps);
     if (lds > kLdsLimit) return hipErrorInvalidValue;
     if (dtype == 0) return launch_tp_t<float>(adj, full, a, batch, lds, s);
     return launch_tp_t<double>(adj, full, a, batch, lds, s);

@@ -69,6 +69,9 @@ struct AngleArgs {
 };
 
 size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps);
+// LDS of an interpreted pass: back-sweeps without dense blocks share one tile buffer between
+// psi and lambda (two only when `full`).
+size_t interpreted_pass_lds_bytes(int dtype, bool adj, bool full, int K, int C, int max_taps);
 // full: the pass holds dense 4x4 / 8x8 / 16x16 blocks (a heavier instantiation)
 hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
                             int max_taps, hipStream_t s);
