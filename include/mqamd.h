@@ -99,7 +99,8 @@ const char* mqa_version(void);
 int mqa_device_count(void);
 /* Tuning knobs (value < 0 restores the default): "tile_qubits_c64", "tile_qubits_c128",
  * "adj_tile_qubits_c64", "adj_tile_qubits_c128", "group_width", "lowq_c64", "lowq_c128",
- * "max_ops_per_pass" (1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
+ * "max_ops_per_pass" (0, the default: the scheduler keeps the cheapest of a few caps under its
+ * pass cost model; 1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
  * MQAMD_HBM_BUDGET_BYTES; 0 = 85% of free memory), "profile", "ham_tile_qubits_c64",
  * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
  * the first launch), "jit_min_amps", "checkpoints", "phase_free" (1, the default: mqa_sim_grad_batch

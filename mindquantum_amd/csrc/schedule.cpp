@@ -251,33 +251,73 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     for (auto& g : gates)
         if (g.slot >= 0 && g.grad) pb.tap_id_of_slot[g.slot] = plan.n_tap_ids++;
     const uint64_t lowmask = (C >= 64) ? all : ((uint64_t(1) << C) - 1);
-    while (!remaining.empty()) {
-        uint64_t L = whole ? all : lowmask;
-        uint64_t blocked = 0;
-        std::vector<int> chosen, rest;
-        int taps = 0;
-        for (int gi : remaining) {
-            const CGate& g = gates[gi];
-            uint64_t q = g.qubits();
-            bool needs_tap = adjoint && g.grad && g.slot >= 0;
-            bool full = (opts.max_ops_per_pass > 0 && int(chosen.size()) >= opts.max_ops_per_pass) ||
-                        (needs_tap && taps >= kMaxTapsPerPass);
-            if (full || (q & blocked)) {
-                blocked |= q;
-                rest.push_back(gi);
-                continue;
+    // Greedy partition into passes: each pass takes, in order, every gate whose targets still fit
+    // its local set and that no deferred gate blocks; cap > 0 closes a pass after cap gates.
+    struct Part {
+        std::vector<int> chosen;
+        uint64_t L;
+    };
+    auto partition = [&](int cap) {
+        std::vector<Part> parts;
+        std::vector<int> todo = remaining;
+        while (!todo.empty()) {
+            uint64_t L = whole ? all : lowmask;
+            uint64_t blocked = 0;
+            std::vector<int> chosen, rest;
+            int taps = 0;
+            for (int gi : todo) {
+                const CGate& g = gates[gi];
+                uint64_t q = g.qubits();
+                bool needs_tap = adjoint && g.grad && g.slot >= 0;
+                bool full = (cap > 0 && int(chosen.size()) >= cap) || (needs_tap && taps >= kMaxTapsPerPass);
+                if (full || (q & blocked)) {
+                    blocked |= q;
+                    rest.push_back(gi);
+                    continue;
+                }
+                uint64_t nl = L | g.targets();
+                if (popc(nl) <= K) {
+                    L = nl;
+                    chosen.push_back(gi);
+                    taps += needs_tap ? 1 : 0;
+                } else {
+                    blocked |= q;
+                    rest.push_back(gi);
+                }
             }
-            uint64_t nl = L | g.targets();
-            if (popc(nl) <= K) {
-                L = nl;
-                chosen.push_back(gi);
-                taps += needs_tap ? 1 : 0;
-            } else {
-                blocked |= q;
-                rest.push_back(gi);
-            }
+            if (chosen.empty()) throw std::runtime_error("scheduler made no progress");
+            parts.push_back(Part{std::move(chosen), L});
+            todo = std::move(rest);
         }
-        if (chosen.empty()) throw std::runtime_error("scheduler made no progress");
+        return parts;
+    };
+    // Model of a pass: max(1, work / B) streams of the state, work = parameterised ops + 0.3 per
+    // other op (register renames and fixed matrices), B = the work at which a pass's arithmetic
+    // takes as long as its streaming (measured on MI355X: about 32 for a complex64 back-sweep,
+    // 48 for a forward sweep; twice that for complex128, which streams twice the bytes).  Greedy
+    // packing can leave one arithmetic-bound pass and stragglers; without an explicit cap, the
+    // partition of lowest modelled time among a few caps is kept.
+    auto cost = [&](const std::vector<Part>& parts) {
+        const double B = (adjoint ? 32.0 : 48.0) * (opts.amp_bytes > 8 ? 2.0 : 1.0);
+        double t = 0;
+        for (const auto& p : parts) {
+            double w = 0;
+            for (int gi : p.chosen) w += gates[gi].slot >= 0 ? 1.0 : 0.3;
+            t += std::max(1.0, w / B);
+        }
+        return t;
+    };
+    std::vector<Part> parts = partition(opts.max_ops_per_pass);
+    if (opts.max_ops_per_pass == 0) {
+        double best = cost(parts);
+        for (int cap : {96, 80, 64, 50, 40}) {
+            auto alt = partition(cap);
+            const double c = cost(alt);
+            if (c < best - 1e-9) best = c, parts = std::move(alt);
+        }
+    }
+    for (auto& part : parts) {
+        uint64_t L = part.L;
         for (int q = 0; popc(L) < K && q < n_eff; q++) L |= uint64_t(1) << q;
         HostPass pass;
         std::vector<int> pos_of(64, -1);
@@ -286,9 +326,8 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
                 pos_of[q] = int(pass.local_qubits.size());
                 pass.local_qubits.push_back(q);
             }
-        pb.groups(&pass, chosen, pos_of, all & ~L, &plan.pool);
+        pb.groups(&pass, part.chosen, pos_of, all & ~L, &plan.pool);
         plan.passes.push_back(std::move(pass));
-        remaining = rest;
     }
 
     // flatten

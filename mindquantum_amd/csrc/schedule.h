@@ -29,7 +29,7 @@ struct SchedOptions {
     int tile_qubits = 12;      // K: amplitudes per tile = 2^K
     int low_run = 5;           // C: lowest qubits always local (contiguous runs of 2^C amplitudes)
     int group_width = 4;       // max target positions per register group (1..4)
-    int max_ops_per_pass = 0;  // 0: unlimited; 1 gives the unfused one-gate-per-pass schedule
+    int max_ops_per_pass = 0;  // 0: chosen by the pass cost model (build_plan); 1: one gate per pass
     int taps_per_group = 8;    // derivative taps a register group may hold (<= kMaxTapsPerGroup)
     int amp_bytes = 8;         // 8 (complex64) or 16 (complex128): selects the LDS swizzle
     bool phase_free = false;   // uncontrolled RX / RY / RZ / GP lower to their OPK_PF_* forms
