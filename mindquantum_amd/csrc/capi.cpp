@@ -630,8 +630,8 @@ std::vector<double> host_angles(const mqa_circuit* c, const double* pv) {
 
 // The global phase a phase-free plan of circuit c drops (exact state = phase x phase-free state),
 // from the angle table the kernels read: -1 for an uncontrolled RX / RY whose half-angle cosine
-// is negative in the state's precision, e^{-i theta/2} for an uncontrolled RZ, e^{-i theta} for an
-// uncontrolled GP (the OPK_PF_* conventions of program.h).
+// is negative in the state's precision, e^{-i theta/2} for an uncontrolled RZ or RZZ, e^{-i theta}
+// for an uncontrolled GP (the OPK_PF_* conventions of program.h).
 std::complex<double> dropped_phase(const mqa_circuit* c, const std::vector<double>& tab, int dtype) {
     std::complex<double> ph(1.0, 0.0);
     for (const auto& g : c->gates) {
@@ -640,7 +640,7 @@ std::complex<double> dropped_phase(const mqa_circuit* c, const std::vector<doubl
         if (g.kind == MQA_GATE_RX || g.kind == MQA_GATE_RY) {
             const bool neg = dtype == MQA_C64 ? (float(cs) < 0.0f) : (cs < 0.0);
             if (neg) ph = -ph;
-        } else if (g.kind == MQA_GATE_RZ || g.kind == MQA_GATE_GP) {
+        } else if (g.kind == MQA_GATE_RZ || g.kind == MQA_GATE_RZZ || g.kind == MQA_GATE_GP) {
             ph *= std::complex<double>(cs, -sn);
         }
     }
@@ -1789,8 +1789,10 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             it.index = i;
             it.n_eff = n_eff_of(n);
             it.dim = uint64_t(1) << it.n_eff;
-            it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true);
-            it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true);
+            // Pauli observables (the only kind items take) are hermitian: phase-free plans
+            const bool pf = g_opt.phase_free.load() != 0;
+            it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true, pf);
+            it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true, pf);
             if (g_opt.ham_engine) {
                 auto* tp = ham_plan_for(const_cast<mqa_ham*>(h), it.n_eff, dtype, so.s);
                 it.hp = tp->ok ? tp : nullptr;

@@ -322,6 +322,10 @@ __device__ __forceinline__ void two_target(Regs<T>& a, const GroupCtx& g, uint32
             T r3 = c * a.r[V3] + s * a.i[V3], i3 = c * a.i[V3] - s * a.r[V3];
             a.r[V0] = r0, a.i[V0] = i0, a.r[V1] = r1, a.i[V1] = i1;
             a.r[V2_] = r2, a.i[V2_] = i2, a.r[V3] = r3, a.i[V3] = i3;
+        } else if constexpr (KIND == OPK_PF_RZZ) {  // diag(1, e, e, 1), e = c + is (full angle)
+            T r1 = c * a.r[V1] - s * a.i[V1], i1 = c * a.i[V1] + s * a.r[V1];
+            T r2 = c * a.r[V2_] - s * a.i[V2_], i2 = c * a.i[V2_] + s * a.r[V2_];
+            a.r[V1] = r1, a.i[V1] = i1, a.r[V2_] = r2, a.i[V2_] = i2;
         } else if constexpr (KIND == OPK_RXX || KIND == OPK_RYY) {
             // RXX = c I - is XX; RYY = c I - is YY with YY|00> = -|11>, YY|01> = |10>
             constexpr int sg = (KIND == OPK_RXX) ? 1 : -1;
@@ -361,7 +365,10 @@ __device__ __forceinline__ T two_target_tap(const Regs<T>& p, const Regs<T>& l, 
         if constexpr (CTRL) {
             if (!ctrl_ok<V0>(g, cl)) return;
         }
-        if constexpr (KIND == OPK_RZZ) {
+        if constexpr (KIND == OPK_PF_RZZ) {  // -Im sum over the odd-parity pair (negated below)
+            cdot_part<true>(l.r[V1], l.i[V1], p.r[V1], p.i[V1], s);
+            cdot_part<true>(l.r[V2_], l.i[V2_], p.r[V2_], p.i[V2_], s);
+        } else if constexpr (KIND == OPK_RZZ) {
             cdot_part<true>(l.r[V0], l.i[V0], p.r[V0], p.i[V0], s);
             cdot_part<true>(l.r[V1], l.i[V1], -p.r[V1], -p.i[V1], s);
             cdot_part<true>(l.r[V2_], l.i[V2_], -p.r[V2_], -p.i[V2_], s);
@@ -374,7 +381,7 @@ __device__ __forceinline__ T two_target_tap(const Regs<T>& p, const Regs<T>& l, 
             cdot_part<true>(l.r[V2_], l.i[V2_], p.r[V1], p.i[V1], s);
         }
     });
-    return T(0.5) * s;
+    return KIND == OPK_PF_RZZ ? -s : T(0.5) * s;
 }
 
 // Dense 2^M x 2^M (M = 3, 4) over register bits 0..M-1 (group positions in the gate's order).
@@ -446,6 +453,7 @@ __device__ __forceinline__ void dispatch_op(int kind, int q0, int q1, F&& f) {
         case OPK_CZ: MQA_D2(OPK_CZ) break;
         case OPK_SWAP: MQA_D2(OPK_SWAP) break;
         case OPK_RZZ: MQA_D2(OPK_RZZ) break;
+        case OPK_PF_RZZ: MQA_D2(OPK_PF_RZZ) break;
         case OPK_RXX: MQA_D2(OPK_RXX) break;
         case OPK_RYY: MQA_D2(OPK_RYY) break;
         case OPK_M2:
@@ -458,7 +466,7 @@ __device__ __forceinline__ void dispatch_op(int kind, int q0, int q1, F&& f) {
 }
 
 template <int KIND>
-constexpr bool kTwoTarget = KIND == OPK_CX || KIND == OPK_CZ || KIND == OPK_SWAP || KIND == OPK_RZZ ||
+constexpr bool kTwoTarget = KIND == OPK_CX || KIND == OPK_CZ || KIND == OPK_SWAP || KIND == OPK_RZZ || KIND == OPK_PF_RZZ ||
                             KIND == OPK_RXX || KIND == OPK_RYY || KIND == OPK_M2;
 
 // G (forward) or G^dagger (back-sweep, angle already negated) on one register set.
@@ -492,11 +500,11 @@ __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, 
             s = T(ang_row[2 * op.slot + 1]);
             if (ADJ) s = -s;  // G^dagger = G(-angle)
         }
-        if (kind == OPK_PF_RZ) {
+        if (kind == OPK_PF_RZ || kind == OPK_PF_RZZ) {  // full angle from the half angle
             const T c2 = c * c - s * s;
             s = T(2) * c * s;
             c = c2;
-            kind = OPK_PS;
+            if (kind == OPK_PF_RZ) kind = OPK_PS;
         } else if (kind == OPK_PF_RX || kind == OPK_PF_RY) {
             if (c < T(0)) c = -c, s = -s;
             kind = kind == OPK_PF_RX ? OPK_RX : OPK_RY;

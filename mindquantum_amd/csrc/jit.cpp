@@ -44,13 +44,14 @@ const char* kind_name(int k) {
         case OPK_RXX: return "OPK_RXX";
         case OPK_RYY: return "OPK_RYY";
         case OPK_RZZ: return "OPK_RZZ";
+        case OPK_PF_RZZ: return "OPK_PF_RZZ";
         default: return nullptr;
     }
 }
 
 bool two_target_kind(int k) {
     return k == OPK_CX || k == OPK_CZ || k == OPK_SWAP || k == OPK_RZZ || k == OPK_RXX || k == OPK_RYY ||
-           k == OPK_M2;
+           k == OPK_M2 || k == OPK_PF_RZZ;
 }
 
 // The compile helper sits next to this library (mqamd_jitc); empty if it is not there.
@@ -88,7 +89,15 @@ void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, std::vect
     }
     b << "      T c = T(ang_row[" << 2 * op.slot << "]);\n"
       << "      T s = " << (adj ? "-" : "") << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
-    if (op.kind == OPK_PF_RZ) {
+    if (op.kind == OPK_PF_RZZ) {
+        const int q1 = int(op.q1);
+        b << "      double_angle(c, s);\n";
+        if (var >= 0)
+            b << "      tap" << var << " = two_target_tap<T, OPK_PF_RZZ, " << q << ", " << q1
+              << ", false>(p, l, g, 0u);\n";
+        b << "      apply_body<T, OPK_PF_RZZ, " << q << ", " << q1 << ", false>(p, g, 0u, c, s, pool);\n";
+        if (adj) b << "      apply_body<T, OPK_PF_RZZ, " << q << ", " << q1 << ", false>(l, g, 0u, c, s, pool);\n";
+    } else if (op.kind == OPK_PF_RZ) {
         b << "      double_angle(c, s);\n";
         if (var >= 0) b << "      tap" << var << " = one_target_tap<T, OPK_PS, " << q << ", false>(p, l, g, 0u);\n";
         b << "      apply_body<T, OPK_PS, " << q << ", 0, false>(p, g, 0u, c, s, pool);\n";

@@ -50,19 +50,24 @@ enum OpKind : int32_t {
     //   PF_RX, PF_RY: RX / RY with (cos, sin) of the half angle negated when cos < 0 (a factor
     //                 -1), so the rotation runs as three shears with |tan| <= 1;
     //   PF_RZ:        diag(1, e^{i theta}) = e^{i theta/2} RZ(theta), a phase on one amplitude;
-    //   PF_GP:        the identity.
-    // Back-sweep taps: PF_RX / PF_RY as RX / RY; PF_RZ and PF_GP rely on <lambda|psi> being
-    // real (lambda = H psi with H hermitian), so Re(tap) of RZ is -Im(sum over |1> of
-    // conj(lambda) psi), the PS tap, and that of GP is zero.
+    //   PF_GP:        the identity;
+    //   PF_RZZ:       diag(1, e^{i theta}, e^{i theta}, 1) = e^{i theta/2} RZZ(theta), a phase on
+    //                 the odd-parity pair.
+    // Back-sweep taps: PF_RX / PF_RY as RX / RY; PF_RZ, PF_RZZ and PF_GP rely on <lambda|psi>
+    // being real (lambda = H psi with H hermitian), so Re(tap) of RZ is -Im(sum over |1> of
+    // conj(lambda) psi) (the PS tap), that of RZZ the same sum over the odd-parity pair, and that
+    // of GP zero.
     OPK_PF_RX = 24,
     OPK_PF_RY = 25,
     OPK_PF_RZ = 26,
     OPK_PF_GP = 27,
+    OPK_PF_RZZ = 28,
 };
 
 inline bool opk_is_param(int k) { return k >= OPK_RX; }
 inline bool opk_is_two(int k) {
-    return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ || k == OPK_CX || k == OPK_CZ;
+    return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ || k == OPK_CX || k == OPK_CZ ||
+           k == OPK_PF_RZZ;
 }
 
 // Byte layout of a tile pass's dynamic LDS: [psi tile][lambda tile (adjoint)][offset table]

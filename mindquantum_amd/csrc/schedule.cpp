@@ -173,6 +173,7 @@ struct PassBuilder {
                     else if (kind == OPK_RY) kind = OPK_PF_RY;
                     else if (kind == OPK_RZ) kind = OPK_PF_RZ;
                     else if (kind == OPK_GP) kind = OPK_PF_GP;
+                    else if (kind == OPK_RZZ) kind = OPK_PF_RZZ;
                 }
                 d.kind = kind;
                 d.slot = opk_is_param(kind) ? g.slot : -1;

@@ -16,6 +16,25 @@ from ..core.operators import Hamiltonian
 from ..ir import NativeCircuit
 
 
+# Native circuits (and with them their schedules, device plans and compiled passes) of the
+# circuits a caller passes again, keyed like the simulator's own cache: a circuit object whose
+# length and parameter names are unchanged maps to the same native circuit.
+_NATIVE_CACHE = {}
+_NATIVE_CACHE_MAX = 4096
+
+
+def _native_circuit(circ, names):
+    key = (id(circ), len(circ), tuple(names))
+    hit = _NATIVE_CACHE.get(key)
+    if hit is not None and hit[0] is circ:
+        return hit[1]
+    nc = NativeCircuit(circ, names)
+    if len(_NATIVE_CACHE) >= _NATIVE_CACHE_MAX:
+        _NATIVE_CACHE.clear()
+    _NATIVE_CACHE[key] = (circ, nc)
+    return nc
+
+
 def expectation_with_grad_batch(circuits, hamiltonians, params, n_qubits=None, dtype=None):
     """Return ``(f, grads)`` for ``len(circuits)`` independent items.
 
@@ -45,7 +64,7 @@ def expectation_with_grad_batch(circuits, hamiltonians, params, n_qubits=None, d
         p = np.asarray(p, dtype=np.float64).reshape(-1)
         if p.size != len(names):
             raise ValueError(f"circuit needs {len(names)} parameters, got {p.size}")
-        natives.append(NativeCircuit(c, names))
+        natives.append(_native_circuit(c, names))
         hams.append(h if isinstance(h, Hamiltonian) else Hamiltonian(h, dt))
         flat.append(p)
         offsets.append(offsets[-1] + p.size)

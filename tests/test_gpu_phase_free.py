@@ -143,3 +143,40 @@ def test_apply_circuit_restores_the_dropped_phase(options, dtype, compiled):
     assert_state(states[0], oracle_state(circ, n, vals, dtype), dtype)
     tol = 1e-12 if dtype == mq.complex128 else 2e-6
     assert np.max(np.abs(states[0] - states[1])) < tol
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("compiled", [False, True])
+def test_rzz_phase_free_gradient_and_state(options, dtype, compiled):
+    """Uncontrolled Rzz runs as a phase on the odd-parity pair: gradients (with checkpoints) and
+    apply_circuit states (dropped phase restored) match the oracle; controlled Rzz stays exact."""
+    from tests.test_gpu_parity import assert_state, oracle_state
+
+    n = 12
+    rng = np.random.default_rng(21)
+    circ = mq.Circuit()
+    for q in range(n):
+        circ += G.RX(f"x{q}").on(q)
+    for k in range(40):
+        a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+        if k % 6 == 5:
+            circ += G.Rzz(f"z{k}").on([a, b], int((max(a, b) + 1) % n) if (max(a, b) + 1) % n not in (a, b) else None)
+        else:
+            circ += G.Rzz(f"z{k}").on([a, b])
+        circ += G.RX(f"m{k}").on(int(rng.integers(n)))
+    circ = narrow(circ, dtype, 8)
+    op = random_hamiltonian(n, 10, 3)
+    names = list(circ.params_name)
+    vals = rng.uniform(-2 * np.pi, 2 * np.pi, size=len(names))
+    set_mode(compiled, True)
+    _native.set_option("checkpoints", 2)
+    sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
+    f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)(vals)
+    assert _native.get_option("phase_free_used") == 1
+    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals, dtype)
+    assert abs(f[0, 0] - f_ref[0, 0]) < (1e-10 if dtype == mq.complex128 else 5e-5)
+    assert_grad(g[0, 0].real, g_ref[0, 0], dtype)
+    sim2 = mq.Simulator("mqvector_amd", n, dtype=dtype)
+    pr = dict(zip(names, vals))
+    sim2.apply_circuit(circ, pr)
+    assert_state(sim2.get_qs(), oracle_state(circ, n, pr, dtype), dtype)
