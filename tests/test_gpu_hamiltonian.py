@@ -100,6 +100,19 @@ def test_flip_wider_than_a_tile_falls_back(dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_wide_tile_kept_when_a_flip_needs_it(dtype):
+    """A flip over 9 qubits above the low run fits the default tile width (14 / 13 qubits for
+    complex64 / complex128) but not one qubit less: the plan keeps the wider tiles.  The ring terms
+    beside it (which alone would take the narrower tiles, test_apply_hamiltonian_ring_multi_pass)
+    share the plan."""
+    n = 20
+    low = 5 if dtype == mq.complex64 else 4
+    wide = mq.QubitOperator(" ".join(f"X{q}" for q in range(low, low + 9)), 0.6)
+    op = wide + mq.QubitOperator(f"Y{low + 2} Z{low + 8}", -0.3) + ring_hamiltonian(n, 13)
+    check_apply(n, op, dtype, 13)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_tile_engine_matches_gather_engine(dtype):
     n = 17
     op = random_hamiltonian(n, 30, 5, max_weight=5) + ring_hamiltonian(n, 5)
