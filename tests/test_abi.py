@@ -105,6 +105,27 @@ def test_unfused_schedule_option():
     assert len(sched) == len(circ)
 
 
+def test_cost_model_partition_splits_the_heavy_pass():
+    """Without an explicit cap the scheduler keeps the cheapest of a few capped partitions: on the
+    20-qubit encoder + HEA-8 gradient circuit it splits the greedy schedule's first pass (all
+    layers on the low qubits) without adding passes."""
+    from mindquantum_amd.algorithm import hardware_efficient_ansatz
+    from mindquantum_amd.algorithm.builders import encoder_rx
+
+    circ = encoder_rx(20) + hardware_efficient_ansatz(20, 8)
+    names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
+    for adjoint in (False, True):
+        auto = ir.NativeCircuit(circ, names).schedule(20, 0, adjoint)
+        _native.set_option("max_ops_per_pass", 100000)  # an explicit cap that never binds: greedy
+        try:
+            greedy = ir.NativeCircuit(circ, names).schedule(20, 0, adjoint)
+        finally:
+            _native.set_option("max_ops_per_pass", -1)
+        _check_schedule(circ, auto, 20, 12, adjoint)
+        assert len(auto) <= len(greedy)
+        assert max(len(ops) for _, ops in auto) < max(len(ops) for _, ops in greedy)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_scheduled_order_gives_the_same_state(seed):
     """Apply the gates in the scheduler's order with the dense simulation: same state."""
