@@ -161,6 +161,8 @@ int current_device() {
 struct DevPlan {
     Plan plan;
     DevBuf passes, groups, ops, pool, tap_ids;
+    DevBuf coef_forms;  // CoefForm per angle slot (the compiled passes' coefficient table)
+    int n_coef_forms = 0;
     bool full = false;
     std::mutex jit_mu;
     std::unique_ptr<JitPlan> jit;          // compiled passes (created on first large launch)
@@ -284,6 +286,9 @@ struct mqa_circuit {
             mqa::upload(dp->groups, dp->plan.dev_groups, s);
             mqa::upload(dp->ops, dp->plan.dev_ops, s);
             mqa::upload(dp->tap_ids, dp->plan.dev_tap_slots, s);
+            const std::vector<uint8_t> forms = mqa::jit_coef_forms(dp->plan);  // alive until the sync
+            mqa::upload(dp->coef_forms, forms, s);
+            dp->n_coef_forms = int(forms.size());
             std::vector<float> pf;  // kept alive until the copy has completed
             if (dtype == MQA_C64) {
                 pf.assign(dp->plan.pool.begin(), dp->plan.pool.end());
@@ -458,6 +463,7 @@ struct mqa_sim {
     mqa::DevBuf state;
     // workspaces (grow-only)
     mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
+    mqa::DevBuf coefs;  // coefficient table of the compiled passes (rebuilt per run_plan call)
     mqa::Timer timer;
     size_t amp_bytes() const { return dtype == MQA_C64 ? 8 : 16; }
     size_t state_bytes() const { return dim_eff * amp_bytes(); }
@@ -544,6 +550,15 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, form);
         if (g_opt.jit == 2) slot->wait();
         if (slot->ready()) jp = slot.get();  // until then (or if it failed): the interpreted kernel
+    }
+    // compiled passes read their coefficients ready-made: the table follows this call's angles
+    if (jp && angles && n_slots > 0 && dp->n_coef_forms > 0) {
+        const size_t real = sim->dtype == MQA_C64 ? sizeof(float) : sizeof(double);
+        sim->coefs.ensure(size_t(batch) * size_t(n_slots) * 2 * real);
+        hip_check(launch_coefs(sim->dtype, angles, dp->coef_forms.as<uint8_t>(), dp->n_coef_forms, n_slots, batch,
+                               sim->coefs.p, sim->stream),
+                  "coefficient table");
+        a.coefs = sim->coefs.p;
     }
     // a single-buffer pass holds one tile of LDS (the forward layout) plus the tap table
     const bool one_tile = jp && jp->form() == JitForm::kSingleBuffer;

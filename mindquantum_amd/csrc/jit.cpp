@@ -76,7 +76,10 @@ bool read_file(const std::string& path, std::string& out) {
 // Body of a phase-free op (OPK_PF_*; see program.h), closing the op's brace.  Taps are taken
 // before the op, as for every kind: PF_RX / PF_RY with the RX / RY tap, PF_RZ with the PS tap of
 // the full angle, PF_GP with a zero tap (its variable keeps its initial zero).
-void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, std::vector<std::pair<int, int>>& taps) {
+// With `ready` (the slot's CoefForm entry exists), the coefficients come from the coefficient
+// table instead of being derived from (cos, sin) in every lane.
+void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, bool ready,
+                     std::vector<std::pair<int, int>>& taps) {
     const int q = int(op.q0);
     int var = -1;
     if (adj && op.tap >= 0) {
@@ -87,18 +90,24 @@ void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, std::vect
         b << "    }\n";
         return;
     }
-    b << "      T c = T(ang_row[" << 2 * op.slot << "]);\n"
-      << "      T s = " << (adj ? "-" : "") << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
+    const char* neg = adj ? "-" : "";
+    if (ready && op.kind != OPK_PF_RX && op.kind != OPK_PF_RY)  // COEF_DOUBLE
+        b << "      const T c = coef_row[" << 2 * op.slot << "];\n"
+          << "      const T s = " << neg << "coef_row[" << 2 * op.slot + 1 << "];\n";
+    else if (!ready)
+        b << "      T c = T(ang_row[" << 2 * op.slot << "]);\n"
+          << "      T s = " << neg << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
+    const char* dbl = ready ? "" : "      double_angle(c, s);\n";
     if (op.kind == OPK_PF_RZZ) {
         const int q1 = int(op.q1);
-        b << "      double_angle(c, s);\n";
+        b << dbl;
         if (var >= 0)
             b << "      tap" << var << " = two_target_tap<T, OPK_PF_RZZ, " << q << ", " << q1
               << ", false>(p, l, g, 0u);\n";
         b << "      apply_body<T, OPK_PF_RZZ, " << q << ", " << q1 << ", false>(p, g, 0u, c, s, pool);\n";
         if (adj) b << "      apply_body<T, OPK_PF_RZZ, " << q << ", " << q1 << ", false>(l, g, 0u, c, s, pool);\n";
     } else if (op.kind == OPK_PF_RZ) {
-        b << "      double_angle(c, s);\n";
+        b << dbl;
         if (var >= 0) b << "      tap" << var << " = one_target_tap<T, OPK_PS, " << q << ", false>(p, l, g, 0u);\n";
         b << "      apply_body<T, OPK_PS, " << q << ", 0, false>(p, g, 0u, c, s, pool);\n";
         if (adj) b << "      apply_body<T, OPK_PS, " << q << ", 0, false>(l, g, 0u, c, s, pool);\n";
@@ -107,14 +116,36 @@ void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, std::vect
         const char* pf = op.kind == OPK_PF_RX ? "OPK_PF_RX" : "OPK_PF_RY";
         if (var >= 0)
             b << "      tap" << var << " = one_target_tap<T, " << base << ", " << q << ", false>(p, l, g, 0u);\n";
-        b << "      const Shear<T> h = shear_of(c, s);\n"
-          << "      pf_rotation<T, " << pf << ", " << q << ">(p, h);\n";
+        if (ready)  // COEF_SHEAR
+            b << "      const Shear<T> h{" << neg << "coef_row[" << 2 * op.slot << "], " << neg << "coef_row["
+              << 2 * op.slot + 1 << "]};\n";
+        else
+            b << "      const Shear<T> h = shear_of(c, s);\n";
+        b << "      pf_rotation<T, " << pf << ", " << q << ">(p, h);\n";
         if (adj) b << "      pf_rotation<T, " << pf << ", " << q << ">(l, h);\n";
     }
     b << "    }\n";
 }
 
 }  // namespace
+
+std::vector<uint8_t> jit_coef_forms(const Plan& pl) {
+    std::vector<uint8_t> forms;
+    std::vector<bool> clash;
+    for (const DevOp& op : pl.dev_ops) {
+        if (!opk_is_param(op.kind) || op.slot < 0 || op.kind == OPK_PF_GP) continue;
+        const uint8_t f = op.kind == OPK_PF_RX || op.kind == OPK_PF_RY     ? COEF_SHEAR
+                          : op.kind == OPK_PF_RZ || op.kind == OPK_PF_RZZ ? COEF_DOUBLE
+                                                                          : COEF_CS;
+        const size_t k = size_t(op.slot);
+        if (k >= forms.size()) forms.resize(k + 1, COEF_NONE), clash.resize(k + 1, false);
+        if (forms[k] != COEF_NONE && forms[k] != f) clash[k] = true;
+        forms[k] = f;
+    }
+    for (size_t k = 0; k < forms.size(); k++)
+        if (clash[k]) forms[k] = COEF_NONE;
+    return forms;
+}
 
 const char* jit_kernel_name(bool adjoint) { return adjoint ? "mqa_jit_tile_pass_adj" : "mqa_jit_tile_pass_fwd"; }
 
@@ -173,6 +204,14 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
     o << "  const_ptr<double> ang_row = args.angles ? as_const(args.angles + 2 * f.sample * uint64_t(args.n_slots)) "
          ": nullptr;\n"
       << "  (void)ang_row;\n";
+    const std::vector<uint8_t> forms = jit_coef_forms(pl);
+    auto ready = [&](const DevOp& op) {
+        return op.slot >= 0 && size_t(op.slot) < forms.size() && forms[size_t(op.slot)] != COEF_NONE;
+    };
+    if (!forms.empty())
+        o << "  const_ptr<T> coef_row = as_const(static_cast<const T*>(args.coefs) + 2 * f.sample * "
+             "uint64_t(args.n_slots));\n"
+          << "  (void)coef_row;\n";
     // single buffer: every group's layout at function scope; psi, then lambda, into group 0's layout
     auto sb_move = [&](const char* what, int gi, const char* reg) {
         o << "  group_" << what << "1<T>(f.t_psi, swz<T>(group_ctx(f.tid, pos_" << gi << ", spos_" << gi
@@ -253,14 +292,17 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
                 continue;
             }
             if (op.kind >= OPK_PF_RX) {  // phase-free forms: never controlled
-                emit_phase_free(b, op, adj, taps);
+                emit_phase_free(b, op, adj, ready(op), taps);
                 continue;
             }
             const char* kn = kind_name(op.kind);
             if (!kn) throw std::invalid_argument("code generation: unknown op kind " + std::to_string(op.kind));
             const bool two = two_target_kind(op.kind);
             const int q1 = two ? int(op.q1) : 0;
-            if (opk_is_param(op.kind)) {
+            if (opk_is_param(op.kind) && ready(op)) {  // COEF_CS
+                b << "      const T c = coef_row[" << 2 * op.slot << "];\n"
+                  << "      const T s = " << (adj ? "-" : "") << "coef_row[" << 2 * op.slot + 1 << "];\n";
+            } else if (opk_is_param(op.kind)) {
                 b << "      const T c = T(ang_row[" << 2 * op.slot << "]);\n"
                   << "      const T s = " << (adj ? "-" : "") << "T(ang_row[" << 2 * op.slot + 1 << "]);\n";
             } else {

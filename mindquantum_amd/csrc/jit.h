@@ -34,6 +34,10 @@ const char* jit_kernel_name(bool adjoint);
 enum class JitForm { kTile = 0, kPersistent = 1, kSingleBuffer = 2 };
 std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile);
 
+// CoefForm of every angle slot a plan's ops read (index = slot; program.h): the layout of the
+// coefficient table its compiled passes take.  Empty when no op has an angle.
+std::vector<uint8_t> jit_coef_forms(const Plan& pl);
+
 // Whether a plan's compiled passes may take a form.
 inline bool jit_can_persist(const Plan& pl, int dtype) { return pl.adjoint && dtype == 0; }
 inline bool jit_can_single_buffer(const Plan& pl, int dtype) {

@@ -559,6 +559,27 @@ __global__ void __launch_bounds__(256) k_angles(AngleArgs a) {
     a.out[2 * e + 1] = sn;
 }
 
+// Coefficient table of the compiled passes: per (sample, slot), the slot's CoefForm of its
+// (cos, sin) in the real type T, with the same device functions the passes would otherwise run
+// in every lane (so the values are the ones those would compute).
+template <typename T>
+__global__ void __launch_bounds__(256) k_coefs(const double* __restrict__ ang, const uint8_t* __restrict__ forms,
+                                               int n_forms, int n_slots, uint64_t total, T* __restrict__ out) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int slot = int(e % uint64_t(n_slots));
+    const uint8_t form = slot < n_forms ? forms[slot] : uint8_t(COEF_NONE);
+    T c = T(ang[2 * e]), s = T(ang[2 * e + 1]);
+    if (form == COEF_SHEAR) {
+        const Shear<T> h = shear_of(c, s);
+        c = h.t, s = h.s;
+    } else if (form == COEF_DOUBLE) {
+        double_angle(c, s);
+    }
+    out[2 * e] = c;
+    out[2 * e + 1] = s;
+}
+
 // rows x n (re, im) -> rows (re, im), one wave per row, fixed order (deterministic)
 __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ in, uint64_t n, double* __restrict__ out,
                                                     uint64_t rows) {
@@ -830,6 +851,20 @@ hipError_t launch_angles(const AngleArgs& a, hipStream_t s) {
     uint64_t total = a.batch * uint64_t(a.n_slots);
     if (total == 0) return hipSuccess;
     hipLaunchKernelGGL(k_angles, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_coefs(int dtype, const double* angles, const uint8_t* forms, int n_forms, int n_slots, uint64_t batch,
+                        void* out, hipStream_t s) {
+    const uint64_t total = batch * uint64_t(n_slots);
+    if (total == 0) return hipSuccess;
+    const dim3 grid(uint32_t((total + 255) / 256));
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_coefs<float>, grid, dim3(256), 0, s, angles, forms, n_forms, n_slots, total,
+                           static_cast<float*>(out));
+    else
+        hipLaunchKernelGGL(k_coefs<double>, grid, dim3(256), 0, s, angles, forms, n_forms, n_slots, total,
+                           static_cast<double*>(out));
     return hipGetLastError();
 }
 

@@ -92,6 +92,10 @@ hipError_t launch_fill(int dtype, void* dst, const void* src, uint64_t dim, uint
                        uint64_t batch, hipStream_t s);
 hipError_t launch_cast(int src_dtype, const void* src, void* dst, uint64_t n_reals, hipStream_t s);
 hipError_t launch_angles(const AngleArgs& a, hipStream_t s);
+// Coefficient table [batch][n_slots][2] (dtype 0: float, 1: double) of an angle table, per the
+// slots' CoefForm entries (forms[0, n_forms); slots beyond: the (cos, sin) pair).
+hipError_t launch_coefs(int dtype, const double* angles, const uint8_t* forms, int n_forms, int n_slots, uint64_t batch,
+                        void* out, hipStream_t s);
 hipError_t launch_reduce_rows(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
 hipError_t launch_reduce_rows1(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
 hipError_t launch_jacobian(const double* taps, int n_tap_ids, const int32_t* jac_begin, const int32_t* jac_tap,

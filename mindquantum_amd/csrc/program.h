@@ -64,6 +64,16 @@ enum OpKind : int32_t {
     OPK_PF_RZZ = 28,
 };
 
+// Coefficient forms of a slot in the compiled passes' coefficient table (TilePassArgs::coefs),
+// derived from the slot's (cos, sin) in the state's real type, so that a pass reads its
+// coefficients ready-made instead of deriving them in every lane:
+//   COEF_CS:     (cos, sin) of the angle table (parameterised ops in their exact form);
+//   COEF_SHEAR:  Shear (t, s) of the sign-normalised pair (PF_RX, PF_RY);
+//   COEF_DOUBLE: (cos, sin) of the doubled angle (PF_RZ, PF_RZZ).
+// A back-sweep inverse negates the second entry (and t), exactly as it negates sin.  COEF_NONE:
+// no entry (the slot has no op in the plan, or ops of two forms).
+enum CoefForm : uint8_t { COEF_NONE = 0, COEF_CS = 1, COEF_SHEAR = 2, COEF_DOUBLE = 3 };
+
 inline bool opk_is_param(int k) { return k >= OPK_RX; }
 inline bool opk_is_two(int k) {
     return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ || k == OPK_CX || k == OPK_CZ ||
@@ -168,6 +178,7 @@ struct TilePassArgs {
     const int32_t* tap_ids;
     double* partials;      // [batch][n_tap_ids][n_tiles] Re(tap)
     uint64_t batch;        // states (persistent passes walk n_tiles * batch units)
+    const void* coefs;     // [batch][n_slots][2] in the real type, per CoefForm (compiled passes)
 };
 
 // Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are

@@ -100,3 +100,19 @@ def test_single_buffer_back_sweep_compiles_c128():
     circ = hea(12, 2, encoder=True)
     circ += G.RY("c").on(2, 11)
     compile_circuit(circ, 12, _native.MQA_C128, 11)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_coefficients_come_ready_made(tmp_path, monkeypatch, mode):
+    """Compiled passes read each parameterised op's coefficients from the coefficient table
+    (shear t and s, the doubled angle's cos and sin, or the (cos, sin) pair) instead of deriving
+    them in every lane; back-sweep inverses negate them in the read."""
+    monkeypatch.setenv("MQAMD_JIT_DUMP", str(tmp_path))
+    circ = hea(12, 2, encoder=True)
+    circ += G.RZ("c").on(2, 11)  # controlled: exact RZ in both plan forms
+    compile_circuit(circ, 12, _native.MQA_C64, mode)
+    src = "".join(p.read_text() for p in sorted(tmp_path.glob("pass*.hip")))
+    assert src.count("coef_row[") >= 2 * len(circ.params_name)
+    assert "ang_row[" not in src and "shear_of(" not in src and "double_angle(" not in src
+    assert ("-coef_row[" in src) == bool(mode & 1)
+    assert ("Shear<T> h{" in src) == bool(mode & 2)
