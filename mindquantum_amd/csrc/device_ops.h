@@ -825,6 +825,16 @@ __device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const
     return sb_begin<T, KC>(args, pass, smem, blockIdx.x, blockIdx.y);
 }
 
+// Orders one wave's LDS accesses without a workgroup barrier: wavefront-scope fences around a
+// scheduling barrier keep the compiler from moving LDS accesses across it, and the LDS executes
+// one wave's accesses in issue order.  Enough between two register layouts in which every wave
+// keeps the same tile indices (the layouts agree on the positions the wave index takes).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One state's registers <-> the LDS tile buffer in a group's layout.
 template <typename T>
 __device__ __forceinline__ void group_load1(const V2<T>* buf, uint32_t a0, const uint16_t* off, Regs<T>& r) {

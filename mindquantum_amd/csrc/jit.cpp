@@ -234,17 +234,33 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         o << "  __syncthreads();\n  tile_load<T>(f.t_psi, f.g_lam, f.offtab, f.C, f.tid, f.NT);\n  __syncthreads();\n";
         sb_move("load", 0, "l");
     }
+    // Thread t of a group holds the tile indices whose non-register positions, in increasing
+    // order, spell t (group_ctx); its wave index takes the top K - 10 of them.  Two layouts that
+    // agree on those positions give every wave the same indices, and so the same LDS slots: the
+    // re-lay between them only has to order each wave's own accesses.
+    const int wave_bits = std::max(0, pl.K - kGroupWidth - 6);
+    auto wave_positions = [&](const DevGroup& g) {
+        uint32_t m = 0;
+        for (int q = pl.K - 1, need = wave_bits; q >= 0 && need > 0; q--) {
+            if (q == g.pos[0] || q == g.pos[1] || q == g.pos[2] || q == g.pos[3]) continue;
+            m |= 1u << q;
+            need--;
+        }
+        return m;
+    };
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
         if (sb) {
             if (gi > 0) {  // re-lay psi, then lambda, from group gi - 1's layout into group gi's
-                o << "  __syncthreads();\n";
+                const DevGroup& prev = pl.dev_groups.at(size_t(ps.group_begin + gi - 1));
+                const char* sync = wave_positions(prev) == wave_positions(g) ? "  wave_sync();\n" : "  __syncthreads();\n";
+                o << sync;
                 sb_move("store", gi - 1, "p");
-                o << "  __syncthreads();\n";
+                o << sync;
                 sb_move("load", gi, "p");
-                o << "  __syncthreads();\n";
+                o << sync;
                 sb_move("store", gi - 1, "l");
-                o << "  __syncthreads();\n";
+                o << sync;
                 sb_move("load", gi, "l");
             }
             o << "  {  // group " << gi << "\n"

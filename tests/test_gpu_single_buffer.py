@@ -52,8 +52,10 @@ def run(circ, n, ops, batch, seed, tiles=None):
 
 
 @pytest.mark.parametrize("phase_free", [False, True])
-@pytest.mark.parametrize("tile", [8, 12])
+@pytest.mark.parametrize("tile", [8, 11, 12, 13])
 def test_single_buffer_gradient_matches_oracle(options, phase_free, tile):
+    """Tiles of one partial wave (8: every group transition orders one wave's accesses only) and of
+    2, 4 and 8 waves (workgroup barriers, except between layouts that keep each wave's indices)."""
     n = 14
     _native.set_option("phase_free", int(phase_free))
     before = _native.get_option("jit_single_buffer_launches")

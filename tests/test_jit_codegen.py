@@ -5,6 +5,7 @@ pass and compiles it with hipRTC, as the library does in the background for larg
 No GPU is needed; parity of the compiled kernels is checked in ``test_gpu_jit.py``.
 """
 import ctypes
+import re
 
 import numpy as np
 import pytest
@@ -116,3 +117,26 @@ def test_coefficients_come_ready_made(tmp_path, monkeypatch, mode):
     assert "ang_row[" not in src and "shear_of(" not in src and "double_angle(" not in src
     assert ("-coef_row[" in src) == bool(mode & 1)
     assert ("Shear<T> h{" in src) == bool(mode & 2)
+
+
+def test_single_buffer_transitions_without_workgroup_barriers(tmp_path, monkeypatch):
+    """A back-sweep re-lay between two register layouts that agree on the positions the wave index
+    takes (the top K - 10 non-register positions) orders one wave's own LDS accesses only; the
+    others keep workgroup barriers."""
+    monkeypatch.setenv("MQAMD_JIT_DUMP", str(tmp_path))
+    compile_circuit(hea(20, 2, encoder=True), 20, _native.MQA_C64, 11)  # 12-qubit tiles: 4 waves
+    local = barrier = 0
+    for path in tmp_path.glob("pass*.hip"):
+        src = path.read_text()
+        pos = [set(int(v) for v in m.split(","))
+               for m in re.findall(r"constexpr uint8_t pos_\d+\[4\] = \{([^}]*)\}", src)]
+        parts = src.split("  {  // group ")  # parts[g + 1] opens with group g; its tail is the re-lay
+        for g in range(1, len(pos)):
+            relay = parts[g].rsplit("    }\n", 1)[-1]
+            if sorted(set(range(12)) - pos[g - 1])[-2:] == sorted(set(range(12)) - pos[g])[-2:]:
+                assert relay.count("wave_sync();") == 4 and "__syncthreads" not in relay
+                local += 1
+            else:
+                assert relay.count("__syncthreads();") == 4 and "wave_sync" not in relay
+                barrier += 1
+    assert local > 0 and barrier > 0
