@@ -109,7 +109,10 @@ int mqa_device_count(void);
  * slot, each walking tiles and loading the next tile while it computes the current one),
  * "jit_persistent_blocks" (> 0: at most that many blocks in a persistent pass), "jit_single_buffer"
  * (1, the default: compiled back-sweep passes keep psi and lambda in registers between register
- * groups and share one LDS tile buffer, so twice as many blocks fit a CU; 0: two tile buffers).  mqa_get_option also reads the counters
+ * groups and share one LDS tile buffer, so twice as many blocks fit a CU; 0: two tile buffers),
+ * "jit_wave_local" (1: a compiled pass re-lays its registers between two layouts that keep every
+ * wave's tile indices without workgroup barriers; 0, the default: barriers on every re-lay; applies
+ * to passes compiled after the change).  mqa_get_option also reads the counters
  * "jit_launches", "jit_compile_ms", "grad_recomputes", "checkpoints_used", "phase_free_used",
  * "jit_persistent_launches" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */

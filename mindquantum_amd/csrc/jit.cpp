@@ -154,7 +154,7 @@ std::atomic<int64_t> g_compile_us{0};
 }
 double jit_compile_seconds_total() { return double(g_compile_us.load()) * 1e-6; }
 
-std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm form) {
+std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm form, bool wave_local_relays) {
     const bool persistent = form == JitForm::kPersistent, sb = form == JitForm::kSingleBuffer;
     if (persistent && !jit_can_persist(pl, dtype))
         throw std::invalid_argument("code generation: persistent passes are complex64 back-sweeps only");
@@ -248,12 +248,15 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         }
         return m;
     };
+    auto wave_local = [&](const DevGroup& a, const DevGroup& b) {
+        return wave_local_relays && wave_positions(a) == wave_positions(b);
+    };
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
         if (sb) {
             if (gi > 0) {  // re-lay psi, then lambda, from group gi - 1's layout into group gi's
                 const DevGroup& prev = pl.dev_groups.at(size_t(ps.group_begin + gi - 1));
-                const char* sync = wave_positions(prev) == wave_positions(g) ? "  wave_sync();\n" : "  __syncthreads();\n";
+                const char* sync = wave_local(prev, g) ? "  wave_sync();\n" : "  __syncthreads();\n";
                 o << sync;
                 sb_move("store", gi - 1, "p");
                 o << sync;
@@ -361,8 +364,20 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
             o << "  }\n";
             continue;
         }
+        // the next group reads the tile in its own layout: a workgroup barrier, unless every wave
+        // keeps its tile indices (dense blocks, which read the tile themselves, keep the barrier)
+        bool local = false;
+        if (!persistent && gi + 1 < ps.n_groups) {
+            const DevGroup& next = pl.dev_groups.at(size_t(ps.group_begin + gi + 1));
+            local = wave_local(g, next);
+            for (const DevGroup* h : {&g, &next})
+                for (int oi = 0; oi < h->n_ops; oi++) {
+                    const int k = pl.dev_ops.at(size_t(h->op_begin + oi)).kind;
+                    if (k == OPK_M3 || k == OPK_M4) local = false;
+                }
+        }
         o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
-          << "    __syncthreads();\n"
+          << (local ? "    wave_sync();\n" : "    __syncthreads();\n")
           << "  }\n";
     }
     if (sb) {  // psi, then lambda, back through the buffer to HBM; the tap partials
@@ -412,7 +427,8 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log) {
     return ok;
 }
 
-JitPlan::JitPlan(const Plan& pl, int dtype, JitForm form) : plan_(pl), dtype_(dtype), form_(form) {
+JitPlan::JitPlan(const Plan& pl, int dtype, JitForm form, bool wave_local)
+    : plan_(pl), dtype_(dtype), form_(form), wave_local_(wave_local) {
     if ((form_ == JitForm::kPersistent && !jit_can_persist(pl, dtype)) ||
         (form_ == JitForm::kSingleBuffer && !jit_can_single_buffer(pl, dtype)))
         form_ = JitForm::kTile;
@@ -439,7 +455,7 @@ void JitPlan::compile_all() {
     std::vector<std::string> src(static_cast<size_t>(n)), err(static_cast<size_t>(n));
     std::vector<char> done(static_cast<size_t>(n), 0);
     try {
-        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, form_);
+        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, form_, wave_local_);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
         error_ = e.what();

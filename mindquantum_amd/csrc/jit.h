@@ -32,7 +32,10 @@ const char* jit_kernel_name(bool adjoint);
 // gridDim.x and prefetch the next unit's tiles into registers; kSingleBuffer (back-sweeps), psi and
 // lambda stay in registers between register groups and share one LDS tile buffer (device_ops.h).
 enum class JitForm { kTile = 0, kPersistent = 1, kSingleBuffer = 2 };
-std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile);
+// wave_local: a register group's re-lay between two layouts that keep every wave's tile indices
+// orders the wave's own LDS accesses instead of taking workgroup barriers (device_ops.h wave_sync).
+std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile,
+                            bool wave_local = false);
 
 // CoefForm of every angle slot a plan's ops read (index = slot; program.h): the layout of the
 // coefficient table its compiled passes take.  Empty when no op has an angle.
@@ -57,7 +60,7 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log);
 class JitPlan {
   public:
     // Copies the plan and starts compiling every pass on a background thread.
-    JitPlan(const Plan& pl, int dtype, JitForm form = JitForm::kTile);
+    JitPlan(const Plan& pl, int dtype, JitForm form = JitForm::kTile, bool wave_local = false);
     ~JitPlan();
     JitPlan(const JitPlan&) = delete;
     JitPlan& operator=(const JitPlan&) = delete;
@@ -82,6 +85,7 @@ class JitPlan {
     Plan plan_;
     int dtype_;
     JitForm form_;
+    bool wave_local_;
     bool persistent_;
     std::map<int, std::vector<unsigned>> grid_;  // per device: resident blocks of each persistent pass
     std::atomic<int> state_{kCompiling};

@@ -15,7 +15,7 @@ from tests.test_gpu_parity import assert_grad, oracle_grad
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ("jit", "jit_min_amps", "phase_free", "checkpoints", "jit_single_buffer")
+KEYS = ("jit", "jit_min_amps", "phase_free", "checkpoints", "jit_single_buffer", "jit_wave_local")
 
 
 @pytest.fixture
@@ -51,13 +51,16 @@ def run(circ, n, ops, batch, seed, tiles=None):
     assert_grad(np.concatenate([g_enc, g_ans], axis=2), g_ref, mq.complex64)
 
 
+@pytest.mark.parametrize("wave_local", [0, 1])
 @pytest.mark.parametrize("phase_free", [False, True])
 @pytest.mark.parametrize("tile", [8, 11, 12, 13])
-def test_single_buffer_gradient_matches_oracle(options, phase_free, tile):
-    """Tiles of one partial wave (8: every group transition orders one wave's accesses only) and of
-    2, 4 and 8 waves (workgroup barriers, except between layouts that keep each wave's indices)."""
+def test_single_buffer_gradient_matches_oracle(options, phase_free, tile, wave_local):
+    """Tiles of one partial wave and of 2, 4 and 8 waves, with workgroup barriers on every re-lay
+    and (option jit_wave_local) without them between layouts that keep each wave's indices (on
+    every re-lay of the one-wave tiles)."""
     n = 14
     _native.set_option("phase_free", int(phase_free))
+    _native.set_option("jit_wave_local", wave_local)
     before = _native.get_option("jit_single_buffer_launches")
     run(circuit(n), n, [random_hamiltonian(n, 9, 4), mq.QubitOperator("X0 Y5") + 0.3 * mq.QubitOperator("Z13")],
         4, tile, tiles=(12, tile))
