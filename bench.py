@@ -258,6 +258,8 @@ def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
     _native.set_option("checkpoints", checkpoints)
+    jit = _native.get_option("jit")
+    _native.set_option("jit", 2)  # the warm-up waits for every compiled pass (checkpoint segments too)
     try:
         grad_ops(ans)  # plans and compiled passes
         _native.profile_reset()
@@ -268,6 +270,7 @@ def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
         _native.set_option("profile", 0)
     finally:
         _native.set_option("checkpoints", -1)
+        _native.set_option("jit", jit)
     amp = 8 if dtype == mq.complex64 else 16
     kern = {k: _native.profile(k) for k in ("tile_pass_fwd", "tile_pass_adj", "hamiltonian")}
     res = {"qubits": n, "dtype": np.dtype(dtype).name, "params": len(circ.params_name), "gates": len(circ),
