@@ -806,13 +806,16 @@ void check_ham(const mqa_sim* sim, const mqa_ham* h) {
     require((h->qubit_mask >> sim->n) == 0 || sim->n >= 64, "hamiltonian acts on more qubits than the simulator has");
 }
 
-size_t hbm_budget() {
+// Workspace budget of a call: the option / environment value, else 85% of the free memory plus
+// `held`, the bytes of the simulator's grow-only workspaces that the call may reuse (a repeated
+// call must see the same budget as the first, which allocated them).
+size_t hbm_budget(size_t held = 0) {
     int64_t b = g_opt.budget;
     if (const char* env = std::getenv("MQAMD_HBM_BUDGET_BYTES")) b = std::atoll(env);
     if (b > 0) return size_t(b);
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    return size_t(double(free_b) * 0.85);
+    return size_t(double(free_b + held) * 0.85);
 }
 
 }  // namespace
@@ -1399,7 +1402,7 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         // many checkpoints are held as the budget allows for one sample.
         const size_t side = size_t(n_tap_ids) * n_tiles * 8 + size_t(circ->n_slots) * 16 +
                             size_t(n_tap_ids + n_params) * 16 + 4096;
-        const size_t budget = hbm_budget();
+        const size_t budget = hbm_budget(sim->psi.bytes + sim->lam.bytes + sim->psi_keep.bytes + sim->ckpt.bytes);
         int n_ck = 0;
         if (g_opt.checkpoints > 0 && n_tap_ids > 0) {
             n_ck = int(std::min<int64_t>(g_opt.checkpoints, int64_t(adj->plan.passes.size()) - 1));
@@ -1414,6 +1417,8 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         chunk = std::min<int64_t>(chunk, batch);
         chunk = std::min<int64_t>(chunk, 65535);
 
+        if (!keep) sim->psi_keep.release();  // counted in the budget: free what this call does not use
+        if (n_ck == 0) sim->ckpt.release();
         sim->psi.ensure(sb * chunk);
         sim->lam.ensure(sb * chunk);
         if (keep) sim->psi_keep.ensure(sb * chunk);
