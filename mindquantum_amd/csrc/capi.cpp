@@ -2052,7 +2052,7 @@ int mqa_sim_expectation_angles(const mqa_sim* sim_c, const mqa_ham* const* hams,
         hipStream_t s = sim->stream;
         const size_t sb = sim->state_bytes();
         DevPlan* fwd = circ->gates.empty() ? nullptr : circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
-        const size_t budget = hbm_budget();
+        const size_t budget = hbm_budget(sim->psi.bytes + sim->lam.bytes);
         int64_t chunk = int64_t(std::max<size_t>(budget / (2 * sb + 4096), 1));
         chunk = std::min<int64_t>({chunk, batch, 65535});
         sim->psi.ensure(sb * chunk);
