@@ -105,16 +105,10 @@ int mqa_device_count(void);
  * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
  * the first launch), "jit_min_amps", "checkpoints", "phase_free" (1, the default: mqa_sim_grad_batch
  * runs plans in which uncontrolled RX / RY / RZ / GP drop their global phase, see there; 0: exact
- * plans), "jit_persistent" (1: compiled complex64 back-sweep passes run one block per resident
- * slot, each walking tiles and loading the next tile while it computes the current one),
- * "jit_persistent_blocks" (> 0: at most that many blocks in a persistent pass), "jit_single_buffer"
- * (1, the default: compiled back-sweep passes keep psi and lambda in registers between register
- * groups and share one LDS tile buffer, so twice as many blocks fit a CU; 0: two tile buffers),
- * "jit_wave_local" (1: a compiled pass re-lays its registers between two layouts that keep every
- * wave's tile indices without workgroup barriers; 0, the default: barriers on every re-lay; applies
- * to passes compiled after the change).  mqa_get_option also reads the counters
- * "jit_launches", "jit_compile_ms", "grad_recomputes", "checkpoints_used", "phase_free_used",
- * "jit_persistent_launches" and "jit_single_buffer_launches".  No counterpart
+ * plans), "jit_single_buffer" (1, the default: compiled back-sweep passes keep psi and lambda in
+ * registers between register groups and share one LDS tile buffer, so twice as many blocks fit a CU;
+ * 0: two tile buffers).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
+ * "grad_recomputes", "checkpoints_used", "phase_free_used" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
 int mqa_get_option(const char* key, int64_t* value);
@@ -158,8 +152,8 @@ int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtyp
                          int64_t cap, int64_t* len);
 /* Generate and compile (hipRTC, host only, no GPU needed) the per-circuit kernels of the
  * forward (adjoint == 0) or back-sweep (1) schedule, or of their phase-free forms (2, 3: those
- * the hermitian gradient runs); + 4 for the persistent form of a complex64 back-sweep (option
- * "jit_persistent") or + 8 for its single-buffer form (option "jit_single_buffer"); as the library
+ * the hermitian gradient runs); + 8 for the single-buffer form of a back-sweep (option
+ * "jit_single_buffer"); as the library
  * does in the background for large launches (option "jit").  *seconds gets the compile wall time.  No counterpart in the
  * reference: a build check of the code generator for tests and for callers that want to pay
  * the compile cost up front. */
@@ -178,7 +172,7 @@ int mqa_ham_create_csr(uint64_t dim, uint64_t nnz, const int64_t* indptr, const 
                        mqa_ham** out);
 int mqa_ham_destroy(mqa_ham* ham);
 
-/* ---- simulator (reference: ccsrc/include/python/vector/bind_vec_state.h:46-84, BindSim) ---- */
+/* ---- simulator (reference: ccsrc/python/simulator/include/python/vector/bind_vec_state.h:46-84, BindSim) ---- */
 /* VectorState(n_qubits, seed) (vector_state.tpp:63-68); the state starts as |0...0>. */
 int mqa_sim_create(int32_t n_qubits, int32_t dtype, uint32_t seed, mqa_sim** out);
 int mqa_sim_destroy(mqa_sim* sim);
@@ -226,6 +220,17 @@ int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ha
 int mqa_sim_grad_batch(const mqa_sim* sim, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ,
                        const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
                        double* g_out);
+/* How mqa_sim_grad_batch sizes its device workspaces (host only, no GPU): for a workspace budget
+ * (85% of free memory plus the workspaces the simulator already holds), one state of state_bytes,
+ * side_bytes of small per-sample tables, n_hams observables and n_checkpoints checkpoints, *chunk
+ * samples run per chunk, *keep tells whether the forward state is kept for further observables, and
+ * need[] holds the bytes of the four workspaces (psi, lambda, kept state, checkpoints).  held[] are
+ * their current sizes; release[i] = 1 when workspace i is freed before any allocation (it must
+ * grow, is no longer used, or a larger-than-needed buffer would not fit beside the others).  No
+ * counterpart in the reference (its workspaces are host memory per thread). */
+int mqa_grad_workspace_plan(uint64_t budget_bytes, uint64_t state_bytes, uint64_t side_bytes, int32_t n_hams,
+                            int32_t n_checkpoints, int64_t batch, const uint64_t held[4], int64_t* chunk, int32_t* keep,
+                            uint64_t need[4], int32_t release[4]);
 /* get_expectation_with_grad_non_hermitian_multi_multi (vector_state.tpp:998-1008 ->
  * gradient.h:83-98, 199-268): f = <phi_l|H|psi_r> with psi_r = circ_right |sim>, phi_l =
  * circ_left |left> (left NULL: sim), and its complex derivative df/dp = <phi_l|H|d psi_r> +

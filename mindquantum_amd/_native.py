@@ -90,6 +90,8 @@ _SIGS = {
     "mqa_sim_expectation_angles": ([_P, _P, _I32, _P, _P, _I64, _P], ctypes.c_int),
     "mqa_sim_astype": ([_P, _I32, _PP], ctypes.c_int),
     "mqa_grad_items": ([_I32, _P, _P, _P, _P, _P, _I32, _P, _P], ctypes.c_int),
+    "mqa_grad_workspace_plan": ([_U64, _U64, _U64, _I32, _I32, _I64, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I32),
+                                 _P, _P], ctypes.c_int),
     "mqa_inner_product": ([_P, _P, _P], ctypes.c_int),
     "mqa_profile_get": ([ctypes.c_char_p, ctypes.POINTER(_I64), _DP, _DP], ctypes.c_int),
     "mqa_profile_reset": ([], ctypes.c_int),
@@ -132,6 +134,18 @@ def ptr(arr):
     if arr is None:
         return None
     return ctypes.c_void_p(arr.ctypes.data)
+
+
+def grad_workspace_plan(budget, state_bytes, side_bytes, n_hams, n_checkpoints, batch, held):
+    """(chunk, keep, need[4], release[4]) of mqa_grad_workspace_plan (host only)."""
+    h = np.ascontiguousarray(held, dtype=np.uint64)
+    need = np.zeros(4, dtype=np.uint64)
+    rel = np.zeros(4, dtype=np.int32)
+    chunk, keep = _I64(0), _I32(0)
+    check(lib().mqa_grad_workspace_plan(int(budget), int(state_bytes), int(side_bytes), int(n_hams),
+                                        int(n_checkpoints), int(batch), ptr(h), ctypes.byref(chunk),
+                                        ctypes.byref(keep), ptr(need), ptr(rel)))
+    return int(chunk.value), bool(keep.value), [int(x) for x in need], [bool(x) for x in rel]
 
 
 def profile(kernel):

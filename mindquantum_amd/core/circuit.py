@@ -8,6 +8,7 @@ ansatz_params_name, all_encoder, all_ansatz), as_encoder / as_ansatz / no_grad, 
 import copy
 
 from . import gates as G
+from .mutation import bump
 from .parameterresolver import ParameterResolver
 
 
@@ -30,17 +31,19 @@ class Circuit(list):
         for g in gates:
             self.append(g)
 
-    # ---- construction
+    # ---- construction (every list mutation advances the mutation epoch)
     def append(self, gate):
         if isinstance(gate, Circuit):
             for g in gate:
                 super().append(g)
+            bump()
             return self
         if not isinstance(gate, G.BasicGate):
             raise TypeError(f"Circuit only accepts gates, but get {type(gate)}")
         if not gate.obj_qubits:
             raise ValueError(f"Gate {gate.name} has not been applied on any qubit; use .on()")
         super().append(gate)
+        bump()
         return self
 
     def extend(self, gates):
@@ -52,6 +55,44 @@ class Circuit(list):
         if not isinstance(gate, G.BasicGate):
             raise TypeError("Circuit only accepts gates")
         super().insert(index, gate)
+        bump()
+
+    def __setitem__(self, index, value):
+        items = list(value) if isinstance(index, slice) else [value]
+        if not all(isinstance(g, G.BasicGate) for g in items):
+            raise TypeError("Circuit only accepts gates")
+        super().__setitem__(index, value)
+        bump()
+
+    def __delitem__(self, index):
+        super().__delitem__(index)
+        bump()
+
+    def pop(self, index=-1):
+        g = super().pop(index)
+        bump()
+        return g
+
+    def remove(self, gate):
+        super().remove(gate)
+        bump()
+
+    def clear(self):
+        super().clear()
+        bump()
+
+    def reverse(self):
+        super().reverse()
+        bump()
+
+    def sort(self, *args, **kwargs):
+        super().sort(*args, **kwargs)
+        bump()
+
+    def __imul__(self, n):
+        res = super().__imul__(n)
+        bump()
+        return res
 
     def __iadd__(self, other):
         if isinstance(other, (Circuit, list)):

@@ -15,6 +15,7 @@ import numbers
 
 import numpy as np
 
+from .mutation import bump
 from .parameterresolver import ParameterResolver
 
 
@@ -38,6 +39,11 @@ class BasicGate:
         self.n_qubits = n_qubits
         self.obj_qubits = []
         self.ctrl_qubits = []
+
+    def __setattr__(self, key, value):
+        # any change of a gate can change the native circuits lowered from circuits that hold it
+        object.__setattr__(self, key, value)
+        bump()
 
     def on(self, obj_qubits, ctrl_qubits=None):
         """A copy of this gate acting on ``obj_qubits`` controlled by ``ctrl_qubits``."""

@@ -9,6 +9,8 @@ import numbers
 
 import numpy as np
 
+from .mutation import bump
+
 
 class ParameterResolver:
     """Affine combination of named real parameters."""
@@ -45,6 +47,7 @@ class ParameterResolver:
     @const.setter
     def const(self, value):
         self._const = value
+        bump()
 
     @property
     def params_name(self):
@@ -94,18 +97,22 @@ class ParameterResolver:
     # ---- flags (return self, like the reference)
     def as_encoder(self):
         self._encoder |= set(self._data)
+        bump()
         return self
 
     def as_ansatz(self):
         self._encoder -= set(self._data)
+        bump()
         return self
 
     def no_grad(self):
         self._no_grad |= set(self._data)
+        bump()
         return self
 
     def requires_grad(self):
         self._no_grad -= set(self._data)
+        bump()
         return self
 
     def no_grad_part(self, *names):
@@ -113,6 +120,7 @@ class ParameterResolver:
             if n not in self._data:
                 raise KeyError(f"Parameter {n} not in this parameter resolver!")
             self._no_grad.add(n)
+        bump()
         return self
 
     def requires_grad_part(self, *names):
@@ -120,16 +128,19 @@ class ParameterResolver:
             if n not in self._data:
                 raise KeyError(f"Parameter {n} not in this parameter resolver!")
             self._no_grad.discard(n)
+        bump()
         return self
 
     def encoder_part(self, *names):
         for n in names:
             self._encoder.add(n)
+        bump()
         return self
 
     def ansatz_part(self, *names):
         for n in names:
             self._encoder.discard(n)
+        bump()
         return self
 
     # ---- evaluation

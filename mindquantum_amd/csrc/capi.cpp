@@ -78,16 +78,9 @@ struct Options {
     std::atomic<int64_t> checkpoints{0};
     // 1: hermitian gradients run phase-free plans (uncontrolled rotations drop global phases)
     std::atomic<int64_t> phase_free{1};
-    // 1: compiled complex64 back-sweep passes run persistent, prefetching the next tile;
-    // jit_persistent_blocks > 0 caps their grid (tests: many units per block)
-    std::atomic<int64_t> jit_persistent{0}, jit_persist_blocks{0};
     // 1: compiled back-sweep passes keep psi and lambda in registers and share one LDS tile
     // buffer (twice the resident blocks per CU)
     std::atomic<int64_t> jit_single_buffer{1};
-    // 1: compiled passes re-lay between register layouts that keep every wave's tile indices
-    // without workgroup barriers (for passes compiled after the change); off by default: no
-    // measurable difference on the 20-qubit gradient (7.302 ms per back-sweep pass either way)
-    std::atomic<int64_t> jit_wave_local{0};
 };
 Options g_opt;
 
@@ -105,9 +98,7 @@ std::vector<OptEntry> opt_table() {
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
-            {"phase_free", &g_opt.phase_free, 1}, {"jit_persistent", &g_opt.jit_persistent, 0},
-            {"jit_persistent_blocks", &g_opt.jit_persist_blocks, 0},
-            {"jit_single_buffer", &g_opt.jit_single_buffer, 1}, {"jit_wave_local", &g_opt.jit_wave_local, 0}};
+            {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -169,9 +160,8 @@ struct DevPlan {
     int n_coef_forms = 0;
     bool full = false;
     std::mutex jit_mu;
-    std::unique_ptr<JitPlan> jit;          // compiled passes (created on first large launch)
-    std::unique_ptr<JitPlan> jit_persist;  // the same as persistent kernels (option "jit_persistent")
-    std::unique_ptr<JitPlan> jit_sb;       // the same with one LDS tile buffer (option "jit_single_buffer")
+    std::unique_ptr<JitPlan> jit;     // compiled passes (created on first large launch)
+    std::unique_ptr<JitPlan> jit_sb;  // the same with one LDS tile buffer (option "jit_single_buffer")
 };
 
 // ---------------------------------------------------------------- kernel timing
@@ -508,7 +498,6 @@ void timed(mqa_sim* sim, const char* name, double bytes, F&& fn) {
 }
 
 std::atomic<int64_t> g_jit_launches{0};
-std::atomic<int64_t> g_jit_persist_launches{0};  // of those, persistent kernels
 std::atomic<int64_t> g_jit_sb_launches{0};       // of those, single-buffer kernels
 std::atomic<int64_t> g_recomputes{0};  // forward sweeps recomputed because a kept copy did not fit
 std::atomic<int64_t> g_ckpt_used{0};    // checkpoints the last batched gradient call held
@@ -548,10 +537,9 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
         std::lock_guard<std::mutex> lk(dp->jit_mu);
         JitForm form = JitForm::kTile;
-        if (g_opt.jit_persistent != 0 && jit_can_persist(pl, sim->dtype)) form = JitForm::kPersistent;
-        else if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, sim->dtype)) form = JitForm::kSingleBuffer;
-        auto& slot = form == JitForm::kPersistent ? dp->jit_persist : form == JitForm::kSingleBuffer ? dp->jit_sb : dp->jit;
-        if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, form, g_opt.jit_wave_local != 0);
+        if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, sim->dtype)) form = JitForm::kSingleBuffer;
+        auto& slot = form == JitForm::kSingleBuffer ? dp->jit_sb : dp->jit;
+        if (!slot) slot = std::make_unique<JitPlan>(pl, sim->dtype, form);
         if (g_opt.jit == 2) slot->wait();
         if (slot->ready()) jp = slot.get();  // until then (or if it failed): the interpreted kernel
     }
@@ -572,8 +560,7 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         a.pass_index = i;
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
             if (jp) {
-                hip_check(jp->launch(i, a, batch, lds, sim->stream, unsigned(std::max<int64_t>(0, g_opt.jit_persist_blocks))),
-                          "compiled tile pass");
+                hip_check(jp->launch(i, a, batch, lds, sim->stream), "compiled tile pass");
             } else {
                 hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C,
                                            std::max(pl.max_taps, 1), sim->stream),
@@ -581,7 +568,6 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
             }
         });
         if (jp) count_jit_launch();
-        if (jp && jp->persistent()) g_jit_persist_launches.fetch_add(1);
         if (one_tile) g_jit_sb_launches.fetch_add(1);
     }
 }
@@ -818,6 +804,48 @@ size_t hbm_budget(size_t held = 0) {
     return size_t(double(free_b + held) * 0.85);
 }
 
+// Device workspaces of the batched gradient for a workspace budget: samples per chunk, whether the
+// forward state is kept for further Hamiltonians, and the bytes of psi, lambda, the kept state and
+// the checkpoints (n_ck already fitted to the budget).  side: per-sample bytes of the small tables.
+struct GradWorkspace {
+    int64_t chunk = 1;
+    bool keep = false;
+    size_t bytes[4] = {0, 0, 0, 0};
+};
+GradWorkspace plan_grad_workspace(size_t budget, size_t sb, size_t side, int n_hams, int n_ck, int64_t batch) {
+    GradWorkspace w;
+    w.keep = n_ck == 0 && n_hams > 1 && budget >= sb * 3 + side;
+    const size_t per_sample = sb * size_t(2 + n_ck + (w.keep ? 1 : 0)) + side;
+    w.chunk = std::min<int64_t>({int64_t(std::max<size_t>(budget / per_sample, 1)), batch, 65535});
+    const size_t one = sb * size_t(w.chunk);
+    w.bytes[0] = w.bytes[1] = one;
+    w.bytes[2] = w.keep ? one : 0;
+    w.bytes[3] = one * size_t(n_ck);
+    return w;
+}
+
+// The budget counted the held workspaces as free memory.  A workspace that must grow is released
+// before anything is allocated; one that is larger than needed is kept only while the kept and the
+// new sizes together stay within the budget (so a repeated call keeps its buffers, and a call after
+// a larger one does not allocate on top of stale ones).  release[i] tells which are released.
+void fit_workspaces(const size_t* held, const size_t* need, int n, size_t budget, bool* release) {
+    size_t total = 0;
+    for (int i = 0; i < n; i++) total += std::max(held[i], need[i]);
+    for (int i = 0; i < n; i++)
+        release[i] = held[i] != need[i] && (held[i] < need[i] || need[i] == 0 || total > budget);
+}
+
+void fit_and_ensure(DevBuf* const* bufs, const size_t* need, int n, size_t budget) {
+    std::vector<size_t> held(size_t(n), 0);
+    for (int i = 0; i < n; i++) held[size_t(i)] = bufs[i]->bytes;
+    std::unique_ptr<bool[]> rel(new bool[size_t(n)]);
+    fit_workspaces(held.data(), need, n, budget, rel.get());
+    for (int i = 0; i < n; i++)
+        if (rel[size_t(i)]) bufs[i]->release();
+    for (int i = 0; i < n; i++)
+        if (need[i] > 0) bufs[i]->ensure(need[i]);
+}
+
 }  // namespace
 }  // namespace mqa
 
@@ -826,6 +854,25 @@ using namespace mqa;
 extern "C" {
 
 const char* mqa_last_error(void) { return g_last_error.c_str(); }
+
+int mqa_grad_workspace_plan(uint64_t budget_bytes, uint64_t state_bytes, uint64_t side_bytes, int32_t n_hams,
+                            int32_t n_checkpoints, int64_t batch, const uint64_t held[4], int64_t* chunk, int32_t* keep,
+                            uint64_t need[4], int32_t release[4]) {
+    return guarded([&] {
+        require(state_bytes > 0 && n_hams >= 1 && n_checkpoints >= 0 && batch >= 1, "bad argument");
+        require(held != nullptr && chunk != nullptr && keep != nullptr && need != nullptr && release != nullptr,
+                "NULL argument");
+        const GradWorkspace w = plan_grad_workspace(size_t(budget_bytes), size_t(state_bytes), size_t(side_bytes),
+                                                    n_hams, n_checkpoints, batch);
+        size_t h[4], nd[4];
+        bool rel[4];
+        for (int i = 0; i < 4; i++) h[i] = size_t(held[i]), nd[i] = w.bytes[i];
+        fit_workspaces(h, nd, 4, size_t(budget_bytes), rel);
+        *chunk = w.chunk;
+        *keep = w.keep ? 1 : 0;
+        for (int i = 0; i < 4; i++) need[i] = nd[i], release[i] = rel[i] ? 1 : 0;
+    });
+}
 
 const char* mqa_version(void) { return "mqamd 0.1.0 (gfx950)"; }
 
@@ -860,10 +907,6 @@ int mqa_get_option(const char* key, int64_t* value) {
         }
         if (std::strcmp(key, "jit_single_buffer_launches") == 0) {  // read-only: of those, single-buffer ones
             *value = g_jit_sb_launches.load();
-            return;
-        }
-        if (std::strcmp(key, "jit_persistent_launches") == 0) {  // read-only: of those, persistent ones
-            *value = g_jit_persist_launches.load();
             return;
         }
         if (std::strcmp(key, "grad_recomputes") == 0) {  // read-only: see mqa_sim_grad_batch
@@ -1044,15 +1087,12 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        require(adjoint >= 0 && adjoint <= 11 && (adjoint & 12) != 12, "adjoint must be 0..3, + 4 or + 8");
-        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, persist = (adjoint & 4) != 0,
-                   one_tile = (adjoint & 8) != 0;
+        require(adjoint >= 0 && adjoint <= 11 && (adjoint & 4) == 0, "adjoint must be 0..3, optionally + 8");
+        const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, one_tile = (adjoint & 8) != 0;
         const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf));
-        require(!persist || jit_can_persist(pl, dtype), "persistent passes are complex64 back-sweeps only");
         require(!one_tile || jit_can_single_buffer(pl, dtype),
                 "single-buffer passes are back-sweeps without dense 3- or 4-qubit blocks");
-        JitPlan jp(pl, dtype, persist ? JitForm::kPersistent : one_tile ? JitForm::kSingleBuffer : JitForm::kTile,
-                   g_opt.jit_wave_local != 0);
+        JitPlan jp(pl, dtype, one_tile ? JitForm::kSingleBuffer : JitForm::kTile);
         jp.wait();
         if (jp.failed()) throw Error(MQA_ERR_RUNTIME, "code generation failed: " + jp.error());
         if (seconds) *seconds = jp.compile_seconds();
@@ -1411,18 +1451,13 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         mqa_circuit::Checkpointed* ck = n_ck > 0 ? circ->checkpointed(sim->n_eff, sim->dtype, n_ck, adj, s) : nullptr;
         if (ck) n_ck = int(ck->bound.size());
         g_ckpt_used.store(n_ck);
-        const bool keep = n_ck == 0 && n_hams > 1 && budget >= sb * 3 + side;
-        const size_t per_sample = sb * size_t(2 + n_ck + (keep ? 1 : 0)) + side;
-        int64_t chunk = int64_t(std::max<size_t>(budget / per_sample, 1));
-        chunk = std::min<int64_t>(chunk, batch);
-        chunk = std::min<int64_t>(chunk, 65535);
-
-        if (!keep) sim->psi_keep.release();  // counted in the budget: free what this call does not use
-        if (n_ck == 0) sim->ckpt.release();
-        sim->psi.ensure(sb * chunk);
-        sim->lam.ensure(sb * chunk);
-        if (keep) sim->psi_keep.ensure(sb * chunk);
-        if (n_ck > 0) sim->ckpt.ensure(sb * chunk * size_t(n_ck));
+        const GradWorkspace ws = plan_grad_workspace(budget, sb, side, n_hams, n_ck, batch);
+        const bool keep = ws.keep;
+        const int64_t chunk = ws.chunk;
+        {
+            DevBuf* bufs[4] = {&sim->psi, &sim->lam, &sim->psi_keep, &sim->ckpt};
+            fit_and_ensure(bufs, ws.bytes, 4, budget);
+        }
         sim->partials.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * n_tiles * 8, 16));
         sim->taps.ensure(std::max<size_t>(size_t(chunk) * n_tap_ids * 8, 16));
         sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
@@ -2055,8 +2090,11 @@ int mqa_sim_expectation_angles(const mqa_sim* sim_c, const mqa_ham* const* hams,
         const size_t budget = hbm_budget(sim->psi.bytes + sim->lam.bytes);
         int64_t chunk = int64_t(std::max<size_t>(budget / (2 * sb + 4096), 1));
         chunk = std::min<int64_t>({chunk, batch, 65535});
-        sim->psi.ensure(sb * chunk);
-        sim->lam.ensure(sb * chunk);
+        {
+            DevBuf* bufs[2] = {&sim->psi, &sim->lam};
+            const size_t need[2] = {sb * size_t(chunk), sb * size_t(chunk)};
+            fit_and_ensure(bufs, need, 2, budget);
+        }
         std::vector<double> tab(size_t(chunk) * std::max(circ->n_slots, 1) * 2), f_host(size_t(chunk) * 2);
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t nb = std::min<int64_t>(chunk, batch - b0);

@@ -825,16 +825,6 @@ __device__ __forceinline__ TileFrame<T> sb_begin(const TilePassArgs& args, const
     return sb_begin<T, KC>(args, pass, smem, blockIdx.x, blockIdx.y);
 }
 
-// Orders one wave's LDS accesses without a workgroup barrier: wavefront-scope fences around a
-// scheduling barrier keep the compiler from moving LDS accesses across it, and the LDS executes
-// one wave's accesses in issue order.  Enough between two register layouts in which every wave
-// keeps the same tile indices (the layouts agree on the positions the wave index takes).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // One state's registers <-> the LDS tile buffer in a group's layout.
 template <typename T>
 __device__ __forceinline__ void group_load1(const V2<T>* buf, uint32_t a0, const uint16_t* off, Regs<T>& r) {
@@ -858,77 +848,6 @@ __device__ __forceinline__ void taps_out(const TilePassArgs& args, const DevPass
         for (int w = 0; w < f.n_waves; w++) sr += f.lds_taps[t * f.n_waves + w];
         const int tap_id = args.tap_ids[pass.tap_begin + t];
         args.partials[(f.sample * uint64_t(args.n_tap_ids) + uint64_t(tap_id)) * args.n_tiles + f.tile] = sr;
-    }
-}
-
-// ---------------------------------------------------------------- persistent passes
-// complex64 back-sweep passes compiled per circuit can run persistent: a block walks the
-// (tile, sample) units blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next unit's psi and
-// lambda tiles into registers while it computes the current one.  The global loads stay in
-// flight across the group barriers (a workgroup barrier waits for LDS accesses only), so the
-// stream overlaps the arithmetic inside each block instead of alternating with it.
-struct Prefetch64 {
-    float4 p[kAmpsPerThread / 2], l[kAmpsPerThread / 2];
-};
-
-// Frame of a persistent pass: LDS regions and the offset table, built once per block.
-template <int KC>
-__device__ __forceinline__ TileFrame<float> stream_setup(const DevPass& pass, unsigned char* smem) {
-    TileFrame<float> f;
-    f.NT = 1u << (KC - kGroupWidth);
-    f.tid = threadIdx.x;
-    f.C = pass.low_run;
-    const int H = KC - f.C;
-    const LdsLayout lay = lds_layout(KC, f.C, true, int(sizeof(float2)), 0);
-    f.t_psi = reinterpret_cast<float2*>(smem + lay.psi);
-    f.t_lam = reinterpret_cast<float2*>(smem + lay.lam);
-    f.offtab = reinterpret_cast<uint64_t*>(smem + lay.offtab);
-    f.lds_taps = reinterpret_cast<double*>(smem + lay.taps);
-    for (uint32_t h = f.tid; h < (1u << H); h += f.NT) f.offtab[h] = pdep64(h, pass.local_hi_mask);
-    __syncthreads();
-    f.wave = int(f.tid >> 6);
-    f.n_waves = int((f.NT + 63) >> 6);
-    f.wave_width = f.NT < 64 ? int(f.NT) : 64;
-    return f;
-}
-
-// Point the frame at unit u = sample * n_tiles + tile (n_tiles a power of two).  The thread
-// index is laundered per unit: the LDS addresses of every group derive from it, and hoisted out
-// of the unit loop they would all stay live in registers at once.
-__device__ __forceinline__ void stream_unit(TileFrame<float>& f, const TilePassArgs& args, const DevPass& pass,
-                                            uint64_t u) {
-    uint32_t tid = f.tid;
-    asm volatile("" : "+v"(tid));
-    f.tid = tid;
-    const int tile_bits = __builtin_ctzll(args.n_tiles);
-    f.tile = u & (args.n_tiles - 1);
-    f.sample = u >> tile_bits;
-    f.base = pdep64(f.tile, pass.nonlocal_mask);
-    f.g_psi = static_cast<float2*>(args.psi) + f.sample * args.stride + f.base;
-    f.g_lam = static_cast<float2*>(args.lam) + f.sample * args.stride + f.base;
-}
-
-// Issue the HBM loads of the frame's tile into registers (no wait).
-__device__ __forceinline__ void stream_fetch(Prefetch64& r, const TileFrame<float>& f) {
-    const uint32_t cmask = (1u << f.C) - 1u;
-#pragma unroll
-    for (int j = 0; j < kAmpsPerThread / 2; j++) {
-        const uint32_t u = 2u * (f.tid + f.NT * j);
-        const uint64_t o = (u & cmask) | f.offtab[u >> f.C];
-        r.p[j] = ld_stream(reinterpret_cast<const float4*>(f.g_psi + o));
-        r.l[j] = ld_stream(reinterpret_cast<const float4*>(f.g_lam + o));
-    }
-}
-
-// Registers -> LDS tiles (the layout tile_load writes).
-__device__ __forceinline__ void stream_stage(const Prefetch64& r, const TileFrame<float>& f) {
-#pragma unroll
-    for (int j = 0; j < kAmpsPerThread / 2; j++) {
-        const uint32_t a = swz<float>(2u * (f.tid + f.NT * j));
-        f.t_psi[a] = make_float2(r.p[j].x, r.p[j].y);
-        f.t_psi[a ^ 1u] = make_float2(r.p[j].z, r.p[j].w);
-        f.t_lam[a] = make_float2(r.l[j].x, r.l[j].y);
-        f.t_lam[a ^ 1u] = make_float2(r.l[j].z, r.l[j].w);
     }
 }
 

@@ -154,10 +154,8 @@ std::atomic<int64_t> g_compile_us{0};
 }
 double jit_compile_seconds_total() { return double(g_compile_us.load()) * 1e-6; }
 
-std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm form, bool wave_local_relays) {
-    const bool persistent = form == JitForm::kPersistent, sb = form == JitForm::kSingleBuffer;
-    if (persistent && !jit_can_persist(pl, dtype))
-        throw std::invalid_argument("code generation: persistent passes are complex64 back-sweeps only");
+std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm form) {
+    const bool sb = form == JitForm::kSingleBuffer;
     if (sb && !jit_can_single_buffer(pl, dtype))
         throw std::invalid_argument(
             "code generation: single-buffer passes are back-sweeps without dense 3- or 4-qubit blocks");
@@ -182,21 +180,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
       << "  pass.low_run = " << ps.low_run << ";\n"
       << "  const T* pool = (const T*)as_const(static_cast<const T*>(args.pool));\n"
       << "  (void)pool;\n";
-    if (persistent) {
-        o << "  TileFrame<T> f = stream_setup<" << pl.K << ">(pass, smem);\n"
-          << "  const uint64_t n_units = args.n_tiles * args.batch;\n"
-          << "  uint64_t unit = blockIdx.x;\n"
-          << "  Prefetch64 pre;\n"
-          << "  if (unit < n_units) {\n    stream_unit(f, args, pass, unit);\n    stream_fetch(pre, f);\n  }\n"
-          << "  for (; unit < n_units; unit += gridDim.x) {\n"
-          << "  stream_unit(f, args, pass, unit);\n"
-          << "  __syncthreads();  // the previous unit's LDS reads are done\n"
-          << "  stream_stage(pre, f);\n"
-          << "  __syncthreads();\n"
-          << "  if (unit + gridDim.x < n_units) {  // next unit's tiles: in flight during this unit's groups\n"
-          << "    TileFrame<T> fn = f;\n    stream_unit(fn, args, pass, unit + gridDim.x);\n"
-          << "    stream_fetch(pre, fn);\n  }\n";
-    } else if (sb) {
+    if (sb) {
         o << "  TileFrame<T> f = sb_begin<T, " << pl.K << ">(args, pass, smem);\n";
     } else {
         o << "  TileFrame<T> f = tile_begin<T, " << A << ", " << pl.K << ">(args, pass, smem);\n";
@@ -234,36 +218,17 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         o << "  __syncthreads();\n  tile_load<T>(f.t_psi, f.g_lam, f.offtab, f.C, f.tid, f.NT);\n  __syncthreads();\n";
         sb_move("load", 0, "l");
     }
-    // Thread t of a group holds the tile indices whose non-register positions, in increasing
-    // order, spell t (group_ctx); its wave index takes the top K - 10 of them.  Two layouts that
-    // agree on those positions give every wave the same indices, and so the same LDS slots: the
-    // re-lay between them only has to order each wave's own accesses.
-    const int wave_bits = std::max(0, pl.K - kGroupWidth - 6);
-    auto wave_positions = [&](const DevGroup& g) {
-        uint32_t m = 0;
-        for (int q = pl.K - 1, need = wave_bits; q >= 0 && need > 0; q--) {
-            if (q == g.pos[0] || q == g.pos[1] || q == g.pos[2] || q == g.pos[3]) continue;
-            m |= 1u << q;
-            need--;
-        }
-        return m;
-    };
-    auto wave_local = [&](const DevGroup& a, const DevGroup& b) {
-        return wave_local_relays && wave_positions(a) == wave_positions(b);
-    };
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
         if (sb) {
             if (gi > 0) {  // re-lay psi, then lambda, from group gi - 1's layout into group gi's
-                const DevGroup& prev = pl.dev_groups.at(size_t(ps.group_begin + gi - 1));
-                const char* sync = wave_local(prev, g) ? "  wave_sync();\n" : "  __syncthreads();\n";
-                o << sync;
+                o << "  __syncthreads();\n";
                 sb_move("store", gi - 1, "p");
-                o << sync;
+                o << "  __syncthreads();\n";
                 sb_move("load", gi, "p");
-                o << sync;
+                o << "  __syncthreads();\n";
                 sb_move("store", gi - 1, "l");
-                o << sync;
+                o << "  __syncthreads();\n";
                 sb_move("load", gi, "l");
             }
             o << "  {  // group " << gi << "\n"
@@ -364,20 +329,9 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
             o << "  }\n";
             continue;
         }
-        // the next group reads the tile in its own layout: a workgroup barrier, unless every wave
-        // keeps its tile indices (dense blocks, which read the tile themselves, keep the barrier)
-        bool local = false;
-        if (!persistent && gi + 1 < ps.n_groups) {
-            const DevGroup& next = pl.dev_groups.at(size_t(ps.group_begin + gi + 1));
-            local = wave_local(g, next);
-            for (const DevGroup* h : {&g, &next})
-                for (int oi = 0; oi < h->n_ops; oi++) {
-                    const int k = pl.dev_ops.at(size_t(h->op_begin + oi)).kind;
-                    if (k == OPK_M3 || k == OPK_M4) local = false;
-                }
-        }
+        // the next group reads the tile in its own layout
         o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
-          << (local ? "    wave_sync();\n" : "    __syncthreads();\n")
+          << "    __syncthreads();\n"
           << "  }\n";
     }
     if (sb) {  // psi, then lambda, back through the buffer to HBM; the tap partials
@@ -392,7 +346,6 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
     } else {
         o << "  tile_end<T, " << A << ">(args, pass, f);\n";
     }
-    if (persistent) o << "  }\n";
     o << "}\n"
       << "}  // namespace mqa\n";
     return o.str();
@@ -427,12 +380,8 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log) {
     return ok;
 }
 
-JitPlan::JitPlan(const Plan& pl, int dtype, JitForm form, bool wave_local)
-    : plan_(pl), dtype_(dtype), form_(form), wave_local_(wave_local) {
-    if ((form_ == JitForm::kPersistent && !jit_can_persist(pl, dtype)) ||
-        (form_ == JitForm::kSingleBuffer && !jit_can_single_buffer(pl, dtype)))
-        form_ = JitForm::kTile;
-    persistent_ = form_ == JitForm::kPersistent;
+JitPlan::JitPlan(const Plan& pl, int dtype, JitForm form) : plan_(pl), dtype_(dtype), form_(form) {
+    if (form_ == JitForm::kSingleBuffer && !jit_can_single_buffer(pl, dtype)) form_ = JitForm::kTile;
     code_.resize(plan_.dev_passes.size());
     worker_ = std::thread([this] { compile_all(); });
 }
@@ -455,7 +404,7 @@ void JitPlan::compile_all() {
     std::vector<std::string> src(static_cast<size_t>(n)), err(static_cast<size_t>(n));
     std::vector<char> done(static_cast<size_t>(n), 0);
     try {
-        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, form_, wave_local_);
+        for (int p = 0; p < n; p++) src[size_t(p)] = jit_pass_source(plan_, p, dtype_, form_);
     } catch (const std::exception& e) {
         std::lock_guard<std::mutex> lk(mu_);
         error_ = e.what();
@@ -532,72 +481,43 @@ std::string JitPlan::error() const {
     return error_;
 }
 
-hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s,
-                           unsigned max_blocks) {
+hipError_t JitPlan::launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
     if (!ready()) return hipErrorNotReady;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     hipFunction_t fn = nullptr;
-    unsigned resident = 0;
     {
         std::lock_guard<std::mutex> lk(mu_);
         auto& fns = functions_[dev];
         if (fns.empty()) {
             auto& mods = modules_[dev];
             mods.assign(code_.size(), nullptr);
-            fns.assign(code_.size(), nullptr);
+            std::vector<hipFunction_t> loaded(code_.size(), nullptr);
             for (size_t p = 0; p < code_.size(); p++) {
                 e = hipModuleLoadData(&mods[p], code_[p].data());
-                if (e == hipSuccess) e = hipModuleGetFunction(&fns[p], mods[p], jit_kernel_name(plan_.adjoint));
+                if (e == hipSuccess) e = hipModuleGetFunction(&loaded[p], mods[p], jit_kernel_name(plan_.adjoint));
                 if (e != hipSuccess) {
-                    fns.clear();
+                    // unload what did load and mark the plan failed: callers use the interpreted
+                    // kernel from now on instead of retrying the load
+                    for (auto& m : mods)
+                        if (m) (void)hipModuleUnload(m), m = nullptr;
+                    mods.clear();
+                    error_ = std::string("module load: ") + hipGetErrorString(e);
+                    state_.store(kFailed);
                     return e;
                 }
             }
+            fns = std::move(loaded);
         }
         fn = fns.at(size_t(pass));
-        if (persistent_) {
-            // one block per resident slot: blocks that could not start at once would run their
-            // whole share of units after the others
-            auto& grid = grid_[dev];
-            if (grid.empty()) grid.assign(code_.size(), 0u);
-            if (grid[size_t(pass)] == 0) {
-                const int nt = int(1u << (plan_.K - kGroupWidth));
-                int per_cu = 0, cus = 0;
-                e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-                if (e != hipSuccess) return e;
-                if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds) != hipSuccess) {
-                    (void)hipGetLastError();
-                    // from the kernel's registers and LDS: 512 VGPRs per SIMD lane, 4 SIMDs, 160 KiB
-                    int regs = 0;
-                    const int waves = std::max(1, nt / 64);
-                    if (hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) == hipSuccess && regs > 0) {
-                        const int waves_per_simd = 512 / ((regs + 7) / 8 * 8);
-                        per_cu = std::min(waves_per_simd * 4 / waves, int((160u << 10) / std::max<size_t>(lds, 1)));
-                    } else {
-                        (void)hipGetLastError();
-                        per_cu = 1;
-                    }
-                }
-                grid[size_t(pass)] = unsigned(std::max(1, per_cu) * std::max(1, cus));
-            }
-            resident = grid[size_t(pass)];
-        }
     }
     TilePassArgs args = a;
     args.batch = batch;
     void* params[] = {&args};
     const unsigned nt = 1u << (plan_.K - kGroupWidth);
-    if (persistent_) {
-        const uint64_t units = a.n_tiles * batch;
-        unsigned blocks = unsigned(std::min<uint64_t>(units, resident));
-        if (max_blocks > 0) blocks = std::min(blocks, max_blocks);
-        return hipModuleLaunchKernel(fn, blocks, 1, 1, nt, 1, 1, unsigned(lds), s, params, nullptr);
-    }
-    e = hipModuleLaunchKernel(fn, unsigned(a.n_tiles), unsigned(batch), 1, nt, 1, 1, unsigned(lds), s, params,
-                              nullptr);
-    return e;
+    return hipModuleLaunchKernel(fn, unsigned(a.n_tiles), unsigned(batch), 1, nt, 1, 1, unsigned(lds), s, params,
+                                 nullptr);
 }
 
 }  // namespace mqa

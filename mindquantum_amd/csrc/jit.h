@@ -28,21 +28,16 @@ const char* jit_kernel_name(bool adjoint);
 // HIP source of pass `pass` of a plan (dtype 0: complex64, 1: complex128).  It includes
 // "device_ops.h" and defines one extern "C" kernel (jit_kernel_name) taking TilePassArgs.
 // Forms of a compiled pass: kTile, one workgroup per (tile, sample) with the tiles in LDS (every
-// plan); kPersistent (complex64 back-sweeps), blocks walk (tile, sample) units with a stride of
-// gridDim.x and prefetch the next unit's tiles into registers; kSingleBuffer (back-sweeps), psi and
-// lambda stay in registers between register groups and share one LDS tile buffer (device_ops.h).
-enum class JitForm { kTile = 0, kPersistent = 1, kSingleBuffer = 2 };
-// wave_local: a register group's re-lay between two layouts that keep every wave's tile indices
-// orders the wave's own LDS accesses instead of taking workgroup barriers (device_ops.h wave_sync).
-std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile,
-                            bool wave_local = false);
+// plan); kSingleBuffer (back-sweeps), psi and lambda stay in registers between register groups and
+// share one LDS tile buffer (device_ops.h).
+enum class JitForm { kTile = 0, kSingleBuffer = 2 };
+std::string jit_pass_source(const Plan& pl, int pass, int dtype, JitForm form = JitForm::kTile);
 
 // CoefForm of every angle slot a plan's ops read (index = slot; program.h): the layout of the
 // coefficient table its compiled passes take.  Empty when no op has an angle.
 std::vector<uint8_t> jit_coef_forms(const Plan& pl);
 
-// Whether a plan's compiled passes may take a form.
-inline bool jit_can_persist(const Plan& pl, int dtype) { return pl.adjoint && dtype == 0; }
+// Whether a plan's compiled passes may take the single-buffer form.
 inline bool jit_can_single_buffer(const Plan& pl, int dtype) {
     (void)dtype;
     if (!pl.adjoint) return false;
@@ -60,7 +55,7 @@ bool jit_compile(const std::string& src, std::string& code, std::string& log);
 class JitPlan {
   public:
     // Copies the plan and starts compiling every pass on a background thread.
-    JitPlan(const Plan& pl, int dtype, JitForm form = JitForm::kTile, bool wave_local = false);
+    JitPlan(const Plan& pl, int dtype, JitForm form = JitForm::kTile);
     ~JitPlan();
     JitPlan(const JitPlan&) = delete;
     JitPlan& operator=(const JitPlan&) = delete;
@@ -70,13 +65,11 @@ class JitPlan {
     void wait();                       // block until compiled (or failed)
     std::string error() const;
     double compile_seconds() const { return seconds_; }
-    bool persistent() const { return persistent_; }
     JitForm form() const { return form_; }
 
-    // Launch pass `pass` on the current device and stream (modules load on first use).
-    // max_blocks > 0 caps the grid of a persistent pass.
-    hipError_t launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s,
-                      unsigned max_blocks = 0);
+    // Launch pass `pass` on the current device and stream (modules load on first use; a load
+    // failure unloads what loaded and marks the plan failed).
+    hipError_t launch(int pass, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s);
 
   private:
     enum { kCompiling = 0, kReady = 1, kFailed = 2 };
@@ -85,9 +78,6 @@ class JitPlan {
     Plan plan_;
     int dtype_;
     JitForm form_;
-    bool wave_local_;
-    bool persistent_;
-    std::map<int, std::vector<unsigned>> grid_;  // per device: resident blocks of each persistent pass
     std::atomic<int> state_{kCompiling};
     std::vector<std::string> code_;
     std::string error_;

@@ -177,7 +177,7 @@ struct TilePassArgs {
     int n_tap_ids;
     const int32_t* tap_ids;
     double* partials;      // [batch][n_tap_ids][n_tiles] Re(tap)
-    uint64_t batch;        // states (persistent passes walk n_tiles * batch units)
+    uint64_t batch;        // states
     const void* coefs;     // [batch][n_slots][2] in the real type, per CoefForm (compiled passes)
 };
 

@@ -6,12 +6,16 @@ sum coeff * params[index]`` terms, fixed gates their kind or a dense matrix.  Co
 (U3, FSim, Givens, SWAPalpha, Rxy/Rxz/Ryz, RotPauliString, GroupedPauli) are expanded into
 their exact lowering first (``BasicGate.lowered``), so gate indices refer to that expansion.
 """
+import collections
 import ctypes
+import hashlib
+import weakref
 
 import numpy as np
 
 from . import _native
 from .core import gates as G
+from .core.mutation import epoch
 from .core.parameterresolver import ParameterResolver
 
 
@@ -60,12 +64,23 @@ def lower(circ, names):
             len(mats), rg)
 
 
+def digest(arrays):
+    """Content digest of lowered arrays (gate records, angle terms, matrices, no-grad flags)."""
+    gates, tp, tc, mats, n_mats, rg = arrays
+    h = hashlib.blake2b(digest_size=16)
+    for a in (gates, tp, tc, mats, rg):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(int(n_mats).to_bytes(8, "little"))
+    return h.digest()
+
+
 class NativeCircuit:
     """C ABI circuit handle for (circuit, parameter order)."""
 
-    def __init__(self, circ, names):
+    def __init__(self, circ, names, arrays=None):
         self.names = list(names)
-        self.arrays = lower(circ, self.names)
+        self.arrays = lower(circ, self.names) if arrays is None else arrays
+        self.digest = digest(self.arrays)
         gates, tp, tc, mats, n_mats, rg = self.arrays
         h = ctypes.c_void_p()
         rgp = _native.ptr(rg) if len(rg) else None
@@ -110,6 +125,50 @@ class NativeCircuit:
         if h is not None and _native._lib is not None:
             _native._lib.mqa_circuit_destroy(h)
             self.handle = None
+
+
+class NativeCache:
+    """Native circuits of Circuit objects, reused across calls.
+
+    The reference lowers its circuit on every call (mqsim.py: ``get_cpp_obj()``).  Here a native
+    circuit (and with it its schedules, device plans and compiled passes) is kept per (circuit
+    object, parameter order): while the global mutation epoch (mindquantum_amd.core.mutation) is
+    unchanged the circuit cannot have changed and the entry is returned at once; after any change
+    anywhere the circuit is lowered again and the entry reused only if the lowered content is
+    identical (a gate replaced, a parameter made no-grad, ... build a new one).  Entries are held
+    weakly by circuit (dropped when the circuit object dies) and at most ``maxsize`` are kept
+    (least recently used first out).
+    """
+
+    def __init__(self, maxsize):
+        self.maxsize = int(maxsize)
+        self._d = collections.OrderedDict()
+
+    def __len__(self):
+        return len(self._d)
+
+    def get(self, circ, names):
+        key = (id(circ), tuple(names))
+        hit = self._d.get(key)
+        now = epoch()
+        if hit is not None and hit[0]() is circ:
+            if hit[2] == now:
+                self._d.move_to_end(key)
+                return hit[1]
+            arrays = lower(circ, names)
+            if digest(arrays) == hit[1].digest:
+                self._d[key] = (hit[0], hit[1], now)
+                self._d.move_to_end(key)
+                return hit[1]
+            nc = NativeCircuit(circ, names, arrays)
+        else:
+            nc = NativeCircuit(circ, names)
+        ref = weakref.ref(circ, lambda _r, k=key, d=self._d: d.pop(k, None) if k in d and d[k][0] is _r else None)
+        self._d[key] = (ref, nc, now)
+        self._d.move_to_end(key)
+        while len(self._d) > self.maxsize:
+            self._d.popitem(last=False)
+        return nc
 
 
 def param_names(circ):

@@ -13,10 +13,11 @@ import numpy as np
 from .. import _native
 from .. import dtype as mqdtype
 from ..core.circuit import Circuit
+from ..core import gates as G
 from ..core.gates import BasicGate
 from ..core.operators import Hamiltonian
-from ..ir import NativeCircuit, resolve
-from .utils import GradOpsWrapper, _thread_balance
+from ..ir import NativeCache, resolve
+from .utils import GradOpsWrapper, check_parallel_worker
 
 
 def _check_encoder(data, n):
@@ -54,7 +55,7 @@ class MQVectorAMD:
         self.seed = int(np.random.randint(1, 2**23)) if seed is None else int(seed)
         self.dtype = mqdtype.to_mq_type(dtype)
         self._code = mqdtype.native_code(self.dtype)
-        self._circ_cache = {}
+        self._circ_cache = NativeCache(64)
         h = ctypes.c_void_p()
         _native.check(_native.lib().mqa_sim_create(self.n_qubits, self._code, self.seed & 0xFFFFFFFF, ctypes.byref(h)))
         self._h = h
@@ -67,7 +68,7 @@ class MQVectorAMD:
         obj.seed = seed
         obj.dtype = mqdtype.to_mq_type(dtype)
         obj._code = mqdtype.native_code(obj.dtype)
-        obj._circ_cache = {}
+        obj._circ_cache = NativeCache(64)
         obj._h = handle
         return obj
 
@@ -85,15 +86,7 @@ class MQVectorAMD:
 
     # ---- circuits
     def _native_circuit(self, circ, names):
-        key = (id(circ), len(circ), tuple(names))
-        hit = self._circ_cache.get(key)
-        if hit is not None and hit[0] is circ:
-            return hit[1]
-        nc = NativeCircuit(circ, names)
-        if len(self._circ_cache) > 64:
-            self._circ_cache.clear()
-        self._circ_cache[key] = (circ, nc)
-        return nc
+        return self._circ_cache.get(circ, names)
 
     def _check_circ(self, circ):
         if not isinstance(circ, Circuit):
@@ -114,10 +107,31 @@ class MQVectorAMD:
                              f"{self.n_qubits}")
 
     def apply_gate(self, gate, pr=None, diff=False):
+        """Apply ``gate`` (mqsim.py:129-160).  With ``diff`` a parameterised gate whose parameters
+        require gradients applies the derivative of its matrix with respect to the angle value on
+        the control subspace and zeroes every amplitude outside it (vector_state.tpp:165-437 with
+        diff; rot_pauli.cpp:431-446 SetToZeroExcept): lowered here to the derivative as a dense
+        matrix with the gate's controls, then diag(0, 1) on every control qubit.  Gates without
+        gradient-requiring parameters apply their matrix, as in the reference."""
         if not isinstance(gate, BasicGate):
             raise TypeError(f"gate requires a quantum gate, but get {type(gate)}")
-        if diff:
-            raise ValueError("the differential form of a gate is not applied on states by this backend")
+        if not gate.obj_qubits:
+            raise ValueError(f"Gate {gate.name} has not been applied on any qubit; use .on()")
+        if diff and getattr(gate, "parameterized", False):
+            if isinstance(gate, G.MultiParamsGate) or not hasattr(gate, "diff_matrix"):
+                # vector_state.tpp:328-366 names U3, Rn, FSim and two-parameter custom gates as not
+                # applicable in differential form
+                raise ValueError(f"Can not apply differential format of {gate.name} gate on quantum states "
+                                 "currently.")
+            if gate.coeff.requires_grad_parameters:
+                if pr is None:
+                    raise ValueError("apply a parameterized gate needs a parameter_resolver")
+                dm = gate.diff_matrix(pr)
+                circ = Circuit([G.UnivMathGate(f"d{gate.name}", dm).on(gate.obj_qubits, gate.ctrl_qubits)])
+                keep_one = np.diag([0.0, 1.0]).astype(np.complex128)
+                for c in gate.ctrl_qubits:
+                    circ += G.UnivMathGate("P1", keep_one).on(c)
+                return self.apply_circuit(circ)
         return self.apply_circuit(Circuit([gate]), pr)
 
     def apply_circuit(self, circuit, pr=None):
@@ -278,8 +292,7 @@ class MQVectorAMD:
             pr_shift = False
         if circ_right.has_measure_gate:
             raise ValueError("circuit for variational algorithm cannot have measure gate")
-        if parallel_worker is not None and not isinstance(parallel_worker, (int, np.integer)):
-            raise TypeError(f"parallel_worker requires a int, but get {type(parallel_worker)}")
+        check_parallel_worker(parallel_worker)
         ansatz_params_name = list(circ_right.all_ansatz.keys())
         encoder_params_name = list(circ_right.all_encoder.keys())
         if non_hermitian:  # mqsim.py:323-331: the left circuit's names follow the right circuit's
@@ -321,7 +334,6 @@ class MQVectorAMD:
                 enc = np.zeros((1, 0), dtype=np.float64)
                 ans = np.ascontiguousarray(inputs[0], dtype=np.float64)
             batch = enc.shape[0]
-            _thread_balance(batch, len(hams), parallel_worker)  # validates like the reference
             if pr_shift:
                 from .param_shift import gradient as shift_gradient
 

@@ -13,26 +13,17 @@ import numpy as np
 from .. import _native
 from .. import dtype as mqdtype
 from ..core.operators import Hamiltonian
-from ..ir import NativeCircuit
+from ..ir import NativeCache
 
-
-# Native circuits (and with them their schedules, device plans and compiled passes) of the
-# circuits a caller passes again, keyed like the simulator's own cache: a circuit object whose
-# length and parameter names are unchanged maps to the same native circuit.
-_NATIVE_CACHE = {}
-_NATIVE_CACHE_MAX = 4096
+# Native circuits (and with them their schedules, device plans and compiled passes) of the circuits
+# a caller passes again: held weakly per circuit object and revalidated after any mutation
+# (ir.NativeCache); the bound keeps a caller that builds fresh circuits per call from holding
+# more than this many.
+_NATIVE_CACHE = NativeCache(1024)
 
 
 def _native_circuit(circ, names):
-    key = (id(circ), len(circ), tuple(names))
-    hit = _NATIVE_CACHE.get(key)
-    if hit is not None and hit[0] is circ:
-        return hit[1]
-    nc = NativeCircuit(circ, names)
-    if len(_NATIVE_CACHE) >= _NATIVE_CACHE_MAX:
-        _NATIVE_CACHE.clear()
-    _NATIVE_CACHE[key] = (circ, nc)
-    return nc
+    return _NATIVE_CACHE.get(circ, names)
 
 
 def expectation_with_grad_batch(circuits, hamiltonians, params, n_qubits=None, dtype=None):

@@ -1,27 +1,17 @@
 """Helpers of the simulator API (mindquantum/simulator/utils.py:18-76)."""
+import numbers
 
 
-def _thread_balance(n_prs, n_meas, parallel_worker):
-    """Split of ``parallel_worker`` threads between samples and Hamiltonians.
-
-    Same arithmetic as the reference (utils.py:18-32).  The HIP backend runs every sample of a
-    batch inside one launch, so the split only validates the argument and is reported back.
-    """
+def check_parallel_worker(parallel_worker):
+    """``parallel_worker`` of get_expectation_with_grad: None or a positive int.  The reference
+    splits that many host threads between samples and Hamiltonians; here every sample of a batch
+    runs inside the same device launches, so the value is validated and otherwise unused."""
     if parallel_worker is None:
-        parallel_worker = n_meas * n_prs
+        return
+    if isinstance(parallel_worker, bool) or not isinstance(parallel_worker, numbers.Integral):
+        raise TypeError(f"parallel_worker requires a int, but get {type(parallel_worker)}")
     if parallel_worker <= 0:
         raise ValueError("parallel_worker must be positive")
-    if n_meas * n_prs <= parallel_worker:
-        batch_threads = n_prs
-        mea_threads = n_meas
-    else:
-        if n_meas < n_prs:
-            batch_threads = min(n_prs, parallel_worker)
-            mea_threads = min(n_meas, max(1, parallel_worker // batch_threads))
-        else:
-            mea_threads = min(n_meas, parallel_worker)
-            batch_threads = min(n_prs, max(1, parallel_worker // mea_threads))
-    return batch_threads, mea_threads
 
 
 class GradOpsWrapper:  # pylint: disable=too-many-instance-attributes

@@ -1,5 +1,5 @@
 """The tile-width autotuner on the GPU: it keeps a candidate, and results are unchanged by the
-width it picks (bars of test_gpu_parity.py)."""
+width it picks (bars of tests/bars.py, here between two runs of the backend)."""
 import numpy as np
 import pytest
 
@@ -7,6 +7,7 @@ import mindquantum_amd as mq
 from mindquantum_amd import _native, autotune, ir
 from mindquantum_amd.algorithm import hardware_efficient_ansatz, layered_random_circuit
 from oracle import cpu
+from tests.bars import assert_f, assert_g, coeff_sum
 from tests.conftest import hea, random_hamiltonian
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +43,6 @@ def test_tune_grad_keeps_gradients(dtype):
     fk, ak = min(times, key=times.get)
     assert grad_ops.native_circuits[0].tile_qubits(mq.dtype.native_code(dtype), True) == ak
     f1, ge1, ga1 = grad_ops(enc, ans)
-    tol = 1e-10 if dtype == mq.complex128 else 2e-5
-    assert np.max(np.abs(f1 - f0)) < tol
-    assert np.max(np.abs(np.concatenate([ge1, ga1], 2) - np.concatenate([ge0, ga0], 2))) < tol * 10
+    assert_f(f1, f0, dtype, coeff_sum(op))
+    assert_g(np.concatenate([ge1, ga1], 2), np.concatenate([ge0, ga0], 2), dtype)
 

@@ -1,8 +1,8 @@
 """GPU parity of the lowered gate families (U3, FSim, Givens, SWAPalpha, Rxy, Rxz, Ryz,
 RotPauliString) against the reference's known answer and the dense / CPU oracles.
 
-Bars: the reference test's atol 1e-6 for its known answer; otherwise those of
-test_gpu_parity.py (1e-10 states / 1e-9 gradients in complex128, 1e-5 relative in complex64).
+Bars: the reference test's atol 1e-6 for its known answer; otherwise those of tests/bars.py against
+the complex128 oracle (1e-12 fidelity / 1e-10 relative in complex128, 1e-5 relative in complex64).
 """
 import numpy as np
 import pytest
@@ -11,6 +11,7 @@ import mindquantum_amd as mq
 from mindquantum_amd import ir
 from mindquantum_amd.core import gates as G
 from oracle import cpu, dense
+from tests.bars import assert_f, assert_g, assert_state, coeff_sum
 from tests.test_gate_families import _golden_circuit, family_circuit
 
 pytestmark = pytest.mark.gpu
@@ -44,13 +45,8 @@ def test_family_gradients_gpu(seed, dtype):
     f_ref, g_ref = dense.gradient(circ, dict(zip(names, vals)), names, n, op)
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)(vals)
-    if dtype == mq.complex128:
-        assert abs(f[0, 0] - f_ref) < 1e-10
-        assert np.max(np.abs(g[0, 0].real - g_ref)) < 1e-9
-    else:
-        scale = max(1.0, float(np.max(np.abs(g_ref))))
-        assert abs(f[0, 0] - f_ref) < 5e-5
-        assert np.max(np.abs(g[0, 0].real - g_ref)) < 4e-5 * scale
+    assert_f(f[0, 0], f_ref, dtype, coeff_sum(op))
+    assert_g(g[0, 0].real, g_ref, dtype)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -64,8 +60,4 @@ def test_family_state_16q(dtype):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     sim.apply_circuit(circ, pr=dict(zip(names, vals)))
     ref = cpu.apply_circuit(cpu.zero_state(n, np.complex128), ir.lower(circ, names), vals)
-    got = sim.get_qs()
-    if dtype == mq.complex128:
-        assert 1 - abs(np.vdot(ref, got)) <= 1e-12
-    else:
-        assert np.max(np.abs(got - ref)) <= 1e-5
+    assert_state(sim.get_qs(), ref, dtype)

@@ -2,7 +2,7 @@
 
 Items of different qubit counts, schedules and Hamiltonians run through shared launches; every
 item must match the oracle's per-circuit gradient (the reference's one-call-per-circuit result).
-Bars as in test_gpu_parity.py.
+Bars of tests/bars.py against the complex128 oracle.
 """
 import numpy as np
 import pytest
@@ -11,6 +11,7 @@ import mindquantum_amd as mq
 from mindquantum_amd import _native, ir
 from mindquantum_amd.algorithm import max_cut_ansatz, max_cut_hamiltonian, random_regular_graph
 from oracle import cpu
+from tests.bars import assert_f, assert_g, coeff_sum
 from tests.conftest import hea, random_circuit, random_hamiltonian
 
 pytestmark = pytest.mark.gpu
@@ -18,9 +19,10 @@ pytestmark = pytest.mark.gpu
 DTYPES = [mq.complex64, mq.complex128]
 
 
-def oracle_item(circ, op, n, params, dtype):
+def oracle_item(circ, op, n, params):
+    """(f, g) of one item from the complex128 oracle."""
     names = list(circ.params_name)
-    f, g = cpu.grad_batch(n, dtype, ir.lower(circ, names), [mq.Hamiltonian(op).terms_ir()], np.zeros((1, 0)),
+    f, g = cpu.grad_batch(n, np.complex128, ir.lower(circ, names), [mq.Hamiltonian(op).terms_ir()], np.zeros((1, 0)),
                           np.asarray(params, dtype=np.float64), batch_threads=1)
     return f[0, 0], g[0, 0].real
 
@@ -29,14 +31,11 @@ def check_items(circs, ops, nqs, params, dtype):
     f, grads = mq.expectation_with_grad_batch(circs, [mq.Hamiltonian(o, dtype) for o in ops], params, nqs, dtype)
     assert f.shape == (len(circs),) and len(grads) == len(circs)
     for i, (c, o, n, p) in enumerate(zip(circs, ops, nqs, params)):
-        f_ref, g_ref = oracle_item(c, o, n, p, dtype)
-        scale = max(1.0, sum(abs(v) for v in o.terms.values()))
-        tol = (1e-10 if dtype == mq.complex128 else 2e-5) * scale
-        assert abs(f[i] - f_ref) < tol, f"item {i} (n={n}): f {f[i]} vs {f_ref}"
-        gs = max(1.0, float(np.max(np.abs(g_ref)))) if g_ref.size else 1.0
+        f_ref, g_ref = oracle_item(c, o, n, p)
+        assert_f(f[i], f_ref, dtype, coeff_sum(o), f"f of item {i} (n={n})")
         assert grads[i].shape == g_ref.shape
         if g_ref.size:
-            assert np.max(np.abs(grads[i] - g_ref)) < tol * gs * 4, f"item {i} (n={n})"
+            assert_g(grads[i], g_ref, dtype, f"g of item {i} (n={n})")
 
 
 @pytest.mark.parametrize("dtype", DTYPES)

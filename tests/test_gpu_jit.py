@@ -1,8 +1,8 @@
 """GPU parity of the compiled (code-generated) tile passes against the CPU oracle.
 
 The options force compilation before the first launch (jit = 2) for every launch size
-(jit_min_amps = 0); ``jit_launches`` proves the compiled kernels ran.  Bars as in
-test_gpu_parity.py.
+(jit_min_amps = 0); ``jit_launches`` proves the compiled kernels ran.  Bars of tests/bars.py
+against the complex128 oracle.
 """
 import numpy as np
 import pytest
@@ -11,6 +11,7 @@ import mindquantum_amd as mq
 from mindquantum_amd import _native
 from mindquantum_amd.core import gates as G
 from oracle import dense
+from tests.bars import assert_f, coeff_sum
 from tests.conftest import hea, params_for, random_circuit, random_hamiltonian
 from tests.test_gpu_parity import DTYPES, assert_grad, assert_state, oracle_grad, oracle_state
 
@@ -69,7 +70,7 @@ def test_grad_every_gate_kind_compiled(compiled, dtype):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)(vals)
     f_ref, g_ref = dense.gradient(circ, dict(zip(names, vals)), names, n, op)
-    assert abs(f[0, 0] - f_ref) < (1e-10 if dtype == mq.complex128 else 5e-5)
+    assert_f(f[0, 0], f_ref, dtype, coeff_sum(op))
     assert_grad(g[0, 0].real, g_ref, dtype)
     assert _native.get_option("jit_launches") > compiled
 
@@ -84,8 +85,8 @@ def test_grad_batch_hea_compiled(compiled, dtype):
     ans = rng.uniform(0, 2 * np.pi, size=len(circ.ansatz_params_name))
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g_enc, g_ans = sim.get_expectation_with_grad([mq.Hamiltonian(o, dtype) for o in ops], circ)(enc, ans)
-    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans, dtype)
-    tol = 1e-10 if dtype == mq.complex128 else 2e-5
-    assert np.max(np.abs(f - f_ref)) < tol * 10
+    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans)
+    for h, o in enumerate(ops):
+        assert_f(f[:, h], f_ref[:, h], dtype, coeff_sum(o))
     assert_grad(np.concatenate([g_enc, g_ans], axis=2), g_ref, dtype)
     assert _native.get_option("jit_launches") > compiled

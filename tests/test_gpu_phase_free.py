@@ -5,7 +5,8 @@ with and without them, in the interpreted and in the compiled passes.
 
 Narrow tiles (7 of 12 qubits local) make most controls of parameterised gates non-local, so a
 tapped op whose control is off for a tile sits between other tapped ops of its register group
-(the interpreted pass must still advance its tap slot).  Bars of test_gpu_parity.py.
+(the interpreted pass must still advance its tap slot).  Bars of tests/bars.py against the
+complex128 oracle.
 """
 import numpy as np
 import pytest
@@ -13,6 +14,7 @@ import pytest
 import mindquantum_amd as mq
 from mindquantum_amd import _native
 from mindquantum_amd.core import gates as G
+from tests.bars import assert_f, coeff_sum
 from tests.conftest import hea, params_for, random_circuit, random_hamiltonian
 from tests.test_gpu_parity import DTYPES, assert_grad, oracle_grad
 
@@ -55,8 +57,8 @@ def test_random_circuit_non_local_controls(options, dtype, compiled, phase_free)
     f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)(vals)
     assert _native.get_option("phase_free_used") == int(phase_free)
     assert names == list(circ.ansatz_params_name)
-    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals, dtype)
-    assert abs(f[0, 0] - f_ref[0, 0]) < (1e-10 if dtype == mq.complex128 else 5e-5)
+    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals)
+    assert_f(f[0, 0], f_ref[0, 0], dtype, coeff_sum(op))
     assert_grad(g[0, 0].real, g_ref[0, 0], dtype)
     assert (_native.get_option("jit_launches") > before) == compiled
 
@@ -80,9 +82,9 @@ def test_hea_batch_with_global_phases_and_checkpoints(options, dtype, compiled):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g_enc, g_ans = sim.get_expectation_with_grad([mq.Hamiltonian(o, dtype) for o in ops], circ)(enc, ans)
     assert _native.get_option("checkpoints_used") > 0
-    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans, dtype)
-    tol = 1e-10 if dtype == mq.complex128 else 2e-5
-    assert np.max(np.abs(f - f_ref)) < tol * 10
+    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans)
+    for h, o in enumerate(ops):
+        assert_f(f[:, h], f_ref[:, h], dtype, coeff_sum(o))
     assert_grad(np.concatenate([g_enc, g_ans], axis=2), g_ref, dtype)
 
 
@@ -173,8 +175,8 @@ def test_rzz_phase_free_gradient_and_state(options, dtype, compiled):
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype)
     f, g = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)(vals)
     assert _native.get_option("phase_free_used") == 1
-    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals, dtype)
-    assert abs(f[0, 0] - f_ref[0, 0]) < (1e-10 if dtype == mq.complex128 else 5e-5)
+    f_ref, g_ref = oracle_grad(circ, n, [op], np.zeros((1, 0)), vals)
+    assert_f(f[0, 0], f_ref[0, 0], dtype, coeff_sum(op))
     assert_grad(g[0, 0].real, g_ref[0, 0], dtype)
     sim2 = mq.Simulator("mqvector_amd", n, dtype=dtype)
     pr = dict(zip(names, vals))

@@ -1,7 +1,7 @@
 """Single-buffer compiled back-sweep passes (option ``jit_single_buffer``, complex64): psi and
 lambda stay in registers between register groups and share one LDS tile buffer.  Gradients must
-match the oracle (bars of test_gpu_parity.py) with exact and phase-free plans, narrow and default
-tiles, checkpoints, controls that cross the tile, and a dense 3-qubit block (which keeps the
+match the complex128 oracle (bars of tests/bars.py) with exact and phase-free plans, narrow and
+default tiles, checkpoints, controls that cross the tile, and a dense 3-qubit block (which keeps the
 two-buffer form for its circuit).
 """
 import numpy as np
@@ -10,12 +10,13 @@ import pytest
 import mindquantum_amd as mq
 from mindquantum_amd import _native
 from mindquantum_amd.core import gates as G
+from tests.bars import assert_f, coeff_sum
 from tests.conftest import hea, random_hamiltonian
 from tests.test_gpu_parity import assert_grad, oracle_grad
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ("jit", "jit_min_amps", "phase_free", "checkpoints", "jit_single_buffer", "jit_wave_local")
+KEYS = ("jit", "jit_min_amps", "phase_free", "checkpoints", "jit_single_buffer")
 
 
 @pytest.fixture
@@ -46,21 +47,18 @@ def run(circ, n, ops, batch, seed, tiles=None):
     ans = rng.uniform(-2 * np.pi, 2 * np.pi, size=len(circ.ansatz_params_name))
     sim = mq.Simulator("mqvector_amd", n, dtype=mq.complex64)
     f, g_enc, g_ans = sim.get_expectation_with_grad([mq.Hamiltonian(o, mq.complex64) for o in ops], circ)(enc, ans)
-    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans, mq.complex64)
-    assert np.max(np.abs(f - f_ref)) < 2e-4
+    f_ref, g_ref = oracle_grad(circ, n, ops, enc, ans)
+    for h, o in enumerate(ops):
+        assert_f(f[:, h], f_ref[:, h], mq.complex64, coeff_sum(o))
     assert_grad(np.concatenate([g_enc, g_ans], axis=2), g_ref, mq.complex64)
 
 
-@pytest.mark.parametrize("wave_local", [0, 1])
 @pytest.mark.parametrize("phase_free", [False, True])
 @pytest.mark.parametrize("tile", [8, 11, 12, 13])
-def test_single_buffer_gradient_matches_oracle(options, phase_free, tile, wave_local):
-    """Tiles of one partial wave and of 2, 4 and 8 waves, with workgroup barriers on every re-lay
-    and (option jit_wave_local) without them between layouts that keep each wave's indices (on
-    every re-lay of the one-wave tiles)."""
+def test_single_buffer_gradient_matches_oracle(options, phase_free, tile):
+    """Tiles of one partial wave and of 2, 4 and 8 waves."""
     n = 14
     _native.set_option("phase_free", int(phase_free))
-    _native.set_option("jit_wave_local", wave_local)
     before = _native.get_option("jit_single_buffer_launches")
     run(circuit(n), n, [random_hamiltonian(n, 9, 4), mq.QubitOperator("X0 Y5") + 0.3 * mq.QubitOperator("Z13")],
         4, tile, tiles=(12, tile))

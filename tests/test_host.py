@@ -5,7 +5,7 @@ import pytest
 import mindquantum_amd as mq
 from mindquantum_amd import _native, ir
 from mindquantum_amd.core import gates as G
-from mindquantum_amd.simulator.utils import _thread_balance
+from mindquantum_amd.simulator.utils import check_parallel_worker
 
 
 def test_parameter_resolver_algebra_and_flags():
@@ -71,7 +71,91 @@ def test_ir_struct_layout():
     assert list(rg) == [1]
 
 
-def test_thread_balance_matches_reference_rule():
-    assert _thread_balance(1024, 1, None) == (1024, 1)
-    assert _thread_balance(10, 2, 4) == (4, 1)
-    assert _thread_balance(2, 4, 3) == (1, 3)
+def test_parallel_worker_validation():
+    check_parallel_worker(None)
+    check_parallel_worker(4)
+    with pytest.raises(TypeError):
+        check_parallel_worker(2.0)
+    with pytest.raises(ValueError):
+        check_parallel_worker(0)
+
+
+def test_native_cache_sees_circuit_and_parameter_changes():
+    """A native circuit is reused only while the circuit's lowered content is unchanged: making a
+    parameter no-grad, replacing a gate, or inserting then removing one builds a new native circuit
+    (or reuses the old one when the content is back to what it was)."""
+    from mindquantum_amd.ir import NativeCache
+
+    cache = NativeCache(8)
+    c = mq.Circuit().rx("a", 0).ry("b", 1)
+    names = ["a", "b"]
+    n0 = cache.get(c, names)
+    assert cache.get(c, names) is n0
+    kinds0 = list(n0.arrays[0]["kind"])
+    c.no_grad()
+    n1 = cache.get(c, names)
+    assert n1 is not n0 and list(n1.arrays[5]) == [0, 0]
+    c.requires_grad()
+    c[0] = G.RZ("a").on(0)
+    n2 = cache.get(c, names)
+    assert list(n2.arrays[0]["kind"]) != kinds0 and list(n2.arrays[5]) == [1, 1]
+    c.insert(1, G.H.on(1))
+    n3 = cache.get(c, names)
+    assert len(n3.arrays[0]) == 3
+    c.pop(1)
+    n4 = cache.get(c, names)
+    assert n4.digest == n2.digest and len(n4.arrays[0]) == 2
+    c[0].coeff.no_grad_part("a")
+    assert list(cache.get(c, names).arrays[5]) == [0, 1]
+    # entries die with their circuit; the size bound holds
+    del c
+    import gc
+
+    gc.collect()
+    assert len(cache) == 0
+    keep = [mq.Circuit().rx(f"p{i}", 0) for i in range(12)]
+    for k in keep:
+        cache.get(k, list(k.params_name))
+    assert len(cache) == 8
+
+
+def test_grad_workspaces_never_outgrow_the_device():
+    """Sequences of batched-gradient calls on one simulator, against a fake device of fixed
+    capacity: each call's budget is 85% of (free + held) as mqa_sim_grad_batch computes it, the
+    planned releases happen first and every allocation must then fit.  Before the plan released
+    stale buffers, a one-Hamiltonian call that filled the budget followed by a two-Hamiltonian call
+    (which keeps a copy of the forward state) allocated the copy on top of the oversized psi and
+    lambda buffers."""
+    cap = 256 << 30
+    sb = 1 << 30
+    side = 4096
+    held = [0, 0, 0, 0]
+
+    def call(n_hams, ck, batch):
+        free = cap - sum(held)
+        budget = int(0.85 * (free + sum(held)))
+        chunk, keep, need, release = _native.grad_workspace_plan(budget, sb, side, n_hams, ck, batch, held)
+        for i in range(4):
+            if release[i]:
+                held[i] = 0
+        for i in range(4):
+            if need[i] > held[i]:  # DevBuf::ensure frees the old buffer, then allocates
+                held[i] = 0
+                assert sum(held) + need[i] <= cap, f"allocation of {need[i]} does not fit"
+                held[i] = need[i]
+        return chunk, keep
+
+    c1, k1 = call(1, 0, 4096)
+    assert not k1 and c1 == int(0.85 * cap) // (2 * sb + side)
+    c2, k2 = call(2, 0, 4096)
+    assert k2 and c2 == int(0.85 * cap) // (3 * sb + side)
+    c3, _ = call(2, 0, 4096)  # repeated: same plan, nothing released
+    assert c3 == c2
+    before = list(held)
+    chunk, keep, need, release = _native.grad_workspace_plan(int(0.85 * cap), sb, side, 2, 0, 4096, held)
+    assert not any(release) and need == before
+    call(1, 4, 4096)  # checkpoints: the kept copy goes, the checkpoint buffer comes
+    assert held[2] == 0 and held[3] > 0
+    call(1, 0, 8)  # a small batch keeps the oversized psi / lambda buffers (they fit the budget)
+    assert held[0] >= 8 * sb and held[3] == 0
+    call(3, 0, 100000)
