@@ -7,14 +7,17 @@ qubit, CNOT ladder i -> i+1; 320 parameters ~ U[0, 2pi)), H = sum_i w_i Z_i Z_{i
 sum_i v_i X_i with w, v ~ U(-1, 1) (40 terms).  One step = one forward + backward of the
 torch.autograd operator (MQOps) over the whole batch; the adjoint sweep runs in the forward.
 
-Also reported in the same JSON line: the same workload in complex128; 28-qubit gate
-application (layered random circuit, depth 200, complex64 / complex128) in HBM GB/s, fused
-(tile passes) and unfused (one gate per pass); the uneven batch of 512 MaxCut QAOA circuits
-(12..24 qubits, p = 4, complex64, one mqa_grad_items call); one 32-qubit complex64 and one
-31-qubit complex128 gradient (HEA-4, 64 random Pauli strings, 4 checkpoints).  ``roofline`` is the dominant kernel of the
-headline (the adjoint tile pass) timed with HIP events on the library's stream;
-``cpu_baseline`` times the CPU oracle (a restatement of the reference CPU mqvector path) on a
-bounded sample of the same workload on this host's cores.
+Also reported in the same JSON line, each with its error against the complex128 oracle (or, where no
+oracle runs at that size, the property the mathematics fixes): the same workload in complex128; the
+16-qubit HEA-10 complex128 single-vector gradient with the CPU oracle beside it; 28-qubit gate
+application (layered random circuit, depth 200, complex64 / complex128) in HBM GB/s, fused (tile
+passes) and unfused (one gate per pass), with the CPU oracle on a 2-layer sample; the uneven batch
+of 512 MaxCut QAOA circuits (12..24 qubits, p = 4, complex64, one mqa_grad_items call); one 32-qubit
+complex64 and one 31-qubit complex128 gradient (HEA-4, 64 random Pauli strings, 4 checkpoints) and
+their 24-qubit analog beside the CPU oracle.  ``roofline`` is the dominant kernel of the headline (the
+adjoint tile pass) timed with HIP events on the library's stream; ``cpu_baseline`` times the CPU
+oracle (a restatement of the reference CPU mqvector path) on 64 samples of the headline workload in
+the reference's three thread layouts on this host's cores.
 """
 import argparse
 import json
@@ -31,6 +34,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel, measured with FETCH_SIZE / WRITE_SIZE counter runs
 # scaled by a copy kernel of known size (the summary file names the commands and the factors)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_20q_b1024_c64_hbm_counters.md")
+METRIC = "adjoint grads/s (20q, batch 1024, c64 vs c128); 28q gate-apply HBM GB/s"
+# sampled rows of the headline batch checked against the complex128 oracle
+ERR_ROWS = np.array([0, 1, 2, 3, 97, 255, 256, 400, 511, 512, 640, 777, 900, 1000, 1022, 1023])
 
 
 def measured_traffic():
@@ -42,39 +48,50 @@ def measured_traffic():
     except OSError:
         pass
     return None
-METRIC = "adjoint grads/s (20q, batch 1024, c64 vs c128); 28q gate-apply HBM GB/s"
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def adjoint_workload(n, layers, batch, seed):
+def host_cores():
+    cores = len(os.sched_getaffinity(0))
+    return min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+
+
+def coeff_sum(op):
+    return float(sum(abs(complex(v)) for v in op.terms.values()))
+
+
+def err_stats(f, g, f_ref, g_ref, csum):
+    """Error against a complex128 reference: |f - f_ref| / max(1, sum |c|) and
+    max |g - g_ref| / max |g_ref|."""
+    f_err = float(np.max(np.abs(np.asarray(f) - np.asarray(f_ref))))
+    g_err = float(np.max(np.abs(np.asarray(g) - np.asarray(g_ref))))
+    g_max = float(np.max(np.abs(g_ref)))
+    return {"f_abs": f_err, "f_rel_sum_c": f_err / max(1.0, csum), "g_abs": g_err, "g_rel_max_g": g_err / g_max}
+
+
+def oracle_grad(circ, n, op, enc, ans, threads):
+    from mindquantum_amd import ir
     import mindquantum_amd as mq
-    from mindquantum_amd.algorithm import hardware_efficient_ansatz
-    from mindquantum_amd.algorithm.builders import encoder_rx
+    from oracle import cpu
 
-    rng = np.random.default_rng(seed)
-    circ = encoder_rx(n) + hardware_efficient_ansatz(n, layers)
-    op = mq.QubitOperator("", 0)
-    w = rng.uniform(-1, 1, size=n)
-    v = rng.uniform(-1, 1, size=n)
-    for i in range(n):
-        op += mq.QubitOperator(f"Z{i} Z{(i + 1) % n}", float(w[i]))
-        op += mq.QubitOperator(f"X{i}", float(v[i]))
-    enc = rng.normal(size=(batch, n))
-    ans = rng.uniform(0, 2 * np.pi, size=len(circ.ansatz_params_name))
-    return circ, op, enc, ans
+    names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
+    return cpu.grad_batch(n, np.complex128, ir.lower(circ, names), [mq.Hamiltonian(op).terms_ir()], enc, ans,
+                          batch_threads=threads, worker_threads=1)
 
 
+# ---------------------------------------------------------------- headline
 def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile, tune=False):
     import torch
 
     import mindquantum_amd as mq
     from mindquantum_amd import _native, autotune
+    from mindquantum_amd.algorithm import batched_adjoint
     from mindquantum_amd.framework import MQOps
 
-    circ, op, enc, ans = adjoint_workload(n, layers, batch, seed)
+    circ, op, enc, ans = batched_adjoint(n, layers, batch, seed)
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
     tiles = None
@@ -112,12 +129,72 @@ def run_adjoint(dtype, n, layers, batch, steps, warmup, seed, profile, tune=Fals
         _native.set_option("profile", 0)
         for k in ("tile_pass_fwd", "tile_pass_adj", "hamiltonian"):
             stats[k] = _native.profile(k)
-    n_params = len(circ.params_name)
-    if tune:
-        stats["tiles"] = tiles
-    return batch * steps / dt, dt / steps * 1e3, stats, n_params, len(circ)
+    # the values the timed steps computed on the sampled rows (the float32 inputs MQOps passes on)
+    rows = ERR_ROWS[ERR_ROWS < batch]
+    x32 = x.detach().numpy().astype(np.float64)
+    a32 = a.detach().numpy().astype(np.float64)
+    f, g_enc, g_ans = grad_ops(x32[rows], a32)
+    sample = {"rows": rows, "enc": x32[rows], "ans": a32, "f": f[:, 0], "g": np.concatenate([g_enc[:, 0],
+                                                                                          g_ans[:, 0]], 1).real,
+              "circ": circ, "op": op}
+    return {"grads_per_s": batch * steps / dt, "ms_per_step": dt / steps * 1e3, "stats": stats, "tiles": tiles,
+            "n_params": len(circ.params_name), "n_gates": len(circ), "sample": sample}
 
 
+def headline_errors(samples, n):
+    """Errors of the sampled rows of each precision against the complex128 oracle on the same
+    (float32-rounded) inputs."""
+    s0 = samples[0]
+    t0 = time.perf_counter()
+    f_ref, g_ref = oracle_grad(s0["circ"], n, s0["op"], s0["enc"], s0["ans"], min(len(s0["rows"]), host_cores()))
+    log(f"c128 oracle on {len(s0['rows'])} sampled rows: {time.perf_counter() - t0:.1f} s")
+    csum = coeff_sum(s0["op"])
+    out = []
+    for s in samples:
+        e = err_stats(s["f"], s["g"], f_ref[:, 0], g_ref[:, 0].real, csum)
+        e["rows"] = int(len(s["rows"]))
+        out.append(e)
+    return out
+
+
+# ---------------------------------------------------------------- single-vector 16q HEA-10 c128
+def run_hea16(reps, cpu):
+    """The CPU-runnable single-vector gradient (b = 1, complex128) on the GPU, beside the CPU oracle
+    (which the reference runs inline with OpenMP over all cores, gradient.h:271-273)."""
+    import mindquantum_amd as mq
+    from mindquantum_amd.algorithm import hea_open_chain
+
+    circ, op, vals = hea_open_chain(16, 10, 0)
+    sim = mq.Simulator("mqvector_amd", 16, dtype=mq.complex128)
+    grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, mq.complex128), circ)
+    for _ in range(3):
+        f, g = grad_ops(vals)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f, g = grad_ops(vals)
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    res = {"qubits": 16, "layers": 10, "params": len(vals), "terms": len(op.terms), "dtype": "complex128",
+           "gpu_ms_per_grad": round(gpu_ms, 3), "calls_timed": reps}
+    if cpu:
+        from oracle import cpu as ocpu
+        from mindquantum_amd import ir
+
+        cores = host_cores()
+        ocpu.set_num_threads(cores)
+        names = list(circ.params_name)
+        t0 = time.perf_counter()
+        f_ref, g_ref = ocpu.grad_batch(16, np.complex128, ir.lower(circ, names), [mq.Hamiltonian(op).terms_ir()],
+                                       np.zeros((1, 0)), vals, batch_threads=1)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        res.update({"cpu_ms_per_grad": round(cpu_ms, 1), "cpu_cores": cores, "cpu_kind": "port",
+                    "speedup_vs_cpu": round(cpu_ms / gpu_ms, 1),
+                    "err_vs_c128_oracle": err_stats(f[0, 0], g[0, 0].real, f_ref[0, 0], g_ref[0, 0].real,
+                                                    coeff_sum(op))})
+    log(f"16q HEA-10 c128 single vector: {res}")
+    return res
+
+
+# ---------------------------------------------------------------- 28q gate application
 def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     import mindquantum_amd as mq
     from mindquantum_amd import _native, autotune, ir
@@ -128,7 +205,8 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     tuned = None
     if tune:  # untimed: the tile width of the passes, chosen on this circuit
         times = autotune.tune_apply(circ, n, dtype)
-        tuned = {"tile_qubits": circ._mqa_tiles[(mq.dtype.native_code(dtype), False)], "seconds": {k: round(v, 4) for k, v in times.items()}}
+        tuned = {"tile_qubits": circ._mqa_tiles[(mq.dtype.native_code(dtype), False)],
+                 "seconds": {k: round(v, 4) for k, v in times.items()}}
         log(f"autotune apply {n}q {np.dtype(dtype).name}: {tuned}")
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     code = 0 if dtype == mq.complex64 else 1
@@ -146,6 +224,7 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     _native.set_option("profile", 0)
     launches, ms, byts = _native.profile("tile_pass_fwd")
     state = (1 << n) * amp
+    norm = mq.inner_product(sim, sim).real
     res = {
         "gates": len(circ), "passes": passes, "seconds": round(dt, 4),
         "hbm_gbs": round(byts / (ms * 1e-3) / 1e9, 1),
@@ -153,7 +232,7 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
         "gate_equivalent_gbs": round(len(circ) * 2 * state / dt / 1e9, 1),
         "kernel_ms": round(ms, 3), "launches": launches,
         "compiled_passes": _native.get_option("jit") != 0, "compile_seconds_untimed": round(compile_s, 1),
-        "autotune": tuned,
+        "autotune": tuned, "norm_drift": abs(norm - 1.0),
     }
     log(f"gate apply {n}q {np.dtype(dtype).name}: {res}")
     # unfused: one gate per pass, on a shorter prefix of the same circuit.  Single-gate passes
@@ -188,16 +267,39 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
     res["unfused"] = {"gates": len(short), "seconds": round(dt2, 4), "hbm_gbs": round(gbs2, 1),
                       "frac_of_peak": round(gbs2 / HBM_PEAK_GBS, 4), "launches": launches2, "tile_qubits": k_best,
                       "hbm_gbs_by_tile_qubits": {k: round(v[3] / (v[2] * 1e-3) / 1e9, 1) for k, v in per_k.items()}}
-    del sim, sim2
-    return res
+    del sim2
+    return res, sim
 
 
+def gate_apply_cpu(dtype, n, layers, depth, seed):
+    """The CPU oracle on the first `layers` layers of the 28-qubit circuit (every layer costs the
+    same: n one-qubit gates and n / 2 two-qubit gates, one sweep each), scaled to the full depth."""
+    import mindquantum_amd as mq
+    from mindquantum_amd import ir
+    from mindquantum_amd.algorithm import layered_random_circuit
+    from oracle import cpu
+
+    circ = layered_random_circuit(n, layers, seed)
+    cores = host_cores()
+    cpu.set_num_threads(cores)
+    qs = cpu.zero_state(n, dtype)
+    t0 = time.perf_counter()
+    cpu.apply_circuit(qs, ir.lower(circ, []), np.zeros(0))
+    dt = time.perf_counter() - t0
+    del qs
+    return {"seconds_sample": round(dt, 2), "sample": f"{layers} of {depth} layers ({len(circ)} gates)",
+            "seconds_full_depth_extrapolated": round(dt * depth / layers, 1), "cores": cores, "kind": "port"}
+
+
+# ---------------------------------------------------------------- uneven QAOA batch
 def run_uneven_items(count, seed, cpu_sample):
     """Uneven MaxCut batch: `count` QAOA circuits (p = 4) on 12..24 qubits, expectation and
     adjoint gradient of every circuit in complex64 through one mqa_grad_items call (shared
     per-step launches over the tiles of all circuits).  Optionally the CPU oracle on a subset
-    (circuits of at most 18 qubits) beside the GPU on the same subset."""
+    (two circuits of every size from 12 to 20 qubits) beside the GPU on the same subset, and the
+    subset's error against the complex128 oracle."""
     import mindquantum_amd as mq
+    from mindquantum_amd import ir
     from mindquantum_amd.algorithm import uneven_qaoa_batch
 
     circs, ops, nqs, params = uneven_qaoa_batch(count, seed)
@@ -215,7 +317,6 @@ def run_uneven_items(count, seed, cpu_sample):
            "reference_schedule_equivalent_gbs": round(ref_bytes / dt / 1e9, 1)}
     log(f"uneven batch: {res}")
     if cpu_sample:
-        from mindquantum_amd import ir
         from oracle import cpu
 
         idx = []  # two circuits of every size from 12 to 20 qubits
@@ -224,10 +325,9 @@ def run_uneven_items(count, seed, cpu_sample):
         sub = ([circs[i] for i in idx], [hams[i] for i in idx], [params[i] for i in idx], [nqs[i] for i in idx])
         mq.expectation_with_grad_batch(*sub, mq.complex64)
         t0 = time.perf_counter()
-        mq.expectation_with_grad_batch(*sub, mq.complex64)
+        f_sub, g_sub = mq.expectation_with_grad_batch(*sub, mq.complex64)
         gpu_dt = time.perf_counter() - t0
-        cores = len(os.sched_getaffinity(0))
-        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        cores = host_cores()
         cpu.set_num_threads(cores)
         t0 = time.perf_counter()
         for i in idx:
@@ -235,26 +335,35 @@ def run_uneven_items(count, seed, cpu_sample):
             cpu.grad_batch(nqs[i], np.complex64, ir.lower(circs[i], names), [hams[i].terms_ir()],
                            np.zeros((1, 0)), params[i], batch_threads=1)
         cpu_dt = time.perf_counter() - t0
+        f_err, g_err, g_rel = 0.0, 0.0, 0.0
+        for k, i in enumerate(idx):
+            names = list(circs[i].params_name)
+            fr, gr = cpu.grad_batch(nqs[i], np.complex128, ir.lower(circs[i], names),
+                                    [mq.Hamiltonian(ops[i]).terms_ir()], np.zeros((1, 0)), params[i], batch_threads=1)
+            f_err = max(f_err, abs(f_sub[k] - fr[0, 0]) / max(1.0, coeff_sum(ops[i])))
+            dg = float(np.max(np.abs(g_sub[k] - gr[0, 0].real)))
+            g_err = max(g_err, dg / max(1.0, coeff_sum(ops[i])))
+            g_rel = max(g_rel, dg / float(np.max(np.abs(gr[0, 0].real))))
         sub_bytes = sum((90.5 * nqs[i] + 8) * (1 << nqs[i]) * 8 for i in idx)
         res["cpu_subset"] = {"circuits": len(idx), "qubits": [min(nqs[i] for i in idx), max(nqs[i] for i in idx)],
                              "cpu_seconds": round(cpu_dt, 3), "gpu_seconds": round(gpu_dt, 4), "cores": cores,
                              "kind": "port", "speedup": round(cpu_dt / gpu_dt, 1),
                              "cpu_seconds_full_batch_extrapolated": round(cpu_dt * ref_bytes / sub_bytes, 1)}
-        log(f"uneven batch cpu subset: {res['cpu_subset']}")
+        res["err_vs_c128_oracle"] = {"items": len(idx), "f_rel_sum_c": f_err, "g_abs_over_sum_c": g_err,
+                                     "g_rel_max_g": g_rel}
+        log(f"uneven batch cpu subset: {res['cpu_subset']} err {res['err_vs_c128_oracle']}")
     return res
 
 
+# ---------------------------------------------------------------- large single instances
 def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
     """One large instance: HEA of `layers` layers on n qubits (RY, RZ, CNOT ladder), 64 random
     Pauli strings of weight 2-4 with N(0,1) coefficients; one (f, grad) through the Simulator API."""
     import mindquantum_amd as mq
     from mindquantum_amd import _native
-    from mindquantum_amd.algorithm import hardware_efficient_ansatz, random_pauli_hamiltonian
+    from mindquantum_amd.algorithm import large_hea
 
-    rng = np.random.default_rng(seed)
-    circ = hardware_efficient_ansatz(n, layers)
-    op = random_pauli_hamiltonian(n, n_terms, seed)
-    ans = rng.uniform(0, 2 * np.pi, size=len(circ.params_name))
+    circ, op, ans = large_hea(n, layers, n_terms, seed)
     sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
     grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
     _native.set_option("checkpoints", checkpoints)
@@ -277,9 +386,61 @@ def run_large_adjoint(dtype, n, layers, n_terms, seed, checkpoints):
            "terms": n_terms, "state_gb": round((1 << n) * amp / 1e9, 2), "seconds": round(dt, 3),
            "checkpoints": checkpoints, "f": float(np.real(f[0, 0])),
            "kernel_ms": {k: round(v[1], 1) for k, v in kern.items()},
-           "kernel_hbm_gbs": {k: round(v[2] / (v[1] * 1e-3) / 1e9, 1) for k, v in kern.items() if v[1] > 0}}
+           "kernel_hbm_gbs": {k: round(v[2] / (v[1] * 1e-3) / 1e9, 1) for k, v in kern.items() if v[1] > 0},
+           "err": "no oracle at this size: see analog_24q (same circuit family) and the closed-form 32q / 31q "
+                  "test in tests/test_gpu_fullsize.py"}
     log(f"large adjoint: {res}")
     del sim, grad_ops
+    return res
+
+
+def run_analog_24q(seed, cpu):
+    """The large-instance circuit family at 24 qubits (HEA-4, 64 Pauli strings): GPU complex64 and
+    complex128 beside the CPU oracle (b = 1, OpenMP over all cores), errors against complex128."""
+    import mindquantum_amd as mq
+    from mindquantum_amd import _native, ir
+    from mindquantum_amd.algorithm import large_hea
+    from oracle import cpu as ocpu
+
+    n = 24
+    circ, op, ans = large_hea(n, 4, 64, seed)
+    names = list(circ.params_name)
+    res = {"qubits": n, "layers": 4, "terms": 64, "params": len(names)}
+    out = {}
+    jit = _native.get_option("jit")
+    _native.set_option("jit", 2)
+    try:
+        for dtype in (mq.complex64, mq.complex128):
+            sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=seed)
+            grad_ops = sim.get_expectation_with_grad(mq.Hamiltonian(op, dtype), circ)
+            grad_ops(ans)
+            t0 = time.perf_counter()
+            f, g = grad_ops(ans)
+            out[dtype] = (time.perf_counter() - t0, f[0, 0], g[0, 0].real)
+            del sim, grad_ops
+    finally:
+        _native.set_option("jit", jit)
+    res["gpu_seconds_c64"] = round(out[mq.complex64][0], 4)
+    res["gpu_seconds_c128"] = round(out[mq.complex128][0], 4)
+    cores = host_cores()
+    ocpu.set_num_threads(cores)
+    arrays = ir.lower(circ, names)
+    terms = [mq.Hamiltonian(op).terms_ir()]
+    t0 = time.perf_counter()
+    f_ref, g_ref = ocpu.grad_batch(n, np.complex128, arrays, terms, np.zeros((1, 0)), ans, batch_threads=1)
+    res["cpu_seconds_c128"] = round(time.perf_counter() - t0, 2)
+    if cpu:
+        t0 = time.perf_counter()
+        ocpu.grad_batch(n, np.complex64, arrays, terms, np.zeros((1, 0)), ans, batch_threads=1)
+        res["cpu_seconds_c64"] = round(time.perf_counter() - t0, 2)
+        res["speedup_c64"] = round(res["cpu_seconds_c64"] / res["gpu_seconds_c64"], 1)
+    res["speedup_c128"] = round(res["cpu_seconds_c128"] / res["gpu_seconds_c128"], 1)
+    res["cpu_cores"] = cores
+    csum = coeff_sum(op)
+    for dtype, key in ((mq.complex64, "err_vs_c128_oracle_c64"), (mq.complex128, "err_vs_c128_oracle_c128")):
+        _, f, g = out[dtype]
+        res[key] = err_stats(f, g, f_ref[0, 0], g_ref[0, 0].real, csum)
+    log(f"24q analog: {res}")
     return res
 
 
@@ -303,43 +464,39 @@ def copy_probe(gib=4, reps=10):
     return res
 
 
-def cpu_baseline(n, layers, seed, budget_s):
-    """The CPU oracle (reference mqvector algorithm restated, C++/OpenMP, -O3 -mavx2) on a
-    bounded sample of the headline workload; best of two thread layouts."""
+def cpu_baseline(n, layers, seed, samples):
+    """The CPU oracle (reference mqvector algorithm restated, C++/OpenMP, -O3 -mavx2) on `samples`
+    rows of the headline workload in the three thread layouts of the reference (ParallelTask
+    workers x OpenMP threads per worker, gradient.h:270-299): one worker per core with one OpenMP
+    thread each; one worker with OpenMP over all cores; the reference default (parallel_worker=None:
+    one worker per sample, each with the OpenMP default of all cores).  The best is reported."""
     import mindquantum_amd as mq
     from mindquantum_amd import ir
+    from mindquantum_amd.algorithm import batched_adjoint
     from oracle import cpu
 
-    circ, op, enc, ans = adjoint_workload(n, layers, 64, seed)
+    circ, op, enc, ans = batched_adjoint(n, layers, samples, seed)
     names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
     arrays = ir.lower(circ, names)
     terms = [mq.Hamiltonian(op).terms_ir()]
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    results = {}
-    # (ii) batch_threads = 1, OpenMP over all cores inside each sample
-    log(f"cpu baseline: 1 sample, OpenMP over {cores} threads")
+    cores = host_cores()
     cpu.set_num_threads(cores)
-    t0 = time.perf_counter()
-    cpu.grad_batch(n, np.complex64, arrays, terms, enc[:1], ans, batch_threads=1)
-    t1 = time.perf_counter() - t0
-    results["omp_per_sample"] = 1 / t1
-    # (i) batch_threads = cores, one OpenMP thread per sample
-    log(f"cpu baseline: {cores} samples, one thread each")
-    cpu.set_num_threads(1)
-    m = max(cores, 1)
-    t0 = time.perf_counter()
-    cpu.grad_batch(n, np.complex64, arrays, terms, enc[:m], ans, batch_threads=m)
-    t2 = time.perf_counter() - t0
-    results["thread_per_sample"] = m / t2
-    cpu.set_num_threads(cores)
-    best = max(results, key=results.get)
-    log(f"cpu baseline: {results} ({cores} cores)")
+    layouts = {"thread_per_sample": (cores, 1), "omp_per_sample": (1, cores), "reference_default": (samples, cores)}
+    rates, secs = {}, {}
+    for name, (workers, omp) in layouts.items():
+        log(f"cpu baseline: {samples} samples, {workers} workers x {omp} OpenMP threads ({name})")
+        t0 = time.perf_counter()
+        cpu.grad_batch(n, np.complex64, arrays, terms, enc, ans, batch_threads=workers, worker_threads=omp)
+        secs[name] = time.perf_counter() - t0
+        rates[name] = samples / secs[name]
+    best = max(rates, key=rates.get)
+    log(f"cpu baseline: {rates} ({cores} cores)")
     return {
-        "value": round(results[best], 4), "unit": "grads/s", "cores": cores, "kind": "port",
-        "sample": f"{n}q c64 HEA-{layers} adjoint gradients: 1 sample with OpenMP over {cores} threads "
-                  f"({t1:.1f} s, {results['omp_per_sample']:.3f}/s) and {m} samples one thread each ({t2:.1f} s, "
-                  f"{results['thread_per_sample']:.3f}/s); best = {best}",
+        "value": round(rates[best], 4), "unit": "grads/s", "cores": cores, "kind": "port",
+        "sample": f"{n}q c64 HEA-{layers} adjoint gradients, {samples} samples per layout: "
+                  + "; ".join(f"{k} {layouts[k][0]} workers x {layouts[k][1]} OpenMP ({secs[k]:.1f} s, "
+                              f"{rates[k]:.3f}/s)" for k in layouts) + f"; best = {best}",
+        "layouts": {k: round(v, 4) for k, v in rates.items()},
     }
 
 
@@ -360,6 +517,8 @@ def main():
     ap.add_argument("--items", type=int, default=512, help="circuits of the uneven QAOA batch (0: skip)")
     ap.add_argument("--no-large", action="store_true", help="skip the 32q c64 / 31q c128 single-instance gradients")
     ap.add_argument("--large-checkpoints", type=int, default=4)
+    ap.add_argument("--c128-steps", type=int, default=5)
+    ap.add_argument("--cpu-samples", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile widths (no autotuning)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the device copy probe (counter calibration for profiling runs)")
@@ -388,10 +547,9 @@ def main():
         _native.set_option(key.strip(), int(val))
 
     tune = not args.no_tune and not args.interpreted
-    value, ms_step, stats, n_params, n_gates = run_adjoint(mq.complex64, args.qubits, args.layers, args.batch,
-                                                           args.steps, args.warmup, args.seed, profile=True,
-                                                           tune=tune)
-    tiles = stats.pop("tiles", None)
+    h64 = run_adjoint(mq.complex64, args.qubits, args.layers, args.batch, args.steps, args.warmup, args.seed,
+                      profile=True, tune=tune)
+    value, ms_step, stats = h64["grads_per_s"], h64["ms_per_step"], h64["stats"]
     log(f"c64: {value:.1f} grads/s, {ms_step:.1f} ms/step")
     launches, ms, byts = stats["tile_pass_adj"]
     avg_ms = ms / max(launches, 1)
@@ -405,31 +563,48 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": per_launch, "launches": launches}
     share = {k: round(v[1], 2) for k, v in stats.items()}
-
     extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3,
-             "autotune": tiles}
+             "autotune": h64["tiles"]}
+    samples = [h64["sample"]]
     if not args.no_extra:
         extra["copy_probe"] = copy_probe()
         roofline["achievable_copy_gbs"] = extra["copy_probe"]["gbs"]
-        v128, ms128, st128, _, _ = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, 1, 1,
-                                               args.seed, profile=False, tune=tune)
-        extra["c128_autotune"] = st128.get("tiles")
-        extra["c128_grads_per_s"] = round(v128, 2)
-        extra["c128_ms_per_step"] = round(ms128, 2)
-        log(f"c128: {v128:.1f} grads/s")
-        extra["gate_apply_28q_c64"] = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 10,
-                                                     tune=tune)
-        extra["gate_apply_28q_c128"] = run_gate_apply(mq.complex128, args.apply_qubits, args.apply_depth, args.seed,
-                                                      10, tune=tune)
+        h128 = run_adjoint(mq.complex128, args.qubits, args.layers, args.batch, args.c128_steps, 1, args.seed,
+                           profile=False, tune=tune)
+        samples.append(h128["sample"])
+        extra["c128"] = {"grads_per_s": round(h128["grads_per_s"], 2), "ms_per_step": round(h128["ms_per_step"], 2),
+                         "steps": args.c128_steps, "autotune": h128["tiles"]}
+        log(f"c128: {h128['grads_per_s']:.1f} grads/s")
+    if args.batch >= 16:
+        errs = headline_errors(samples, args.qubits)
+        extra["err_vs_c128_oracle"] = errs[0]
+        if len(errs) > 1:
+            extra["c128"]["err_vs_c128_oracle"] = errs[1]
+    if not args.no_extra:
+        extra["hea16_c128_single"] = run_hea16(20, not args.no_cpu)
+        g64, s64 = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 10, tune=tune)
+        g128, s128 = run_gate_apply(mq.complex128, args.apply_qubits, args.apply_depth, args.seed, 10, tune=tune)
+        up = s64.astype(mq.complex128)
+        g64["fidelity_defect_vs_c128_run"] = abs(1.0 - abs(mq.inner_product(s128, up)))
+        del up, s64, s128
+        if not args.no_cpu:
+            g64["cpu_baseline"] = gate_apply_cpu(np.complex64, args.apply_qubits, 2, args.apply_depth, args.seed)
+            g128["cpu_baseline"] = gate_apply_cpu(np.complex128, args.apply_qubits, 2, args.apply_depth, args.seed)
+            for g in (g64, g128):
+                g["speedup_vs_cpu"] = round(g["cpu_baseline"]["seconds_full_depth_extrapolated"] / g["seconds"], 1)
+            log(f"28q cpu: {g64['cpu_baseline']} / {g128['cpu_baseline']}")
+        extra["gate_apply_28q_c64"] = g64
+        extra["gate_apply_28q_c128"] = g128
         if args.items > 0:
             extra["uneven_qaoa_c64"] = run_uneven_items(args.items, args.seed, not args.no_cpu)
     if not args.no_large and not args.no_extra:
         extra["large_32q_c64"] = run_large_adjoint(mq.complex64, 32, 4, 64, args.seed, args.large_checkpoints)
         extra["large_31q_c128"] = run_large_adjoint(mq.complex128, 31, 4, 64, args.seed, args.large_checkpoints)
+        extra["analog_24q"] = run_analog_24q(args.seed, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.qubits, args.layers, args.seed, 20)
+        cpu = cpu_baseline(args.qubits, args.layers, args.seed, args.cpu_samples)
         extra["speedup_vs_cpu"] = round(value / cpu["value"], 1)
 
     line = {
@@ -438,7 +613,7 @@ def main():
         "vs_baseline": None, "dtype": "c64", "data": "synthetic",
         "config": {"workload": f"{args.qubits}q encoder-RX + HEA-{args.layers} adjoint gradient, batch {args.batch}, "
                                f"complex64 (40-term ZZ ring + X Hamiltonian)",
-                   "qubits": args.qubits, "batch": args.batch, "params": n_params, "gates": n_gates},
+                   "qubits": args.qubits, "batch": args.batch, "params": h64["n_params"], "gates": h64["n_gates"]},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -226,6 +226,7 @@ struct mqa_circuit {
     int n_params = 0;
     int n_slots = 0;
     int max_qubit = -1;
+    bool unitary = true;  // every dense matrix is unitary (the other kinds always are)
     // angle table inputs (per slot)
     std::vector<double> slot_const;
     std::vector<int32_t> slot_term_begin;
@@ -365,6 +366,10 @@ struct mqa_circuit {
 // ---------------------------------------------------------------- Hamiltonian
 struct mqa_ham {
     std::vector<mqa_pauli_term> terms;
+    // sum of the identity terms' coefficients, and the same operator without them (Pauli form only;
+    // NULL when there is no identity term)
+    double identity = 0;
+    std::unique_ptr<mqa_ham> traceless;
     std::vector<mqa::DevHamGroup> groups;
     std::vector<mqa::DevHamTerm> dterms;
     uint64_t qubit_mask = 0;
@@ -775,6 +780,13 @@ void vdot(mqa_sim* sim, const void* a, const void* b, double out[2]) {
     hip_check(hipStreamSynchronize(sim->stream), "sync");
 }
 
+// <state|state> of the simulator's own state.
+double state_norm2(mqa_sim* sim) {
+    double out[2];
+    vdot(sim, sim->state.p, sim->state.p, out);
+    return out[0];
+}
+
 void check_circuit(const mqa_sim* sim, const mqa_circuit* c, int n_params) {
     require(c != nullptr, "circuit is NULL");
     require(c->max_qubit < sim->n, "Simulator has " + std::to_string(sim->n) + " qubits, but circuit has " +
@@ -790,6 +802,56 @@ void check_ham(const mqa_sim* sim, const mqa_ham* h) {
         return;
     }
     require((h->qubit_mask >> sim->n) == 0 || sim->n >= 64, "hamiltonian acts on more qubits than the simulator has");
+}
+
+// A Pauli-term Hamiltonian grouped by flip mask x|y; the coefficient carries i^{|y|} (the
+// reference's POLAR phase, mq_base/utils.cc:22): P|src> = i^{|y|} (-1)^{popcount(src & (y|z))}
+// |src ^ (x|y)>.
+std::unique_ptr<mqa_ham> pauli_ham(const mqa_pauli_term* terms, int32_t n_terms) {
+    auto h = std::make_unique<mqa_ham>();
+    std::map<uint64_t, std::vector<int>> by_flip;
+    for (int t = 0; t < n_terms; t++) {
+        const auto& tm = terms[t];
+        h->terms.push_back(tm);
+        h->qubit_mask |= tm.x_mask | tm.y_mask | tm.z_mask;
+        by_flip[tm.x_mask | tm.y_mask].push_back(t);
+    }
+    for (auto& [flip, ids] : by_flip) {
+        DevHamGroup g{};
+        g.flip = flip;
+        g.term_begin = int32_t(h->dterms.size());
+        g.n_terms = int32_t(ids.size());
+        for (int t : ids) {
+            const auto& tm = terms[t];
+            const int ny = __builtin_popcountll(tm.y_mask) & 3;
+            static const double pr[4] = {1, 0, -1, 0}, pi[4] = {0, 1, 0, -1};
+            DevHamTerm d{};
+            d.yz = tm.y_mask | tm.z_mask;
+            d.coef_re = tm.coeff * pr[ny];
+            d.coef_im = tm.coeff * pi[ny];
+            h->dterms.push_back(d);
+        }
+        h->groups.push_back(g);
+    }
+    if (h->groups.empty()) {  // zero operator: one empty group keeps the kernels uniform
+        DevHamGroup g{};
+        g.flip = 0;
+        g.term_begin = 0;
+        g.n_terms = 0;
+        h->groups.push_back(g);
+    }
+    return h;
+}
+
+// The observable whose product with psi is the hermitian gradient's co-state lambda: for a unitary
+// circuit, H without its identity part c I, which adds c <psi|psi> (the initial state's norm) to f
+// and nothing to any derivative; keeping it out of lambda keeps its rounding out of the taps (a
+// MaxCut cost has c = |E| / 2 beside terms of size 1 / 2).  *shift gets c (0: H itself is used).
+mqa_ham* costate_ham(mqa_ham* h, const mqa_circuit* c, double* shift) {
+    *shift = 0;
+    if (h->csr || !h->traceless || !c->unitary) return h;
+    *shift = h->identity;
+    return h->traceless.get();
 }
 
 // Workspace budget of a call: the option / environment value, else 85% of the free memory plus
@@ -995,6 +1057,17 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
                         "gate " + std::to_string(i) + ": matrix out of range");
                 for (int64_t e = 0; e < m * m; e++)
                     g.mat.push_back({mats[2 * (s.mat_begin + e)], mats[2 * (s.mat_begin + e) + 1]});
+                // M^dagger M == I to 1e-12 (a projector or a derivative matrix is not unitary)
+                for (int64_t r = 0; r < m && c->unitary; r++)
+                    for (int64_t q = 0; q < m; q++) {
+                        std::complex<double> acc = 0;
+                        for (int64_t k = 0; k < m; k++)
+                            acc += std::conj(g.mat[size_t(k * m + r)]) * g.mat[size_t(k * m + q)];
+                        if (std::abs(acc - (r == q ? 1.0 : 0.0)) > 1e-12) {
+                            c->unitary = false;
+                            break;
+                        }
+                    }
             }
             if (param_kind) {
                 g.slot = c->n_slots++;
@@ -1103,41 +1176,23 @@ int mqa_ham_create(const mqa_pauli_term* terms, int32_t n_terms, mqa_ham** out) 
     return guarded([&] {
         require(out != nullptr, "out is NULL");
         require(n_terms >= 0 && (n_terms == 0 || terms != nullptr), "bad term array");
-        auto h = std::make_unique<mqa_ham>();
-        std::map<uint64_t, std::vector<int>> by_flip;
         for (int t = 0; t < n_terms; t++) {
             const auto& tm = terms[t];
             require((tm.x_mask & tm.y_mask) == 0 && (tm.x_mask & tm.z_mask) == 0 && (tm.y_mask & tm.z_mask) == 0,
                     "pauli masks overlap");
-            h->terms.push_back(tm);
-            h->qubit_mask |= tm.x_mask | tm.y_mask | tm.z_mask;
-            by_flip[tm.x_mask | tm.y_mask].push_back(t);
         }
-        for (auto& [flip, ids] : by_flip) {
-            DevHamGroup g{};
-            g.flip = flip;
-            g.term_begin = int32_t(h->dterms.size());
-            g.n_terms = int32_t(ids.size());
-            for (int t : ids) {
-                const auto& tm = terms[t];
-                // c * i^{|y|}: P|src> = i^{|y|} (-1)^{popcount(src & (y|z))} |src ^ (x|y)>
-                int ny = __builtin_popcountll(tm.y_mask) & 3;
-                static const double pr[4] = {1, 0, -1, 0}, pi[4] = {0, 1, 0, -1};
-                DevHamTerm d{};
-                d.yz = tm.y_mask | tm.z_mask;
-                d.coef_re = tm.coeff * pr[ny];
-                d.coef_im = tm.coeff * pi[ny];
-                h->dterms.push_back(d);
+        std::unique_ptr<mqa_ham> h = pauli_ham(terms, n_terms);
+        std::vector<mqa_pauli_term> rest;
+        bool has_identity = false;
+        for (int t = 0; t < n_terms; t++) {
+            if ((terms[t].x_mask | terms[t].y_mask | terms[t].z_mask) == 0) {
+                h->identity += terms[t].coeff;
+                has_identity = true;
+            } else {
+                rest.push_back(terms[t]);
             }
-            h->groups.push_back(g);
         }
-        if (h->groups.empty()) {  // zero operator: one empty group keeps the kernels uniform
-            DevHamGroup g{};
-            g.flip = 0;
-            g.term_begin = 0;
-            g.n_terms = 0;
-            h->groups.push_back(g);
-        }
+        if (has_identity) h->traceless = pauli_ham(rest.data(), int32_t(rest.size()));
         *out = h.release();
     });
 }
@@ -1463,6 +1518,7 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         sim->grads.ensure(std::max<size_t>(size_t(chunk) * n_params * 8, 8));
         std::vector<double> g_host(size_t(chunk) * n_params), f_host(size_t(chunk) * 2);
         auto ckpt_at = [&](int i) { return static_cast<char*>(sim->ckpt.p) + size_t(i) * sb * size_t(chunk); };
+        double norm0 = -1;  // <state|state> of the initial state, when an identity part needs it
 
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t nb = std::min<int64_t>(chunk, batch - b0);
@@ -1493,8 +1549,9 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                     forward();
                     g_recomputes.fetch_add(1);
                 }
-                auto* ham = const_cast<mqa_ham*>(hams[h]);
-                // lam = H psi, f = <psi|H psi>
+                double shift = 0;
+                mqa_ham* ham = costate_ham(const_cast<mqa_ham*>(hams[h]), circ, &shift);
+                // lam = H psi, f = <psi|H psi> (+ the identity part times the initial norm)
                 ham_apply(sim, ham, sim->psi.p, sim->psi.p, sim->lam.p, nb);
                 hip_check(hipMemcpyAsync(f_host.data(), sim->fsum.p, nb * 2 * sizeof(double), hipMemcpyDeviceToHost, s),
                           "d2h f");
@@ -1527,9 +1584,10 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                     }
                 }
                 sync(sim, "grad batch");
+                if (shift != 0 && norm0 < 0) norm0 = state_norm2(sim);
                 for (int64_t b = 0; b < nb; b++) {
                     double* fo = f_out + ((b0 + b) * n_hams + h) * 2;
-                    fo[0] = f_host[2 * b];
+                    fo[0] = f_host[2 * b] + (shift != 0 ? shift * norm0 : 0.0);
                     fo[1] = f_host[2 * b + 1];
                     double* go = g_out + ((b0 + b) * n_hams + h) * size_t(n_params);
                     for (int p = 0; p < n_params; p++) go[p] = g_host[size_t(b) * n_params + p];
@@ -1835,10 +1893,13 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
         StreamOwner so;
         const size_t amp = dtype == MQA_C64 ? 8 : 16;
         std::vector<GradItem> all;
+        std::vector<const mqa_ham*> costate(size_t(n_items), nullptr);  // the co-state observable of each item
+        std::vector<double> shift(size_t(n_items), 0.0);
         for (int i = 0; i < n_items; i++) {
             auto* c = const_cast<mqa_circuit*>(circs[i]);
-            const mqa_ham* h = hams[i];
-            require(c != nullptr && h != nullptr, "item " + std::to_string(i) + ": NULL circuit or hamiltonian");
+            require(c != nullptr && hams[i] != nullptr, "item " + std::to_string(i) + ": NULL circuit or hamiltonian");
+            const mqa_ham* h = costate_ham(const_cast<mqa_ham*>(hams[i]), c, &shift[size_t(i)]);
+            costate[size_t(i)] = h;
             const int n = n_qubits[i];
             require(n >= 1 && n <= 36, "item " + std::to_string(i) + ": bad qubit count");
             require(c->max_qubit < n, "item " + std::to_string(i) + ": circuit is wider than its qubit count");
@@ -1870,9 +1931,10 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             size_t b = a, used = 0;
             while (b < all.size() && (b == a || used + all[b].bytes <= budget)) used += all[b++].bytes;
             std::vector<GradItem> chunk(all.begin() + long(a), all.begin() + long(b));
-            run_grad_chunk(chunk, circs, hams, params, param_offset, dtype, so.s, f_out, g_out);
+            run_grad_chunk(chunk, circs, costate.data(), params, param_offset, dtype, so.s, f_out, g_out);
             a = b;
         }
+        for (int i = 0; i < n_items; i++) f_out[2 * i] += shift[size_t(i)];  // every item starts from |0...0>
     });
 }
 

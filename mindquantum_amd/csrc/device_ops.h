@@ -186,6 +186,20 @@ __device__ __forceinline__ void cdot_acc(T xr, T xi, T yr, T yi, T& sr, T& si) {
     si = fma(-xi, yr, si);
 }
 
+// x / sqrt(2).  In float, 1 / sqrt(2) rounds down (2 r^2 = 1 - 3.1e-8), so a plain product shrinks
+// the norm of the state by that factor at every Hadamard (-4e-5 after 1400 of them); the product
+// with r = hi + lo (lo the rounding remainder of hi) rounds without that bias.
+template <typename T>
+__device__ __forceinline__ T hmul(T x) {
+    if constexpr (sizeof(T) == 4) {
+        constexpr float hi = 0.70710678118654752440f;
+        constexpr float lo = float(0.70710678118654752440084436210484903928 - double(hi));
+        return fma(x, hi, x * lo);
+    } else {
+        return x * T(0.70710678118654752440084436210484903928);
+    }
+}
+
 // One-target op on a register set.  c, s: cos / sin of the half angle (RX, RY, RZ) or of the
 // angle (PS, GP); m: dense 2x2 (M1) in the matrix pool.  CTRL: some control qubits are local;
 // their predicate is evaluated per pair and applied with selects (no divergent branches).
@@ -198,9 +212,8 @@ __device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32
         if constexpr (KIND == OPK_X) {
             xr = br, xi = bi, yr = ar, yi = ai;
         } else if constexpr (KIND == OPK_H) {  // [[1, 1], [1, -1]] / sqrt(2)
-            constexpr T r = T(0.70710678118654752440084436210484903928);
-            xr = (ar + br) * r, xi = (ai + bi) * r;
-            yr = (ar - br) * r, yi = (ai - bi) * r;
+            xr = hmul<T>(ar + br), xi = hmul<T>(ai + bi);
+            yr = hmul<T>(ar - br), yi = hmul<T>(ai - bi);
         } else if constexpr (KIND == OPK_RX) {  // [[c, -is], [-is, c]]
             xr = c * ar + s * bi, xi = c * ai - s * br;
             yr = c * br + s * ai, yi = c * bi - s * ar;

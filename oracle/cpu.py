@@ -35,6 +35,7 @@ def lib():
         _lib.oracle_grad_batch.argtypes = [I, I, P, P, I, P, P, P, P, P, I, P, ctypes.c_int64, I, P, I, I, P]
         _lib.oracle_num_threads.restype = I
         _lib.oracle_set_num_threads.argtypes = [I]
+        _lib.oracle_set_worker_threads.argtypes = [I]
     return _lib
 
 
@@ -88,8 +89,11 @@ def vdot(a, b):
     return complex(out[0], out[1])
 
 
-def grad_batch(n, dtype, ir_arrays, hams_terms, enc, ans, batch_threads=1, init=None):
-    """Reference hermitian adjoint gradient: returns f [b, H] and g [b, H, P] (complex)."""
+def grad_batch(n, dtype, ir_arrays, hams_terms, enc, ans, batch_threads=1, init=None, worker_threads=0):
+    """Reference hermitian adjoint gradient: returns f [b, H] and g [b, H, P] (complex).
+
+    batch_threads worker threads share the samples (gradient.h:270-299, one sample runs inline); each
+    worker uses worker_threads OpenMP threads (0: the OpenMP default)."""
     gates, tp, tc, mats, _, _ = ir_arrays
     enc = np.ascontiguousarray(enc, dtype=np.float64)
     ans = np.ascontiguousarray(ans, dtype=np.float64)
@@ -105,6 +109,7 @@ def grad_batch(n, dtype, ir_arrays, hams_terms, enc, ans, batch_threads=1, init=
     out = np.zeros((batch, len(hams_terms), 1 + n_params, 2), dtype=np.float64)
     if init is not None:
         init = np.ascontiguousarray(init, dtype=dtype)
+    lib().oracle_set_worker_threads(int(worker_threads))
     _chk(lib().oracle_grad_batch(n, _code(dtype), _p(init), _p(gates), len(gates), _p(tp), _p(tc), _p(mats),
                                  _p(terms), _p(off), len(hams_terms), _p(enc) if n_enc else None, batch, n_enc,
                                  _p(ans) if n_ans else None, n_ans, batch_threads, _p(out)))

@@ -31,6 +31,11 @@
 
 namespace oracle {
 
+// OpenMP threads of each ParallelTask worker (0: the OpenMP default).  A std::thread starts with the
+// default thread count, not the one omp_set_num_threads set on the caller, so the layouts of the
+// reference's thread balance (batch threads x OpenMP threads) are set here explicitly.
+static int g_worker_threads = 0;
+
 using index_t = uint64_t;
 constexpr index_t kDimTh = index_t(1) << 13;  // cpu_vector_policy.h:47
 
@@ -535,6 +540,7 @@ struct Sim {
             int64_t start = end;
             end = start + offset + (i < left ? 1 : 0);
             tasks.emplace_back([&, start, end]() {
+                if (g_worker_threads > 0) omp_set_num_threads(g_worker_threads);
                 for (int64_t b = start; b < end; b++) run(b);
             });
         }
@@ -589,6 +595,8 @@ const char* oracle_last_error(void) { return oracle::g_err.c_str(); }
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 
 void oracle_set_num_threads(int n) { omp_set_num_threads(n); }
+
+void oracle_set_worker_threads(int n) { oracle::g_worker_threads = n < 0 ? 0 : n; }
 
 // qs: interleaved complex (float pairs for c64, double pairs for c128), length 2^n, in place.
 int oracle_apply_circuit(int n, int dtype, void* qs, const mqa_gate* gates, int n_gates, const int32_t* term_param,

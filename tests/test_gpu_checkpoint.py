@@ -139,4 +139,4 @@ def test_uneven_qaoa_batch_full_depth():
         f_ref, g_ref = cpu.grad_batch(n, mq.complex128, ir.lower(c, list(c.params_name)),
                                       [mq.Hamiltonian(o).terms_ir()], np.zeros((1, 0)), p, batch_threads=1)
         assert_f(f[i], f_ref[0, 0], mq.complex64, coeff_sum(o), f"f of item {i} (n={n})")
-        assert_g(grads[i], g_ref[0, 0].real, mq.complex64, f"g of item {i} (n={n})")
+        assert_g(grads[i], g_ref[0, 0].real, mq.complex64, f"g of item {i} (n={n})", csum=coeff_sum(o, identity=False))

@@ -35,7 +35,7 @@ def check_items(circs, ops, nqs, params, dtype):
         assert_f(f[i], f_ref, dtype, coeff_sum(o), f"f of item {i} (n={n})")
         assert grads[i].shape == g_ref.shape
         if g_ref.size:
-            assert_g(grads[i], g_ref, dtype, f"g of item {i} (n={n})")
+            assert_g(grads[i], g_ref, dtype, f"g of item {i} (n={n})", csum=coeff_sum(o, identity=False))
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
