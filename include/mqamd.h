@@ -194,6 +194,15 @@ int mqa_sim_apply_hamiltonian(mqa_sim* sim, const mqa_ham* ham);
  * state; the simulator's own state is left unchanged.  circ may be NULL (no circuit). */
 int mqa_sim_expectation(const mqa_sim* sim, const mqa_ham* ham, const mqa_circuit* circ, const double* params,
                         int32_t n_params, double out[2]);
+/* mqa_sim_expectation with no circuit, <state|H|state> of the simulator's own state, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  It calls no hipMalloc / hipFree: the
+ * Hamiltonian's tables, the per-block partials and the reduced sum live in one file-scope device
+ * array of 4 MiB per GPU (held until the process ends); Pauli-term Hamiltonians of at most 8192 flip
+ * groups and 60000 terms (MQA_ERR_ARG beyond; CSR matrices are rejected).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several threads or streams) race.
+ * Same kernels and result as mqa_sim_expectation(sim, ham, NULL, ...). */
+int mqa_sim_expectation_noalloc(const mqa_sim* sim, const mqa_ham* ham, double out[2]);
 /* get_expectation(ham, circ_right, circ_left, sim_left, pr) (vector_state.tpp:728-768):
  * <sim_left|Ul^dag H Ur|sim>.  left may be NULL (== sim); either circuit may be NULL. */
 int mqa_sim_expectation_lr(const mqa_sim* sim, const mqa_sim* left, const mqa_ham* ham, const mqa_circuit* circ_right,

@@ -84,6 +84,7 @@ _SIGS = {
     "mqa_sim_apply_hamiltonian": ([_P, _P], ctypes.c_int),
     "mqa_sim_expectation": ([_P, _P, _P, _P, _I32, _P], ctypes.c_int),
     "mqa_sim_expectation_lr": ([_P, _P, _P, _P, _P, _P, _I32, _P], ctypes.c_int),
+    "mqa_sim_expectation_noalloc": ([_P, _P, _P], ctypes.c_int),
     "mqa_sim_grad_batch": ([_P, _P, _I32, _P, _P, _I64, _I32, _P, _I32, _P, _P], ctypes.c_int),
     "mqa_sim_grad_nonherm": ([_P, _P, _P, _I32, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P], ctypes.c_int),
     "mqa_sim_sample": ([_P, _P, _P, _I32, _P, _U64, _P], ctypes.c_int),

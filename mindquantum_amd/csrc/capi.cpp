@@ -1455,6 +1455,95 @@ int mqa_sim_expectation(const mqa_sim* sim, const mqa_ham* ham, const mqa_circui
     return mqa_sim_expectation_lr(sim, nullptr, ham, circ, nullptr, params, n_params, out);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free expectation
+// Device scratch of mqa_sim_expectation_noalloc, one copy per GPU for the life of the process:
+// [the reduced (re, im) sum][per-block (re, im) partials][flip groups][terms].  In host code the
+// array's name is not a device address: each device's address comes from hipGetSymbolAddress.
+namespace mqa {
+namespace {
+constexpr size_t kNoAllocBytes = size_t(4) << 20;
+constexpr size_t kNoAllocPartialsAt = 256;
+constexpr uint64_t kNoAllocMaxBlocks = uint64_t(1) << 17;
+constexpr size_t kNoAllocGroupsAt = kNoAllocPartialsAt + kNoAllocMaxBlocks * 16;
+constexpr size_t kNoAllocMaxGroups = 8192;
+constexpr size_t kNoAllocTermsAt = kNoAllocGroupsAt + kNoAllocMaxGroups * sizeof(DevHamGroup);
+constexpr size_t kNoAllocMaxTerms = (kNoAllocBytes - kNoAllocTermsAt) / sizeof(DevHamTerm);
+static_assert(kNoAllocMaxTerms >= 60000, "scratch layout");
+}  // namespace
+}  // namespace mqa
+
+static __device__ __align__(256) unsigned char scratch[mqa::kNoAllocBytes];
+
+namespace mqa {
+namespace {
+// Address of `scratch` on the current device (looked up once per device, with it current).
+unsigned char* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> ptrs;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    ptrs[device] = static_cast<unsigned char*>(p);
+    return static_cast<unsigned char*>(p);
+}
+}  // namespace
+}  // namespace mqa
+
+extern "C" {
+
+int mqa_sim_expectation_noalloc(const mqa_sim* sim_c, const mqa_ham* ham, double out[2]) {
+    return guarded([&] {
+        require(sim_c != nullptr && ham != nullptr && out != nullptr, "NULL argument");
+        auto* sim = const_cast<mqa_sim*>(sim_c);
+        check_ham(sim, ham);
+        require(!ham->csr, "mqa_sim_expectation_noalloc takes Pauli-term hamiltonians only (a CSR matrix needs a "
+                           "workspace state)");
+        require(ham->groups.size() <= kNoAllocMaxGroups,
+                "hamiltonian has " + std::to_string(ham->groups.size()) + " flip groups; mqa_sim_expectation_noalloc "
+                "holds at most " + std::to_string(kNoAllocMaxGroups));
+        require(ham->dterms.size() <= kNoAllocMaxTerms,
+                "hamiltonian has " + std::to_string(ham->dterms.size()) + " terms; mqa_sim_expectation_noalloc "
+                "holds at most " + std::to_string(kNoAllocMaxTerms));
+        DeviceGuard dg(sim->device);
+        unsigned char* base = noalloc_scratch(sim->device);
+        hipStream_t s = sim->stream;
+        hip_check(hipMemcpyAsync(base + kNoAllocGroupsAt, ham->groups.data(), ham->groups.size() * sizeof(DevHamGroup),
+                                 hipMemcpyHostToDevice, s),
+                  "groups to scratch");
+        if (!ham->dterms.empty())
+            hip_check(hipMemcpyAsync(base + kNoAllocTermsAt, ham->dterms.data(), ham->dterms.size() * sizeof(DevHamTerm),
+                                     hipMemcpyHostToDevice, s),
+                      "terms to scratch");
+        // the gather kernel of mqa_sim_expectation on the state itself (no circuit, so no copy); blocks
+        // of 2048 amplitudes, or wider so that the partials fit the scratch
+        uint64_t per_block = std::min<uint64_t>(sim->dim_eff, 2048);
+        while ((sim->dim_eff + per_block - 1) / per_block > kNoAllocMaxBlocks) per_block *= 2;
+        require(per_block <= 0xffffffffull, "state too large for mqa_sim_expectation_noalloc");
+        HamArgs h{};
+        h.ket = sim->state.p;
+        h.ket_stride = sim->dim_eff;
+        h.bra = sim->state.p;
+        h.bra_stride = sim->dim_eff;
+        h.out = nullptr;
+        h.out_stride = sim->dim_eff;
+        h.dim = sim->dim_eff;
+        h.amps_per_block = uint32_t(per_block);
+        h.n_blocks = uint32_t((sim->dim_eff + per_block - 1) / per_block);
+        h.groups = reinterpret_cast<const DevHamGroup*>(base + kNoAllocGroupsAt);
+        h.n_groups = int(ham->groups.size());
+        h.terms = reinterpret_cast<const DevHamTerm*>(base + kNoAllocTermsAt);
+        h.partial = reinterpret_cast<double*>(base + kNoAllocPartialsAt);  // every block writes its entry
+        hip_check(launch_ham(sim->dtype, false, h, 1, s), "hamiltonian");
+        hip_check(launch_reduce_rows(h.partial, h.n_blocks, reinterpret_cast<double*>(base), 1, s), "reduce");
+        hip_check(hipMemcpyAsync(out, base, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "expectation (no allocation)");
+    });
+}
+
 int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t n_hams, const mqa_circuit* circ_c,
                        const double* enc, int64_t batch, int32_t n_enc, const double* ans, int32_t n_ans, double* f_out,
                        double* g_out) {
