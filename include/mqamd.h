@@ -61,6 +61,16 @@ extern "C" {
 #define MQA_GATE_RXX 37
 #define MQA_GATE_RYY 38
 #define MQA_GATE_RZZ 39
+/* parameterised dense matrix (custom gates of one or two parameters, parameterised Rn; 1 or 2 targets):
+ * the matrix depends on the parameters through a host function, so its values come per call and per
+ * sample from mqa_circuit_set_custom_matrices.  mat_begin is the offset (in doubles) of the gate's
+ * block in a sample's row: [G][G^dagger][M_0], each D x D interleaved complex (D = 2^n_obj), with
+ * M_0 = dG/dv_0 G^dagger for the gate's first parameter expression v_0 (the angle fields).  A gate
+ * with further expressions v_j is followed by one MQA_GATE_PMATRIX_TAP record per expression (same
+ * targets and controls; mat_begin: the offset of M_j = dG/dv_j G^dagger; angle fields: v_j), which
+ * only contributes a derivative. */
+#define MQA_GATE_PMATRIX 40
+#define MQA_GATE_PMATRIX_TAP 41
 
 #define MQA_MAX_OBJ 4
 
@@ -138,6 +148,12 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
                        int32_t n_terms, const double* mats, int64_t n_mat_entries, int32_t n_params,
                        const uint8_t* param_requires_grad, mqa_circuit** out);
 int mqa_circuit_destroy(mqa_circuit* circ);
+/* Values of a circuit's parameterised matrices (MQA_GATE_PMATRIX / _TAP) for the calls that follow:
+ * rows x stride doubles, row b for sample b of a batched call (rows == 1: one row shared by every
+ * sample).  A call that runs the circuit on more samples than rows (other than 1) fails.  The reference
+ * evaluates the gate's matrix function per sample inside its C++ loop (CustomGate, vector_state.tpp:
+ * 402-413); here the host evaluates it and the kernels read the table. */
+int mqa_circuit_set_custom_matrices(mqa_circuit* circ, int64_t rows, int64_t stride, const double* table);
 /* Tile width (qubits per LDS tile) of this circuit's forward (adjoint == 0) or back-sweep passes
  * in one precision; tile_qubits < 0 returns to the global option ("tile_qubits_c64" ...).  Set by
  * the autotuner (mindquantum_amd.autotune), which times candidate widths on the device.  The get

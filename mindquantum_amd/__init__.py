@@ -9,6 +9,7 @@ from .core import (CNOT, FSIM, H, I, ISWAP, RX, RY, RZ, SWAP, SX, U3, UN, Barrie
                    GlobalPhase, GroupedPauli, Hamiltonian, Measure, MeasureResult, ParameterResolver, PhaseShift, Power, QubitOperator, Rn,
                    RotPauliString, Rxx, Rxy, Rxz, Ryy, Ryz, Rzz, S, SWAPalpha, T, UnivMathGate, X, Y, Z)
 from .core import gates
+from .core.gates import gene_univ_parameterized_gate, gene_univ_two_params_gate
 from .dtype import complex64, complex128
 from .simulator import Simulator, expectation_with_grad_batch, inner_product
 

@@ -26,6 +26,7 @@ GATE_KIND = {
     "I": 0, "X": 1, "Y": 2, "Z": 3, "H": 4, "S": 5, "Sdag": 6, "T": 7, "Tdag": 8, "SX": 9, "SXdag": 10,
     "SWAP": 11, "ISWAP": 12, "ISWAPdag": 13, "MATRIX": 14,
     "RX": 32, "RY": 33, "RZ": 34, "PS": 35, "GP": 36, "Rxx": 37, "Ryy": 38, "Rzz": 39,
+    "PMATRIX": 40, "PMATRIX_TAP": 41,
 }
 
 # numpy views of the C structs (64 and 32 bytes, natural alignment)
@@ -66,6 +67,7 @@ _SIGS = {
     "mqa_get_option": ([ctypes.c_char_p, ctypes.POINTER(_I64)], ctypes.c_int),
     "mqa_circuit_create": ([_P, _I32, _P, _P, _I32, _P, _I64, _I32, _P, _PP], ctypes.c_int),
     "mqa_circuit_destroy": ([_P], ctypes.c_int),
+    "mqa_circuit_set_custom_matrices": ([_P, _I64, _I64, _P], ctypes.c_int),
     "mqa_circuit_set_tile_qubits": ([_P, _I32, _I32, _I32], ctypes.c_int),
     "mqa_circuit_get_tile_qubits": ([_P, _I32, _I32, ctypes.POINTER(_I32)], ctypes.c_int),
     "mqa_circuit_schedule": ([_P, _I32, _I32, _I32, _P, _I64, ctypes.POINTER(_I64)], ctypes.c_int),

@@ -133,6 +133,21 @@ def layered_random_circuit(n_qubits, depth, seed):
     return circ
 
 
+def qft(qubits):
+    """Quantum Fourier transform on `qubits` (algorithm/library/quantum_fourier.py): per qubit i, H on
+    it and PhaseShift(2 pi / 2^k) controlled by each later qubit (k = 2, 3, ...), then the qubit order
+    reversed by SWAPs."""
+    qubits = list(qubits)
+    circ = Circuit()
+    for i, q in enumerate(qubits):
+        circ += G.H.on(q)
+        for k, c in enumerate(qubits[i + 1:]):
+            circ += G.PhaseShift(2 * np.pi / 2 ** (k + 2)).on(q, c)
+    for j in range(len(qubits) // 2):
+        circ += G.SWAP.on([qubits[j], qubits[len(qubits) - 1 - j]])
+    return circ
+
+
 def random_pauli_hamiltonian(n_qubits, n_terms, seed, min_weight=2, max_weight=4):
     rng = np.random.default_rng(seed)
     op = QubitOperator("", 0)

@@ -698,12 +698,159 @@ class Rn(MultiParamsGate):
         gen = (a * _PX + b * _PY + c * _PZ) / f
         return np.cos(f / 2) * np.eye(2) - 1j * np.sin(f / 2) * gen
 
+    def _dmats(self, a, b, c):
+        """d/dv_j of cos(f/2) I - i sin(f/2) n.sigma with v = (a, b, c), f = |v|, n = v / f:
+        -(sin(f/2) / 2) n_j I - i [(cos(f/2) / 2) n_j n.sigma + sin(f/2) (sigma_j - n_j n.sigma) / f];
+        at f = 0: -i sigma_j / 2."""
+        v = np.array([a, b, c], dtype=float)
+        f = float(np.sqrt(v @ v))
+        sig = (_PX, _PY, _PZ)
+        if f == 0:
+            return [-0.5j * p for p in sig]
+        nv = v / f
+        ns = nv[0] * _PX + nv[1] * _PY + nv[2] * _PZ
+        cf, sf = np.cos(f / 2), np.sin(f / 2)
+        return [-0.5 * sf * nv[j] * np.eye(2) - 1j * (0.5 * cf * nv[j] * ns + sf * (sig[j] - nv[j] * ns) / f)
+                for j in range(3)]
+
+    @property
+    def ir_kind(self):
+        return "PMATRIX"
+
     def lowered(self):
-        if self.parameterized:
-            raise ValueError("Rn with variable parameters is not supported by the HIP backend "
-                             "(its rotation axis depends on the parameters non-linearly)")
+        if self.parameterized:  # per-sample matrices from the closed form above
+            return [self]
         m = UnivMathGate("Rn", self.matrix())
         return [m.on(self.obj_qubits, self.ctrl_qubits)]
+
+
+class ParamCustomGate(ParameterGate):
+    """A gate of one parameter whose matrix is a user function (basicgate.py:1828-1960
+    gene_univ_parameterized_gate): ``matrix_generator(x)`` and ``diff_matrix_generator(x)`` (its
+    derivative) return 2^k x 2^k complex arrays.  The hermitian gate keeps the parameter and takes the
+    conjugate transposes (basicgate.py:1905-1911, 1937-1945).  The backend evaluates the functions on
+    the host per sample and runs the matrices as dense ops (k <= 2)."""
+
+    _gen = None
+    _dgen = None
+
+    def __init__(self, pr):
+        super().__init__(self._name, self._n_qubits, pr)
+        self.hermitianed = False
+
+    def _mat(self, x):
+        m = np.asarray(self._gen(x), dtype=np.complex128)
+        return m.conj().T.copy() if self.hermitianed else m
+
+    def _dmat(self, x):
+        m = np.asarray(self._dgen(x), dtype=np.complex128)
+        return m.conj().T.copy() if self.hermitianed else m
+
+    def hermitian(self):
+        new = copy.deepcopy(self)
+        new.hermitianed = not self.hermitianed
+        return new
+
+    @property
+    def ir_kind(self):
+        return "PMATRIX"
+
+
+def _check_generator(fn, n_args, what):
+    import inspect
+
+    if len(inspect.signature(fn).parameters) != n_args:
+        raise ValueError(f"{what} can only have {n_args} argument{'s' if n_args > 1 else ''}, but get "
+                         f"{len(inspect.signature(fn).parameters)}.")
+
+
+def gene_univ_parameterized_gate(name, matrix_generator, diff_matrix_generator):
+    """A gate class of one parameter with a user matrix and derivative (basicgate.py:1828-1960): the
+    same checks on the two generators, no numba needed."""
+    _check_generator(matrix_generator, 1, "matrix_generator")
+    _check_generator(diff_matrix_generator, 1, "diff_matrix_generator")
+    m, dm = matrix_generator(0.123), diff_matrix_generator(0.123)
+    if not isinstance(m, np.ndarray) or m.dtype != complex:
+        raise ValueError(f"matrix_generator should return numpy array with complex type, but get {type(m)}.")
+    if not isinstance(dm, np.ndarray) or dm.dtype != complex:
+        raise ValueError(f"diff_matrix_generator should return numpy array with complex type, but get {type(dm)}")
+    if m.shape != dm.shape:
+        raise ValueError("matrix_generator and diff_matrix_generator should generate same shape matrix.")
+    n = int(round(np.log2(m.shape[0])))
+    return type(name, (ParamCustomGate,), {"_name": name, "_n_qubits": n, "_gen": staticmethod(matrix_generator),
+                                           "_dgen": staticmethod(diff_matrix_generator)})
+
+
+class TwoParamCustomGate(BasicGate):
+    """A gate of two parameters with user matrix functions (basicgate.py:2039-2230
+    gene_univ_two_params_gate): ``matrix_generator(a, b)`` and its two partial derivatives."""
+
+    _gen = None
+    _dgens = ()
+
+    def __init__(self, pr1, pr2):
+        super().__init__(self._name, self._n_qubits)
+        self.prs = []
+        for pr in (pr1, pr2):
+            p = ParameterResolver(pr)
+            if p.is_complex:
+                raise ValueError(f"{self._name} gate only supports real parameters")
+            self.prs.append(p)
+        self.hermitianed = False
+
+    @property
+    def parameterized(self):
+        return any(not p.is_const() for p in self.prs)
+
+    def param_resolvers(self):
+        return self.prs
+
+    def _values(self, pr):
+        return MultiParamsGate._values(self, pr)
+
+    def matrix(self, pr=None):
+        m = np.asarray(self._gen(*self._values(pr)), dtype=np.complex128)
+        return m.conj().T.copy() if self.hermitianed else m
+
+    def diff_matrices(self, pr=None):
+        vals = self._values(pr)
+        out = [np.asarray(d(*vals), dtype=np.complex128) for d in self._dgens]
+        return [d.conj().T.copy() for d in out] if self.hermitianed else out
+
+    def hermitian(self):
+        new = copy.deepcopy(self)
+        new.hermitianed = not self.hermitianed
+        return new
+
+    def no_grad(self):
+        for p in self.prs:
+            p.no_grad()
+        return self
+
+    def requires_grad(self):
+        for p in self.prs:
+            p.requires_grad()
+        return self
+
+    @property
+    def ir_kind(self):
+        return "PMATRIX"
+
+
+def gene_univ_two_params_gate(name, matrix_generator, diff_matrix_generator_1, diff_matrix_generator_2):
+    """A gate class of two parameters with a user matrix and its partial derivatives
+    (basicgate.py:2039-2230), no numba needed."""
+    _check_generator(matrix_generator, 2, "matrix_generator")
+    _check_generator(diff_matrix_generator_1, 2, "diff_matrix_generator_1")
+    _check_generator(diff_matrix_generator_2, 2, "diff_matrix_generator_2")
+    m = matrix_generator(0.123, 0.456)
+    for d in (diff_matrix_generator_1(0.123, 0.456), diff_matrix_generator_2(0.123, 0.456)):
+        if not isinstance(d, np.ndarray) or d.shape != np.shape(m):
+            raise ValueError("matrix_generator and the diff matrix generators should generate same shape matrix.")
+    n = int(round(np.log2(np.shape(m)[0])))
+    return type(name, (TwoParamCustomGate,), {
+        "_name": name, "_n_qubits": n, "_gen": staticmethod(matrix_generator),
+        "_dgens": (diff_matrix_generator_1, diff_matrix_generator_2)})
 
 
 class GroupedPauli(NoneParameterGate):
@@ -768,5 +915,6 @@ __all__ = [
     "XGate", "YGate", "ZGate", "HGate", "SGate", "TGate", "SXGate", "IGate", "SWAPGate", "ISWAPGate", "CNOTGate",
     "UnivMathGate", "RX", "RY", "RZ", "Rxx", "Ryy", "Rzz", "PhaseShift", "GlobalPhase",
     "PauliRotationGate", "Rxy", "Rxz", "Ryz", "RotPauliString", "Givens", "SWAPalpha", "MultiParamsGate", "U3", "FSim",
-    "Rn", "GroupedPauli", "Power", "BarrierGate", "FSIM",
+    "Rn", "GroupedPauli", "Power", "BarrierGate", "FSIM", "ParamCustomGate", "TwoParamCustomGate",
+    "gene_univ_parameterized_gate", "gene_univ_two_params_gate",
 ]

@@ -227,6 +227,11 @@ struct mqa_circuit {
     int n_slots = 0;
     int max_qubit = -1;
     bool unitary = true;  // every dense matrix is unitary (the other kinds always are)
+    // parameterised matrices (MQA_GATE_PMATRIX / _TAP): doubles a sample's row needs, and the table
+    // of the calls that follow (mqa_circuit_set_custom_matrices)
+    int64_t custom_need = 0;
+    int64_t custom_rows = 0, custom_stride = 0;
+    std::vector<double> custom_table;
     // angle table inputs (per slot)
     std::vector<double> slot_const;
     std::vector<int32_t> slot_term_begin;
@@ -274,7 +279,8 @@ struct mqa_circuit {
         dp->plan = std::move(plan);
         // dense blocks or controls that stay per-amplitude predicates need the FULL instantiation
         for (auto& op : dp->plan.dev_ops)
-            if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0)
+            if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0 ||
+                mqa::opk_is_custom(op.kind))
                 dp->full = true;
         if (upload_dev) {
             mqa::upload(dp->passes, dp->plan.dev_passes, s);
@@ -463,6 +469,7 @@ struct mqa_sim {
     // workspaces (grow-only)
     mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
     mqa::DevBuf coefs;  // coefficient table of the compiled passes (rebuilt per run_plan call)
+    mqa::DevBuf custom[2];  // rows of the custom-gate matrices of the running circuits (two: bra and ket sides)
     mqa::Timer timer;
     size_t amp_bytes() const { return dtype == MQA_C64 ? 8 : 16; }
     size_t state_bytes() const { return dim_eff * amp_bytes(); }
@@ -516,8 +523,13 @@ void sync(mqa_sim* sim, const char* what) {
 
 // Run every pass of a plan on `batch` states.
 // Passes [p0, p1) of the plan (p1 < 0: to the end).
+struct CustomRows {
+    const double* p = nullptr;
+    uint64_t stride = 0;
+};
+
 void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, const double* angles, int n_slots,
-              double* partials, int n_tap_ids, int p0 = 0, int p1 = -1) {
+              double* partials, int n_tap_ids, int p0 = 0, int p1 = -1, CustomRows custom = {}) {
     const Plan& pl = dp->plan;
     if (p1 < 0 || p1 > int(pl.passes.size())) p1 = int(pl.passes.size());
     if (p0 >= p1) return;
@@ -537,6 +549,8 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     a.tap_ids = dp->tap_ids.as<int32_t>();
     a.partials = partials;
     a.batch = batch;
+    a.custom = custom.p;
+    a.custom_stride = custom.stride;
     const double bytes = double(batch) * double(sim->state_bytes()) * (pl.adjoint ? 4.0 : 2.0);
     JitPlan* jp = nullptr;
     if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
@@ -575,6 +589,33 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         if (jp) count_jit_launch();
         if (one_tile) g_jit_sb_launches.fetch_add(1);
     }
+}
+
+// Device rows [b0, b0 + nb) of a circuit's custom-gate matrices (a one-row table serves every sample),
+// in the simulator's workspace `which` (0 or 1: the two circuits of one call use separate ones).
+// one_row: every state of the launch belongs to sample b0 (row b0 for all of them).
+CustomRows custom_rows(mqa_sim* sim, mqa_circuit* c, int64_t b0, int64_t nb, int which = 0, bool one_row = false) {
+    CustomRows r;
+    if (c == nullptr || c->custom_need == 0) return r;
+    std::lock_guard<std::mutex> lk(c->mu);
+    require(c->custom_rows >= 1, "the circuit has parameterised matrix gates: set their values first "
+                                 "(mqa_circuit_set_custom_matrices)");
+    const bool shared = c->custom_rows == 1;
+    const int64_t need_rows = one_row ? b0 + 1 : b0 + nb;
+    require(shared || c->custom_rows >= need_rows, "custom gate matrices hold " + std::to_string(c->custom_rows) +
+                                                       " rows, the call needs " + std::to_string(need_rows));
+    const bool single = shared || one_row;
+    const size_t n = size_t(single ? 1 : nb) * size_t(c->custom_stride);
+    DevBuf& buf = sim->custom[which];
+    buf.ensure(n * sizeof(double));
+    hip_check(hipMemcpyAsync(buf.p, c->custom_table.data() + (shared ? 0 : size_t(b0) * size_t(c->custom_stride)),
+                             n * sizeof(double), hipMemcpyHostToDevice, sim->stream),
+              "custom matrices");
+    // the host table may change after the call returns: the copy completes before that (every entry
+    // point synchronises its stream before returning)
+    r.p = buf.as<double>();
+    r.stride = single ? 0 : uint64_t(c->custom_stride);
+    return r;
 }
 
 // (cos, sin) table for `batch` rows: params p < n_enc from enc rows, the rest from ans.
@@ -1041,7 +1082,8 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
             for (int q = 0; q < 64; q++)
                 if ((s.ctrl_mask >> q) & 1) c->max_qubit = std::max(c->max_qubit, q);
             require(!(g.targets() & g.ctrl), "gate " + std::to_string(i) + ": a qubit is both target and control");
-            bool param_kind = s.kind >= MQA_GATE_RX && s.kind <= MQA_GATE_RZZ;
+            const bool custom_kind = s.kind == MQA_GATE_PMATRIX || s.kind == MQA_GATE_PMATRIX_TAP;
+            bool param_kind = (s.kind >= MQA_GATE_RX && s.kind <= MQA_GATE_RZZ) || custom_kind;
             bool fixed_kind = (s.kind >= MQA_GATE_I && s.kind <= MQA_GATE_MATRIX);
             if (!param_kind && !fixed_kind)
                 throw Error(MQA_ERR_UNSUPPORTED, "Apply of gate " + std::to_string(s.kind) + " not implement.");
@@ -1049,8 +1091,16 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
             if (s.kind == MQA_GATE_SWAP || s.kind == MQA_GATE_ISWAP || s.kind == MQA_GATE_ISWAPDAG ||
                 s.kind == MQA_GATE_RXX || s.kind == MQA_GATE_RYY || s.kind == MQA_GATE_RZZ)
                 want = 2;
-            if (s.kind != MQA_GATE_MATRIX)
+            if (custom_kind) {
+                require(s.n_obj <= 2, "gate " + std::to_string(i) +
+                                          ": parameterised matrix gates act on one or two target qubits");
+                require(s.mat_begin >= 0, "gate " + std::to_string(i) + ": bad custom matrix offset");
+                const int64_t dd = int64_t(2) << (2 * s.n_obj);  // doubles of one D x D matrix
+                g.custom_off = s.mat_begin;
+                c->custom_need = std::max(c->custom_need, int64_t(s.mat_begin) + (s.kind == MQA_GATE_PMATRIX ? 3 : 1) * dd);
+            } else if (s.kind != MQA_GATE_MATRIX) {
                 require(s.n_obj == want, "gate " + std::to_string(i) + ": wrong number of target qubits");
+            }
             if (s.kind == MQA_GATE_MATRIX) {
                 int64_t m = int64_t(1) << s.n_obj;
                 require(mats != nullptr && s.mat_begin >= 0 && s.mat_begin + m * m <= n_mat_entries,
@@ -1122,6 +1172,19 @@ int mqa_circuit_get_tile_qubits(const mqa_circuit* circ, int32_t dtype, int32_t 
         require(circ != nullptr && tile_qubits != nullptr, "NULL argument");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         *tile_qubits = circ->options(dtype, adjoint != 0).tile_qubits;
+    });
+}
+
+int mqa_circuit_set_custom_matrices(mqa_circuit* circ, int64_t rows, int64_t stride, const double* table) {
+    return guarded([&] {
+        require(circ != nullptr, "circuit is NULL");
+        require(rows >= 1 && stride >= circ->custom_need && table != nullptr,
+                "custom matrix table: need rows >= 1 and a row of at least " + std::to_string(circ->custom_need) +
+                    " doubles");
+        std::lock_guard<std::mutex> lk(circ->mu);
+        circ->custom_table.assign(table, table + rows * stride);
+        circ->custom_rows = rows;
+        circ->custom_stride = stride;
     });
 }
 
@@ -1379,7 +1442,7 @@ int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double*
         } else {
             ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
         }
-        run_plan(sim, dp, sim->state.p, nullptr, 1, ang, circ->n_slots, nullptr, 0);
+        run_plan(sim, dp, sim->state.p, nullptr, 1, ang, circ->n_slots, nullptr, 0, 0, -1, custom_rows(sim, circ, 0, 1));
         if (phase != std::complex<double>(1.0, 0.0))
             hip_check(launch_scale(sim->dtype, sim->state.p, sim->state.p, sim->dim_eff, phase.real(), phase.imag(),
                                    sim->stream),
@@ -1427,7 +1490,7 @@ int mqa_sim_expectation_lr(const mqa_sim* sim_c, const mqa_sim* left_c, const mq
             auto* cr = const_cast<mqa_circuit*>(circ_right);
             DevPlan* dp = cr->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
             const double* ang = build_angles(sim, cr, nullptr, 1, 0, params, n_params);
-            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, cr->n_slots, nullptr, 0);
+            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, cr->n_slots, nullptr, 0, 0, -1, custom_rows(sim, cr, 0, 1, 0));
         }
         const void* bra = sim->psi.p;
         bool two = (left != nullptr && left != sim) || (circ_left != nullptr && circ_left != circ_right);
@@ -1440,7 +1503,8 @@ int mqa_sim_expectation_lr(const mqa_sim* sim_c, const mqa_sim* left_c, const mq
                 auto* clm = const_cast<mqa_circuit*>(cl);
                 DevPlan* dp = clm->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true);
                 const double* ang = build_angles(sim, clm, nullptr, 1, 0, params, n_params);
-                run_plan(sim, dp, sim->lam.p, nullptr, 1, ang, clm->n_slots, nullptr, 0);
+                run_plan(sim, dp, sim->lam.p, nullptr, 1, ang, clm->n_slots, nullptr, 0, 0, -1,
+                         custom_rows(sim, clm, 0, 1, 1));
             }
             bra = sim->lam.p;
         }
@@ -1612,17 +1676,19 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
             const int64_t nb = std::min<int64_t>(chunk, batch - b0);
             const double* ang = build_angles(sim, circ, n_enc ? enc + b0 * n_enc : nullptr, nb, n_enc, ans, n_ans);
+            const CustomRows cust = custom_rows(sim, circ, b0, nb);
             // forward sweep into psi (and the checkpoints)
             auto forward = [&] {
                 hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff,
                                       nb, s),
                           "fill");
                 if (!ck) {
-                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust);
                     return;
                 }
                 for (int k = 0; k <= n_ck; k++) {
-                    run_plan(sim, ck->segs[size_t(k)].get(), sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0);
+                    run_plan(sim, ck->segs[size_t(k)].get(), sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0,
+                             -1, cust);
                     if (k < n_ck)
                         hip_check(hipMemcpyAsync(ckpt_at(n_ck - 1 - k), sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s),
                                   "checkpoint");
@@ -1649,12 +1715,12 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                     void* cur = sim->psi.p;
                     for (int i = 0; i < n_ck; i++) {  // back-sweep segment, then continue from checkpoint i
                         run_plan(sim, adj, cur, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(),
-                                 n_tap_ids, p0, ck->bound[size_t(i)]);
+                                 n_tap_ids, p0, ck->bound[size_t(i)], cust);
                         p0 = ck->bound[size_t(i)];
                         cur = ckpt_at(i);
                     }
                     run_plan(sim, adj, cur, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(), n_tap_ids,
-                             p0);
+                             p0, -1, cust);
                     hip_check(launch_reduce_rows1(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
                                                   uint64_t(nb) * n_tap_ids, s),
                               "reduce taps");
@@ -1993,6 +2059,8 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             require(n >= 1 && n <= 36, "item " + std::to_string(i) + ": bad qubit count");
             require(c->max_qubit < n, "item " + std::to_string(i) + ": circuit is wider than its qubit count");
             require(!h->csr, "item " + std::to_string(i) + ": sparse (CSR) hamiltonians are not supported in item batches");
+            require(c->custom_need == 0, "item " + std::to_string(i) + ": parameterised matrix gates are not supported "
+                                                                         "in item batches");
             require((h->qubit_mask >> n) == 0, "item " + std::to_string(i) + ": hamiltonian is wider than its qubit count");
             require(c->n_params == 0 || params != nullptr, "params is NULL");
             GradItem it;
@@ -2089,11 +2157,13 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
             const double* ang[2];
             for (int i = 0; i < 2; i++)
                 ang[i] = build_angles(sim, side[i].c, n_enc ? enc2.data() : nullptr, 2, n_enc, ans, n_ans, &side[i].ang);
+            CustomRows cust[2];
+            for (int i = 0; i < 2; i++) cust[i] = custom_rows(sim, side[i].c, b, 1, i, true);
             // forward states: psi_r twice (slots 0, 1), phi_l twice (slots 2, 3)
             hip_check(launch_fill(sim->dtype, P, sim->state.p, dim, dim, dim, 2, s), "fill");
-            run_plan(sim, side[0].fwd, P, nullptr, 2, ang[0], cr->n_slots, nullptr, 0);
+            run_plan(sim, side[0].fwd, P, nullptr, 2, ang[0], cr->n_slots, nullptr, 0, 0, -1, cust[0]);
             hip_check(launch_fill(sim->dtype, P + 2 * sb, left->state.p, dim, dim, dim, 2, s), "fill");
-            run_plan(sim, side[1].fwd, P + 2 * sb, nullptr, 2, ang[1], cl->n_slots, nullptr, 0);
+            run_plan(sim, side[1].fwd, P + 2 * sb, nullptr, 2, ang[1], cl->n_slots, nullptr, 0, 0, -1, cust[1]);
             for (int h = 0; h < n_hams; h++) {
                 // the back-sweeps of the previous hamiltonian consumed the forward pairs: redo them
                 for (int i = 0; h > 0 && i < 2; i++) {
@@ -2101,7 +2171,8 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
                     char* psi_pair = P + size_t(2 * i) * sb;
                     hip_check(launch_fill(sim->dtype, psi_pair, (i == 0 ? sim : left)->state.p, dim, dim, dim, 2, s),
                               "fill");
-                    run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0);
+                    run_plan(sim, side[i].fwd, psi_pair, nullptr, 2, ang[i], side[i].c->n_slots, nullptr, 0, 0, -1,
+                             cust[i]);
                 }
                 auto* ham = const_cast<mqa_ham*>(hams[h]);
                 ham_apply(sim, ham, P + 2 * sb, nullptr, L, 1, true);  // H^dagger phi_l
@@ -2117,7 +2188,7 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim_c, const mqa_sim* left_c, const mqa_
                     char* psi_pair = P + size_t(2 * i) * sb;
                     char* lam_pair = L + size_t(2 * i) * sb;  // right sweep: H phi_l; left: H psi_r
                     run_plan(sim, side[i].adj, psi_pair, lam_pair, 2, ang[i], side[i].c->n_slots,
-                             side[i].part.as<double>(), side[i].c->n_tap_ids);
+                             side[i].part.as<double>(), side[i].c->n_tap_ids, 0, -1, cust[i]);
                     hip_check(launch_reduce_rows1(side[i].part.as<double>(), side[i].n_tiles, side[i].taps.as<double>(),
                                                   uint64_t(2) * side[i].c->n_tap_ids, s),
                               "reduce taps");
@@ -2175,7 +2246,7 @@ int mqa_sim_sample(const mqa_sim* sim_c, const mqa_circuit* circ_c, const double
         if (circ && !circ->gates.empty()) {
             DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, s, true);
             const double* ang = build_angles(sim, circ, nullptr, 1, 0, params, n_params);
-            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, circ->n_slots, nullptr, 0);
+            run_plan(sim, dp, sim->psi.p, nullptr, 1, ang, circ->n_slots, nullptr, 0, 0, -1, custom_rows(sim, circ, 0, 1));
         }
         // cumulative probability by blocks (device sums, host prefix), then the search inside the
         // blocks the shots land in (GetCumulativeProbs / LowerBound, vector_state.tpp:1357-1361)
@@ -2233,6 +2304,8 @@ int mqa_sim_expectation_angles(const mqa_sim* sim_c, const mqa_ham* const* hams,
         auto* circ = const_cast<mqa_circuit*>(circ_c);
         require(circ->max_qubit < sim->n, "circuit is wider than the simulator");
         require(circ->n_slots == 0 || angles != nullptr, "angles is NULL");
+        require(circ->custom_need == 0, "angle settings do not reach parameterised matrix gates (their matrices come "
+                                        "from mqa_circuit_set_custom_matrices)");
         for (int h = 0; h < n_hams; h++) check_ham(sim, hams[h]);
         DeviceGuard dg(sim->device);
         hipStream_t s = sim->stream;

@@ -426,6 +426,88 @@ __device__ __forceinline__ void dense_from_lds(Regs<T>& a, const GroupCtx& g, ui
     }
 }
 
+// ---------------------------------------------------------------- parameterised dense ops
+// Custom gates whose matrix is a host function of their parameters: a sample's row of the custom
+// table holds, at the op's offset, [G][G^dagger][M] (D x D interleaved complex doubles, D = 2^NQ) with
+// M = dG G^dagger the tap matrix; a tap-only op (a further parameter expression) holds [M].
+template <typename T, int NQ>
+struct DMat {
+    T m[2 << (2 * NQ)];
+};
+template <typename T, int NQ>
+__device__ __forceinline__ DMat<T, NQ> load_dmat(const double* p) {
+    DMat<T, NQ> d;
+#pragma unroll
+    for (int i = 0; i < (2 << (2 * NQ)); i++) d.m[i] = T(p[i]);
+    return d;
+}
+
+// Re <l|M|p> over the op's control subspace (M a dense 2x2 on register position Q).
+template <typename T, int Q, bool CTRL>
+__device__ __forceinline__ T dense1_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl,
+                                        const T* __restrict__ m) {
+    T s = 0;
+    for_pairs<Q>([&]<int V, int W>() {
+        T xr = p.r[V], xi = p.i[V], yr = p.r[W], yi = p.i[W];
+        mat2(m, xr, xi, yr, yi);
+        T u = 0;
+        cdot_part<false>(l.r[V], l.i[V], xr, xi, u);
+        cdot_part<false>(l.r[W], l.i[W], yr, yi, u);
+        if constexpr (CTRL) s += ctrl_ok<V>(g, cl) ? u : T(0);
+        else s += u;
+    });
+    return s;
+}
+
+// The same for a dense 4x4 on register positions (Q0, Q1) (quad order 00, 01, 10, 11, bit 0 <=> Q0).
+template <typename T, int Q0, int Q1, bool CTRL>
+__device__ __forceinline__ T dense2_tap(const Regs<T>& p, const Regs<T>& l, const GroupCtx& g, uint32_t cl,
+                                        const T* __restrict__ m) {
+    T s = 0;
+    for_quads<Q0, Q1>([&]<int V0, int V1, int V2_, int V3>() {
+        if constexpr (CTRL) {
+            if (!ctrl_ok<V0>(g, cl)) return;
+        }
+        const int idx[4] = {V0, V1, V2_, V3};
+#pragma unroll
+        for (int row = 0; row < 4; row++) {
+            T ur = 0, ui = 0;
+#pragma unroll
+            for (int col = 0; col < 4; col++) {
+                const T mr = m[2 * (row * 4 + col)], mi = m[2 * (row * 4 + col) + 1];
+                ur += mr * p.r[idx[col]] - mi * p.i[idx[col]];
+                ui += mr * p.i[idx[col]] + mi * p.r[idx[col]];
+            }
+            cdot_part<false>(l.r[idx[row]], l.i[idx[row]], ur, ui, s);
+        }
+    });
+    return s;
+}
+
+// One parameterised dense op.  Forward: G on p.  Back-sweep (ADJ): Re(tap) with M (returned), then
+// G^dagger on p and l; a tap-only op (TAP_ONLY) changes nothing.
+template <typename T, bool ADJ, int NQ, int Q0, int Q1, bool CTRL, bool TAP_ONLY>
+__device__ __forceinline__ T pm_op(Regs<T>& p, Regs<T>& l, const GroupCtx& g, uint32_t cl, const double* m) {
+    constexpr int DD = 2 << (2 * NQ);  // doubles of one matrix
+    T tr = 0;
+    if constexpr (ADJ) {
+        const DMat<T, NQ> mt = load_dmat<T, NQ>(TAP_ONLY ? m : m + 2 * DD);
+        if constexpr (NQ == 1) tr = dense1_tap<T, Q0, CTRL>(p, l, g, cl, mt.m);
+        else tr = dense2_tap<T, Q0, Q1, CTRL>(p, l, g, cl, mt.m);
+    }
+    if constexpr (!TAP_ONLY) {
+        const DMat<T, NQ> gm = load_dmat<T, NQ>(ADJ ? m + DD : m);
+        if constexpr (NQ == 1) {
+            one_target<T, OPK_M1, Q0, CTRL>(p, g, cl, T(0), T(0), gm.m);
+            if constexpr (ADJ) one_target<T, OPK_M1, Q0, CTRL>(l, g, cl, T(0), T(0), gm.m);
+        } else {
+            two_target<T, OPK_M2, Q0, Q1, CTRL>(p, g, cl, T(0), T(0), gm.m);
+            if constexpr (ADJ) two_target<T, OPK_M2, Q0, Q1, CTRL>(l, g, cl, T(0), T(0), gm.m);
+        }
+    }
+    return tr;
+}
+
 // Calls f.template operator()<KIND, Q0, Q1>() for an op's kind and register positions: one
 // switch per op, so the tap and both register sets are updated inside a single case body (one
 // control-flow merge per op, which keeps the register allocator from copying the state).
@@ -502,9 +584,37 @@ __device__ __forceinline__ void apply_body(Regs<T>& a, const GroupCtx& g, uint32
 template <typename T, bool ADJ, bool FULL>
 __device__ __forceinline__ void run_op(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g, uint64_t base,
                                        const T* __restrict__ pool, const double* __restrict__ ang_row,
-                                       TapAcc<T>& acc) {
+                                       TapAcc<T>& acc, const double* __restrict__ pm_row = nullptr) {
     T tr = 0;  // Re(tap); zero when a non-local control is off (the op is then a no-op)
     int kind = op.kind;
+    if constexpr (FULL) {  // parameterised dense ops run in FULL plans only, on register positions 0 (and 1)
+        if (kind == OPK_PM || kind == OPK_PMTAP) {
+            if ((base & op.ctrl_global) == op.ctrl_global) {
+                const uint32_t cl = op.ctrl_local;
+                const double* m = pm_row + op.mat;
+                const bool tap_only = kind == OPK_PMTAP;
+                if (op.pad0 == 2) {
+                    if (cl) tr = tap_only ? pm_op<T, ADJ, 2, 0, 1, true, true>(p, l, g, cl, m)
+                                          : pm_op<T, ADJ, 2, 0, 1, true, false>(p, l, g, cl, m);
+                    else tr = tap_only ? pm_op<T, ADJ, 2, 0, 1, false, true>(p, l, g, cl, m)
+                                       : pm_op<T, ADJ, 2, 0, 1, false, false>(p, l, g, cl, m);
+                } else {
+                    if (cl) tr = tap_only ? pm_op<T, ADJ, 1, 0, 0, true, true>(p, l, g, cl, m)
+                                          : pm_op<T, ADJ, 1, 0, 0, true, false>(p, l, g, cl, m);
+                    else tr = tap_only ? pm_op<T, ADJ, 1, 0, 0, false, true>(p, l, g, cl, m)
+                                       : pm_op<T, ADJ, 1, 0, 0, false, false>(p, l, g, cl, m);
+                }
+            }
+            if constexpr (ADJ) {
+                if (op.tap >= 0) {
+#pragma unroll
+                    for (int j = kTapSlots<T> - 1; j > 0; j--) acc.r[j] = acc.r[j - 1];
+                    acc.r[0] = tr;
+                }
+            }
+            return;
+        }
+    }
     if ((base & op.ctrl_global) == op.ctrl_global && kind != OPK_PF_GP) {
         const uint32_t cl = op.ctrl_local;
         T c = 0, s = 0;

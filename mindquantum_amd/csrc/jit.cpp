@@ -133,7 +133,7 @@ std::vector<uint8_t> jit_coef_forms(const Plan& pl) {
     std::vector<uint8_t> forms;
     std::vector<bool> clash;
     for (const DevOp& op : pl.dev_ops) {
-        if (!opk_is_param(op.kind) || op.slot < 0 || op.kind == OPK_PF_GP) continue;
+        if (!opk_is_param(op.kind) || op.slot < 0 || op.kind == OPK_PF_GP || opk_is_custom(op.kind)) continue;
         const uint8_t f = op.kind == OPK_PF_RX || op.kind == OPK_PF_RY     ? COEF_SHEAR
                           : op.kind == OPK_PF_RZ || op.kind == OPK_PF_RZZ ? COEF_DOUBLE
                                                                           : COEF_CS;
@@ -188,6 +188,10 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
     o << "  const_ptr<double> ang_row = args.angles ? as_const(args.angles + 2 * f.sample * uint64_t(args.n_slots)) "
          ": nullptr;\n"
       << "  (void)ang_row;\n";
+    bool has_custom = false;
+    for (const DevOp& op : pl.dev_ops) has_custom = has_custom || opk_is_custom(op.kind);
+    if (has_custom)
+        o << "  const double* pm_row = args.custom + f.sample * args.custom_stride;\n";
     const std::vector<uint8_t> forms = jit_coef_forms(pl);
     auto ready = [&](const DevOp& op) {
         return op.slot >= 0 && size_t(op.slot) < forms.size() && forms[size_t(op.slot)] != COEF_NONE;
@@ -273,6 +277,19 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
                     b << "      dense_from_lds<T, " << m << ">(l, g, " << op.ctrl_local << "u, pool + " << op.mat
                       << ", f.t_lam);\n";
                 b << "    }\n";
+                continue;
+            }
+            if (opk_is_custom(op.kind)) {  // parameterised dense ops: matrices from the per-sample table
+                const int nq = op.pad0 == 2 ? 2 : 1;
+                b << "      ";
+                if (adj && op.tap >= 0) {
+                    const int var = int(taps.size());
+                    taps.push_back({var, op.tap});
+                    b << "tap" << var << " = ";
+                }
+                b << "pm_op<T, " << A << ", " << nq << ", " << int(op.q0) << ", " << (nq == 2 ? int(op.q1) : 0) << ", "
+                  << ctrl << ", " << (op.kind == OPK_PMTAP ? "true" : "false") << ">(p, l, g, " << op.ctrl_local
+                  << "u, pm_row + " << op.mat << ");\n    }\n";
                 continue;
             }
             if (op.kind >= OPK_PF_RX) {  // phase-free forms: never controlled

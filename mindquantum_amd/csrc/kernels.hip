@@ -115,6 +115,7 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
     }
     TileFrame<T> f = tile_begin<T, ADJ>(args, pass, smem, tile, sample);
     const double* ang_row = args.angles ? args.angles + 2 * f.sample * uint64_t(args.n_slots) : nullptr;
+    const double* pm_row = args.custom ? args.custom + f.sample * args.custom_stride : nullptr;
 
     for (int gi = 0; gi < pass.n_groups; gi++) {
         const DevGroup grp = args.groups[pass.group_begin + gi];
@@ -143,7 +144,7 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
         } else {
             for (int oi = 0; oi < grp.n_ops; oi++) {
                 const DevOp op = args.ops[grp.op_begin + oi];
-                run_op<T, ADJ, FULL>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc);
+                run_op<T, ADJ, FULL>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc, pm_row);
             }
         }
         if constexpr (ADJ) {

@@ -62,6 +62,12 @@ enum OpKind : int32_t {
     OPK_PF_RZ = 26,
     OPK_PF_GP = 27,
     OPK_PF_RZZ = 28,
+    // Parameterised dense 2x2 / 4x4 (custom gates): the matrices come from the per-sample table
+    // TilePassArgs::custom at DevOp::mat (doubles): [G][G^dagger][M] with M = dG G^dagger, the tap
+    // matrix (Re(tap) = Re <lambda|M|psi>, taken before G^dagger is applied); DevOp::pad0 holds the
+    // target count.  PMTAP: the tap of a further parameter expression only ([M] at DevOp::mat).
+    OPK_PM = 29,
+    OPK_PMTAP = 30,
 };
 
 // Coefficient forms of a slot in the compiled passes' coefficient table (TilePassArgs::coefs),
@@ -79,6 +85,7 @@ inline bool opk_is_two(int k) {
     return k == OPK_SWAP || k == OPK_M2 || k == OPK_RXX || k == OPK_RYY || k == OPK_RZZ || k == OPK_CX || k == OPK_CZ ||
            k == OPK_PF_RZZ;
 }
+inline bool opk_is_custom(int k) { return k == OPK_PM || k == OPK_PMTAP; }
 
 // Byte layout of a tile pass's dynamic LDS: [psi tile][lambda tile (adjoint)][offset table]
 // [tap partials].  Shared by the kernel and the host launcher so the two cannot disagree.
@@ -179,6 +186,8 @@ struct TilePassArgs {
     double* partials;      // [batch][n_tap_ids][n_tiles] Re(tap)
     uint64_t batch;        // states
     const void* coefs;     // [batch][n_slots][2] in the real type, per CoefForm (compiled passes)
+    const double* custom;  // per-sample matrices of parameterised dense ops (OPK_PM / OPK_PMTAP)
+    uint64_t custom_stride;  // doubles between samples' rows (0: one row for every sample)
 };
 
 // Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are

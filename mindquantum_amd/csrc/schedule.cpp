@@ -91,6 +91,8 @@ static int lower_kind(const CGate& g, bool adjoint, std::vector<cplx>* mat) {
         case MQA_GATE_RXX: return OPK_RXX;
         case MQA_GATE_RYY: return OPK_RYY;
         case MQA_GATE_RZZ: return OPK_RZZ;
+        case MQA_GATE_PMATRIX: return OPK_PM;
+        case MQA_GATE_PMATRIX_TAP: return OPK_PMTAP;
         default: throw std::invalid_argument("Apply of gate " + std::to_string(g.kind) + " not implement.");
     }
 }
@@ -127,6 +129,13 @@ struct PassBuilder {
                 }
                 std::vector<cplx> m;
                 int kind = lower_kind(g, adjoint, &m);
+                // a parameterised dense op opens a register group, so that its targets take register
+                // positions 0 (and 1): the interpreted pass then needs one instantiation per shape
+                if (opk_is_custom(kind) && !grp.ops.empty()) {
+                    blocked |= q;
+                    keep.push_back(gi);
+                    continue;
+                }
                 int reg_ctrl = -1;  // control qubit bound to a register position (CX / CZ)
                 if (kind == OPK_M3 || kind == OPK_M4) {
                     if (!grp.ops.empty()) {  // dense multi-qubit blocks take a group of their own
@@ -184,6 +193,11 @@ struct PassBuilder {
                     grp.n_taps++;
                 }
                 d.mat = -1;
+                d.pad0 = 0;
+                if (opk_is_custom(kind)) {  // per-sample matrices: the offset in a sample's row
+                    d.mat = g.custom_off;
+                    d.pad0 = uint8_t(g.objs.size());
+                }
                 if (!m.empty()) {
                     d.mat = int(pool->size());
                     for (auto& c : m) {

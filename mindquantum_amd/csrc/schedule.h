@@ -21,6 +21,7 @@ struct CGate {
     double const_val = 0;                      // angle = const + sum coeff * param
     std::vector<std::pair<int, double>> terms;  // (param index, coefficient)
     std::vector<cplx> mat;                     // MATRIX kind, row-major
+    int custom_off = -1;                       // PMATRIX kinds: offset (doubles) in a sample's custom row
     uint64_t qubits() const;                   // targets | controls
     uint64_t targets() const;
 };

@@ -19,32 +19,66 @@ from .core.mutation import epoch
 from .core.parameterresolver import ParameterResolver
 
 
-def lower(circ, names):
-    """Return (gates, term_param, term_coeff, mats, requires_grad_per_param) numpy arrays."""
+def _is_custom(g):
+    return getattr(g, "ir_kind", None) == "PMATRIX"
+
+
+def lower(circ, names, layout=None):
+    """Return (gates, term_param, term_coeff, mats, n_mat_entries, requires_grad_per_param) numpy arrays.
+
+    Parameterised matrix gates (custom gates, parameterised Rn) become one MQA_GATE_PMATRIX record
+    plus one MQA_GATE_PMATRIX_TAP record per further parameter expression; ``layout`` (a list, if
+    given) receives (gate, offset, n_targets, n_expressions) for each of them, the offset (in doubles)
+    of its block [G][G^dagger][M_0][M_1]... in a sample's row of the custom-matrix table."""
     index = {n: i for i, n in enumerate(names)}
     prims = [p for g in circ for p in g.lowered()]
-    gates = np.zeros(len(prims), dtype=_native.GATE_DTYPE)
+    records = []  # (gate, ir kind, parameter resolver or None, custom offset)
+    offset = 0
+    for g in prims:
+        if _is_custom(g):
+            k = len(g.obj_qubits)
+            if k > 2:
+                raise ValueError(f"parameterised matrix gate {g.name} acts on {k} qubits; the HIP backend runs them "
+                                 "on one or two target qubits")
+            prs = g.param_resolvers()
+            dd = 2 << (2 * k)
+            if layout is not None:
+                layout.append((g, offset, k, len(prs)))
+            records.append((g, "PMATRIX", prs[0], offset))
+            for j in range(1, len(prs)):
+                records.append((g, "PMATRIX_TAP", prs[j], offset + (2 + j) * dd))
+            offset += (2 + len(prs)) * dd
+        else:
+            records.append((g, None, None, -1))
+    gates = np.zeros(len(records), dtype=_native.GATE_DTYPE)
     t_param, t_coeff, mats = [], [], []
     no_grad = set()
-    for i, g in enumerate(prims):
+
+    def terms_of(rec, pr):
+        rec["const_val"] = float(np.real(pr.const))
+        rec["term_begin"] = len(t_param)
+        for name, c in pr.items():
+            if name not in index:
+                raise ValueError(f"Parameter {name} is not in the parameter list")
+            t_param.append(index[name])
+            t_coeff.append(float(np.real(c)))
+        rec["n_term"] = len(pr)
+        no_grad.update(pr.no_grad_parameters)
+        rec["requires_grad"] = 1 if pr.requires_grad_parameters else 0
+
+    for i, (g, custom_kind, pr, off) in enumerate(records):
         rec = gates[i]
         rec["n_obj"] = len(g.obj_qubits)
         for k, q in enumerate(g.obj_qubits):
             rec["obj"][k] = q
         rec["ctrl_mask"] = sum(1 << c for c in g.ctrl_qubits)
-        if isinstance(g, G.ParameterGate):
+        if custom_kind is not None:
+            rec["kind"] = _native.GATE_KIND[custom_kind]
+            rec["mat_begin"] = off
+            terms_of(rec, pr)
+        elif isinstance(g, G.ParameterGate):
             rec["kind"] = _native.GATE_KIND[g.ir_kind]
-            pr = g.coeff
-            rec["const_val"] = float(np.real(pr.const))
-            rec["term_begin"] = len(t_param)
-            for name, c in pr.items():
-                if name not in index:
-                    raise ValueError(f"Parameter {name} is not in the parameter list")
-                t_param.append(index[name])
-                t_coeff.append(float(np.real(c)))
-            rec["n_term"] = len(pr)
-            no_grad.update(pr.no_grad_parameters)
-            rec["requires_grad"] = 1 if pr.requires_grad_parameters else 0
+            terms_of(rec, g.coeff)
         elif isinstance(g, G.NoneParameterGate):
             kind = g.ir_kind
             rec["kind"] = _native.GATE_KIND[kind]
@@ -64,6 +98,28 @@ def lower(circ, names):
             len(mats), rg)
 
 
+def custom_table(layout, names, values):
+    """Rows of the custom-matrix table (mqa_circuit_set_custom_matrices) for parameter values
+    ``values`` [rows, len(names)]: per gate the matrix G, G^dagger and M_j = dG/dv_j G^dagger of each
+    parameter expression v_j, evaluated on the host (the reference evaluates its numba functions per
+    sample, vector_state.tpp:402-413)."""
+    values = np.atleast_2d(np.asarray(values, dtype=np.float64))
+    rows = values.shape[0]
+    stride = max(off + (2 + m) * (2 << (2 * k)) for _, off, k, m in layout)
+    table = np.zeros((rows, stride), dtype=np.float64)
+    index = {n: i for i, n in enumerate(names)}
+    for r in range(rows):
+        pr = {n: values[r, i] for n, i in index.items()}
+        for g, off, k, m in layout:
+            gm = np.asarray(g.matrix(pr), dtype=np.complex128)
+            blocks = [gm, gm.conj().T] + [np.asarray(d, dtype=np.complex128) @ gm.conj().T
+                                          for d in g.diff_matrices(pr)]
+            flat = np.concatenate([b.ravel() for b in blocks])
+            table[r, off:off + 2 * flat.size:2] = flat.real
+            table[r, off + 1:off + 2 * flat.size:2] = flat.imag
+    return table, stride
+
+
 def digest(arrays):
     """Content digest of lowered arrays (gate records, angle terms, matrices, no-grad flags)."""
     gates, tp, tc, mats, n_mats, rg = arrays
@@ -77,9 +133,13 @@ def digest(arrays):
 class NativeCircuit:
     """C ABI circuit handle for (circuit, parameter order)."""
 
-    def __init__(self, circ, names, arrays=None):
+    def __init__(self, circ, names, arrays=None, layout=None):
         self.names = list(names)
-        self.arrays = lower(circ, self.names) if arrays is None else arrays
+        if arrays is None:
+            layout = []
+            arrays = lower(circ, self.names, layout)
+        self.arrays = arrays
+        self.custom = layout or []  # (gate, offset, targets, expressions) of parameterised matrix gates
         self.digest = digest(self.arrays)
         gates, tp, tc, mats, n_mats, rg = self.arrays
         h = ctypes.c_void_p()
@@ -92,6 +152,16 @@ class NativeCircuit:
         # tile widths chosen by the autotuner for this circuit: {(dtype code, adjoint): qubits}
         for (code, adjoint), k in getattr(circ, "_mqa_tiles", {}).items():
             self.set_tile_qubits(code, adjoint, k)
+
+    def set_custom(self, values):
+        """Evaluate the parameterised matrix gates for the parameter rows ``values`` [rows, len(names)]
+        (one row: shared by every sample) and hand them to the library for the calls that follow."""
+        if not self.custom:
+            return
+        table, stride = custom_table(self.custom, self.names, values)
+        table = np.ascontiguousarray(table)
+        _native.check(_native.lib().mqa_circuit_set_custom_matrices(self.handle, table.shape[0], stride,
+                                                                     _native.ptr(table)))
 
     def set_tile_qubits(self, dtype_code, adjoint, tile_qubits):
         """Per-circuit tile width of the forward (adjoint False) or back-sweep passes; negative
@@ -155,12 +225,14 @@ class NativeCache:
             if hit[2] == now:
                 self._d.move_to_end(key)
                 return hit[1]
-            arrays = lower(circ, names)
+            layout = []
+            arrays = lower(circ, names, layout)
             if digest(arrays) == hit[1].digest:
+                hit[1].custom = layout  # the same content: take the current gate objects
                 self._d[key] = (hit[0], hit[1], now)
                 self._d.move_to_end(key)
                 return hit[1]
-            nc = NativeCircuit(circ, names, arrays)
+            nc = NativeCircuit(circ, names, arrays, layout)
         else:
             nc = NativeCircuit(circ, names)
         ref = weakref.ref(circ, lambda _r, k=key, d=self._d: d.pop(k, None) if k in d and d[k][0] is _r else None)

@@ -142,6 +142,7 @@ class MQVectorAMD:
             raise ValueError("apply_circuit here runs unitary circuits; sample measurements with sampling()")
         names, vals = resolve(circuit, pr)
         nc = self._native_circuit(circuit, names)
+        nc.set_custom(vals[None, :])
         _native.check(_native.lib().mqa_sim_apply_circuit(self._h, nc.handle, _native.ptr(vals), len(vals)))
         return None
 
@@ -177,7 +178,9 @@ class MQVectorAMD:
         handle, vals = None, np.zeros(0)
         if len(body):
             names, vals = resolve(body, pr)
-            handle = self._native_circuit(body, names).handle
+            nc = self._native_circuit(body, names)
+            nc.set_custom(vals[None, :])
+            handle = nc.handle
         _native.check(_native.lib().mqa_sim_sample(self._h, handle, _native.ptr(vals) if len(vals) else None,
                                                     len(vals), _native.ptr(u), int(shots), _native.ptr(idx)))
         res = MeasureResult()
@@ -249,6 +252,9 @@ class MQVectorAMD:
                 raise ValueError("Parameterized circuit needs a parameter resolver.")
             mapping = dict(pr.items()) if hasattr(pr, "items") else dict(zip(names, np.asarray(pr).ravel()))
             vals = np.array([float(np.real(mapping[n])) for n in names], dtype=np.float64)
+        ncs = [self._native_circuit(c, names) for c in (circ_right, circ_left) if c is not None and len(c)]
+        for nc in ncs:
+            nc.set_custom(vals[None, :])
         cr = self._native_circuit(circ_right, names).handle if len(circ_right) else None
         cl = self._native_circuit(circ_left, names).handle if circ_left is not None and len(circ_left) else None
         left = simulator_left._h if simulator_left is not None else None
@@ -334,6 +340,11 @@ class MQVectorAMD:
                 enc = np.zeros((1, 0), dtype=np.float64)
                 ans = np.ascontiguousarray(inputs[0], dtype=np.float64)
             batch = enc.shape[0]
+            if nc.custom or (ncl is not None and ncl.custom):  # per-sample matrices of custom gates
+                rows = np.concatenate([enc, np.broadcast_to(ans, (batch, ans.size))], axis=1)
+                nc.set_custom(rows)
+                if ncl is not None:
+                    ncl.set_custom(rows)
             if pr_shift:
                 from .param_shift import gradient as shift_gradient
 

@@ -55,6 +55,9 @@ def gradient(sim, hams, circ, enc_names, ans_names, enc, ans):
     names = list(enc_names) + list(ans_names)
     n_enc = len(enc_names)
     nc = NativeCircuit(circ, names)
+    if nc.custom:
+        raise ValueError("parameter-shift gradients of parameterised matrix gates (custom gates, parameterised Rn) "
+                         "are not supported: their matrices do not come from angle settings; use the adjoint gradient")
     slots = slot_table(circ, names)
     # the shifted expressions: (gate, expression, shift, coeff, slot -> factor)
     shifts = []

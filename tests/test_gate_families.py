@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import mindquantum_amd as mq
-from mindquantum_amd import ir
+from mindquantum_amd import _native, ir
 from mindquantum_amd.core import gates as G
 from oracle import cpu, dense
 
@@ -123,7 +123,10 @@ def test_family_gradients_match_dense(seed):
     assert np.allclose(g[0, 0].real, g_ref, atol=1e-9)
 
 
-def test_parameterized_rn_is_rejected():
-    circ = mq.Circuit([G.Rn("a", 0.1, 0.2).on(0)])
-    with pytest.raises(ValueError, match="Rn"):
-        ir.lower(circ, ["a"])
+def test_parameterized_rn_lowers_to_a_parameterised_matrix():
+    """Rn with a variable parameter runs as a parameterised matrix (per-sample values from its closed
+    form); a constant Rn stays a fixed matrix."""
+    circ = mq.Circuit([G.Rn("a", 0.1, 0.2).on(0), G.Rn(0.3, 0.1, 0.2).on(1)])
+    kinds = list(ir.lower(circ, ["a"])[0]["kind"])
+    assert kinds == [_native.GATE_KIND["PMATRIX"], _native.GATE_KIND["PMATRIX_TAP"], _native.GATE_KIND["PMATRIX_TAP"],
+                     _native.GATE_KIND["MATRIX"]]
