@@ -87,5 +87,51 @@ def build(verbose=False):
     return LIB
 
 
+SANITIZE_DIR = os.path.join(BUILD, "sanitize")
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer; device code is built as usual.
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(verbose=False):
+    """Build ``_build/sanitize/libmqamd.so`` (and its compile helper) with the host code - the C ABI,
+    the tile-pass scheduler and the code generator - under ASan/UBSan.  It reuses the device objects
+    of the regular build (run ``build()`` first).  Load it by setting ``MQAMD_LIBRARY`` to its path,
+    with the ASan runtime preloaded, e.g.::
+
+        LD_PRELOAD=$(hipcc -print-file-name=libclang_rt.asan-x86_64.so) ASAN_OPTIONS=detect_leaks=0 \\
+        MQAMD_LIBRARY=mindquantum_amd/_build/sanitize/libmqamd.so python -m pytest tests -m "not gpu"
+    """
+    for obj in ("kernels.o", "aux_kernels.o"):
+        if not os.path.exists(os.path.join(BUILD, obj)):
+            raise RuntimeError("run build() first: the sanitized library reuses " + obj)
+    os.makedirs(SANITIZE_DIR, exist_ok=True)
+    _write_jit_headers()
+    hipcc = _hipcc()
+    flags = [f for f in COMMON if f != "-O3"] + ["-O1"] + SANITIZE
+    out = lambda name: os.path.join(SANITIZE_DIR, name)  # noqa: E731
+    jobs = [
+        [hipcc, *flags, *DEVICE, "-c", os.path.join(CSRC, "capi.cpp"), "-o", out("capi.o")],
+        [hipcc, *flags, *DEVICE, "-c", os.path.join(CSRC, "schedule.cpp"), "-o", out("schedule.o")],
+        [hipcc, *flags, *DEVICE, "-I", BUILD, "-c", os.path.join(CSRC, "jit.cpp"), "-o", out("jit.o")],
+        [hipcc, *flags, "-c", os.path.join(CSRC, "jitc_main.cpp"), "-o", out("jitc_main.o")],
+    ]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        outs = list(ex.map(_run, jobs))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    lib = out("libmqamd.so")
+    _run([hipcc, *DEVICE, *SANITIZE, "-shared", "-fPIC", "-shared-libsan", "-o", lib,
+          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), out("capi.o"), out("schedule.o"),
+          out("jit.o"), "-lhiprtc"])
+    _run([hipcc, *SANITIZE, "-shared-libsan", "-o", out("mqamd_jitc"), out("jitc_main.o"), out("jit.o"), "-lhiprtc"])
+    return lib
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize" in sys.argv:
+        print(build_sanitized(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))

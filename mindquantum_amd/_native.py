@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmqamd.so")
+# MQAMD_LIBRARY names another build of the same library (the host-sanitizer build of _build.py).
+LIB_PATH = os.environ.get("MQAMD_LIBRARY") or os.path.join(_HERE, "libmqamd.so")
 
 MQA_OK = 0
 MQA_ERR_ARG = 1

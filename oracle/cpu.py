@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIBRARY names another build of the same source (`make sanitize`: ASan/UBSan).
+LIB = os.environ.get("ORACLE_LIBRARY") or os.path.join(HERE, "_build", "liboracle.so")
 _lib = None
 
 
