@@ -268,7 +268,66 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
                       "frac_of_peak": round(gbs2 / HBM_PEAK_GBS, 4), "launches": launches2, "tile_qubits": k_best,
                       "hbm_gbs_by_tile_qubits": {k: round(v[3] / (v[2] * 1e-3) / 1e9, 1) for k, v in per_k.items()}}
     del sim2
+    res["fusion"] = run_fusion_rows(dtype, n, circ, sim)
     return res, sim
+
+
+def run_fusion_rows(dtype, n, circ, ref_sim):
+    """Dense k-qubit fusion (k = 2..5) of the same circuit: blocks built on the host (untimed), then
+    (a) one streaming dense sweep per block and (b) for k <= 4 the blocks as dense gates in the tile
+    passes.  Each row: blocks, seconds, HBM GB/s of its kernels, and the fidelity defect against the
+    unfused product-path state of the same precision; complex64 k = 5 also on the VALU instead of the
+    matrix cores."""
+    import mindquantum_amd as mq
+    from mindquantum_amd import _native, fusion
+
+    rows = {}
+    state = (1 << n) * (8 if dtype == mq.complex64 else 16)
+    sim = mq.Simulator("mqvector_amd", n, dtype=dtype, seed=1)
+
+    def defect():
+        return abs(1.0 - abs(mq.inner_product(ref_sim, sim)))
+
+    def timed_apply(fn, kernels):
+        sim.reset()
+        _native.profile_reset()
+        _native.set_option("profile", 1)
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        _native.set_option("profile", 0)
+        launches = ms = byts = 0
+        for kname in kernels:
+            a, b, c = _native.profile(kname)
+            launches, ms, byts = launches + a, ms + b, byts + c
+        return {"seconds": round(dt, 4), "launches": launches, "kernel_ms": round(ms, 3),
+                "hbm_gbs": round(byts / (ms * 1e-3) / 1e9, 1) if ms else None,
+                "frac_of_peak": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms else None,
+                "gate_equivalent_gbs": round(len(circ) * 2 * state / dt / 1e9, 1)}
+
+    for k in (2, 3, 4, 5):
+        t0 = time.perf_counter()
+        fc = fusion.fuse(circ, k)
+        row = {"blocks": len(fc), "block_widths": fc.histogram(), "fuse_seconds_untimed": round(time.perf_counter() - t0, 2)}
+        row["sweep"] = timed_apply(lambda: fc.apply(sim), ("dense_sweep", "dense_sweep_mfma"))
+        row["sweep"]["fidelity_defect_vs_unfused"] = defect()
+        if k == 5 and dtype == mq.complex64:
+            _native.set_option("dense_mfma", 0)
+            try:
+                row["sweep_valu_only"] = timed_apply(lambda: fc.apply(sim), ("dense_sweep",))
+                row["sweep_valu_only"]["fidelity_defect_vs_unfused"] = defect()
+            finally:
+                _native.set_option("dense_mfma", -1)
+            _native.profile_reset()
+        if k <= 4:
+            tc = fc.as_circuit()
+            sim.apply_circuit(tc)  # plan and compiled passes, untimed
+            row["tile_passes"] = timed_apply(lambda: sim.apply_circuit(tc), ("tile_pass_fwd",))
+            row["tile_passes"]["fidelity_defect_vs_unfused"] = defect()
+        rows[f"k{k}"] = row
+        log(f"fusion {n}q {np.dtype(dtype).name} k={k}: {row}")
+    del sim
+    return rows
 
 
 def gate_apply_cpu(dtype, n, layers, depth, seed):

@@ -117,7 +117,8 @@ int mqa_device_count(void);
  * runs plans in which uncontrolled RX / RY / RZ / GP drop their global phase, see there; 0: exact
  * plans), "jit_single_buffer" (1, the default: compiled back-sweep passes keep psi and lambda in
  * registers between register groups and share one LDS tile buffer, so twice as many blocks fit a CU;
- * 0: two tile buffers).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
+ * 0: two tile buffers), "dense_mfma" (1, the default: complex64 5-qubit blocks of
+ * mqa_sim_apply_dense_blocks run on the FP32 matrix cores; 0: on the vector ALUs).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
  * "grad_recomputes", "checkpoints_used", "phase_free_used" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
@@ -204,6 +205,14 @@ int mqa_sim_get_qs(const mqa_sim* sim, void* host_out, uint64_t dim);
 int mqa_sim_set_qs(mqa_sim* sim, const void* host_in, uint64_t dim);
 /* apply_circuit(circ, pr) (vector_state.tpp:948-960); params holds the circuit's parameters. */
 int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ, const double* params, int32_t n_params);
+/* A sequence of dense k-qubit matrices (k = 1..5), each applied in one streaming sweep over the
+ * state: the apply of ApplyNQubitsMatrix (cpu_vector_core_matrix_gate.cpp:28-83, no controls) for the
+ * dense blocks a gate-fusion pass builds from runs of gates (mindquantum_amd.fusion).  Block b acts
+ * on block_qubits[b] qubits, listed in turn in `qubits` (matrix index bit i <-> the block's i-th
+ * qubit), with a row-major 2^k x 2^k complex matrix of interleaved doubles, listed in turn in
+ * `matrices`.  Every block is checked before the first sweep runs. */
+int mqa_sim_apply_dense_blocks(mqa_sim* sim, int32_t n_blocks, const int32_t* block_qubits, const int32_t* qubits,
+                               const double* matrices);
 /* apply_hamiltonian(ham) (vector_state.tpp:962-974): state <- H state. */
 int mqa_sim_apply_hamiltonian(mqa_sim* sim, const mqa_ham* ham);
 /* get_expectation(ham, circ, pr) (vector_state.tpp:711-726): <0|U^dag H U|0> on a copy of the

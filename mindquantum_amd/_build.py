@@ -1,7 +1,7 @@
 """Build the HIP library ``libmqamd.so`` in-tree for gfx950 (hipcc, no cmake).
 
 ``python -m mindquantum_amd._build`` or ``__graft_entry__.build()`` runs it; every call
-recompiles.  Sources: ``csrc/kernels.hip`` and ``csrc/aux_kernels.hip`` (device code), ``csrc/capi.cpp``,
+recompiles.  Sources: ``csrc/kernels.hip``, ``csrc/aux_kernels.hip`` and ``csrc/dense_sweep.hip`` (device code), ``csrc/capi.cpp``,
 ``csrc/schedule.cpp`` and ``csrc/jit.cpp`` (host code).  The text of ``csrc/program.h`` and
 ``csrc/device_ops.h`` is embedded into the library (``_build/jit_headers.inc``) for the
 per-circuit kernels that ``jit.cpp`` compiles with hipRTC at run time.
@@ -63,12 +63,13 @@ def build(verbose=False):
     jobs = [
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "kernels.hip"), "-o", os.path.join(BUILD, "kernels.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "aux_kernels.hip"), "-o", os.path.join(BUILD, "aux_kernels.o")],
+        [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "dense_sweep.hip"), "-o", os.path.join(BUILD, "dense_sweep.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "capi.cpp"), "-o", os.path.join(BUILD, "capi.o")],
         [hipcc, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "schedule.cpp"), "-o", os.path.join(BUILD, "schedule.o")],
         [hipcc, *COMMON, *DEVICE, "-I", BUILD, "-c", os.path.join(CSRC, "jit.cpp"), "-o", os.path.join(BUILD, "jit.o")],
         [hipcc, *COMMON, "-c", os.path.join(CSRC, "jitc_main.cpp"), "-o", os.path.join(BUILD, "jitc_main.o")],
     ]
-    with ThreadPoolExecutor(max_workers=6) as ex:
+    with ThreadPoolExecutor(max_workers=7) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:
         for o in outs:
@@ -76,8 +77,8 @@ def build(verbose=False):
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, *DEVICE, "-shared", "-fPIC", "-o", tmp,
-          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), os.path.join(BUILD, "capi.o"),
-          os.path.join(BUILD, "schedule.o"),
+          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), os.path.join(BUILD, "dense_sweep.o"),
+          os.path.join(BUILD, "capi.o"), os.path.join(BUILD, "schedule.o"),
           os.path.join(BUILD, "jit.o"), "-lhiprtc"])
     os.replace(tmp, LIB)
     # compile helper of the code generator: hipRTC runs one compile at a time per process
@@ -102,7 +103,7 @@ def build_sanitized(verbose=False):
         LD_PRELOAD=$(hipcc -print-file-name=libclang_rt.asan-x86_64.so) ASAN_OPTIONS=detect_leaks=0 \\
         MQAMD_LIBRARY=mindquantum_amd/_build/sanitize/libmqamd.so python -m pytest tests -m "not gpu"
     """
-    for obj in ("kernels.o", "aux_kernels.o"):
+    for obj in ("kernels.o", "aux_kernels.o", "dense_sweep.o"):
         if not os.path.exists(os.path.join(BUILD, obj)):
             raise RuntimeError("run build() first: the sanitized library reuses " + obj)
     os.makedirs(SANITIZE_DIR, exist_ok=True)
@@ -124,7 +125,8 @@ def build_sanitized(verbose=False):
                 print(o)
     lib = out("libmqamd.so")
     _run([hipcc, *DEVICE, *SANITIZE, "-shared", "-fPIC", "-shared-libsan", "-o", lib,
-          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), out("capi.o"), out("schedule.o"),
+          os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "aux_kernels.o"), os.path.join(BUILD, "dense_sweep.o"),
+          out("capi.o"), out("schedule.o"),
           out("jit.o"), "-lhiprtc"])
     _run([hipcc, *SANITIZE, "-shared-libsan", "-o", out("mqamd_jitc"), out("jitc_main.o"), out("jit.o"), "-lhiprtc"])
     return lib

@@ -84,6 +84,7 @@ _SIGS = {
     "mqa_sim_get_qs": ([_P, _P, _U64], ctypes.c_int),
     "mqa_sim_set_qs": ([_P, _P, _U64], ctypes.c_int),
     "mqa_sim_apply_circuit": ([_P, _P, _P, _I32], ctypes.c_int),
+    "mqa_sim_apply_dense_blocks": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "mqa_sim_apply_hamiltonian": ([_P, _P], ctypes.c_int),
     "mqa_sim_expectation": ([_P, _P, _P, _P, _I32, _P], ctypes.c_int),
     "mqa_sim_expectation_lr": ([_P, _P, _P, _P, _P, _P, _I32, _P], ctypes.c_int),

@@ -19,6 +19,7 @@
 
 #include "../../include/mqamd.h"
 #include "aux_kernels.h"
+#include "dense_sweep.h"
 #include "jit.h"
 #include "launch.h"
 #include "program.h"
@@ -81,6 +82,8 @@ struct Options {
     // 1: compiled back-sweep passes keep psi and lambda in registers and share one LDS tile
     // buffer (twice the resident blocks per CU)
     std::atomic<int64_t> jit_single_buffer{1};
+    // 1: complex64 5-qubit dense sweeps run on the FP32 matrix cores
+    std::atomic<int64_t> dense_mfma{1};
 };
 Options g_opt;
 
@@ -98,7 +101,8 @@ std::vector<OptEntry> opt_table() {
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
-            {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1}};
+            {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1},
+            {"dense_mfma", &g_opt.dense_mfma, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -470,6 +474,7 @@ struct mqa_sim {
     mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
     mqa::DevBuf coefs;  // coefficient table of the compiled passes (rebuilt per run_plan call)
     mqa::DevBuf custom[2];  // rows of the custom-gate matrices of the running circuits (two: bra and ket sides)
+    mqa::DevBuf dense;      // matrices of the dense blocks of mqa_sim_apply_dense_blocks
     mqa::Timer timer;
     size_t amp_bytes() const { return dtype == MQA_C64 ? 8 : 16; }
     size_t state_bytes() const { return dim_eff * amp_bytes(); }
@@ -1448,6 +1453,59 @@ int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double*
                                    sim->stream),
                       "global phase");
         sync(sim, "apply_circuit");
+    });
+}
+
+int mqa_sim_apply_dense_blocks(mqa_sim* sim, int32_t n_blocks, const int32_t* block_qubits, const int32_t* qubits,
+                               const double* matrices) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        require(n_blocks >= 0, "n_blocks must be >= 0");
+        if (n_blocks == 0) return;
+        require(block_qubits != nullptr && qubits != nullptr && matrices != nullptr, "NULL block arrays");
+        // validate everything before anything runs; offsets of each block's qubits and matrix
+        std::vector<size_t> q_off(n_blocks), m_off(n_blocks);
+        size_t nq = 0, nm = 0;
+        for (int b = 0; b < n_blocks; b++) {
+            const int k = block_qubits[b];
+            require(k >= 1 && k <= kDenseMaxQubits, "a dense block acts on 1 to 5 qubits");
+            require(k <= sim->n, "a dense block acts on more qubits than the simulator has");
+            uint64_t seen = 0;
+            for (int i = 0; i < k; i++) {
+                const int q = qubits[nq + i];
+                require(q >= 0 && q < sim->n, "dense block qubit out of range");
+                require(!(seen >> q & 1), "dense block qubits must be distinct");
+                seen |= uint64_t(1) << q;
+            }
+            q_off[b] = nq, m_off[b] = nm;
+            nq += size_t(k);
+            nm += size_t(2) << (2 * k);
+        }
+        DeviceGuard dg(sim->device);
+        // matrices in the state's precision, uploaded once for the whole sequence
+        const size_t real = sim->dtype == MQA_C64 ? 4 : 8;
+        std::vector<unsigned char> host(nm * real);
+        if (sim->dtype == MQA_C64) {
+            float* f = reinterpret_cast<float*>(host.data());
+            for (size_t i = 0; i < nm; i++) f[i] = float(matrices[i]);
+        } else {
+            std::memcpy(host.data(), matrices, nm * real);
+        }
+        sim->dense.ensure(host.size());
+        hip_check(hipMemcpyAsync(sim->dense.p, host.data(), host.size(), hipMemcpyHostToDevice, sim->stream),
+                  "dense matrices upload");
+        const bool mfma = g_opt.dense_mfma.load() != 0;
+        for (int b = 0; b < n_blocks; b++) {
+            const int k = block_qubits[b];
+            const void* mat = static_cast<const unsigned char*>(sim->dense.p) + m_off[b] * real;
+            const char* name = dense_sweep_uses_mfma(sim->dtype, sim->n_eff, k, mfma) ? "dense_sweep_mfma" : "dense_sweep";
+            timed(sim, name, 2.0 * double(sim->state_bytes()), [&] {
+                hip_check(launch_dense_sweep(sim->dtype, sim->state.p, sim->n_eff, qubits + q_off[b], k, 0, mat, mfma,
+                                             sim->stream),
+                          "dense sweep");
+            });
+        }
+        sync(sim, "apply_dense_blocks");
     });
 }
 
