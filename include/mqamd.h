@@ -113,7 +113,9 @@ int mqa_device_count(void);
  * pass cost model; 1 = one gate per pass), "hbm_budget_bytes" (also the environment variable
  * MQAMD_HBM_BUDGET_BYTES; 0 = 85% of free memory), "profile", "ham_tile_qubits_c64",
  * "ham_tile_qubits_c128", "ham_tile_engine", "jit" (0 off, 1 background compile, 2 compile before
- * the first launch), "jit_min_amps", "checkpoints", "phase_free" (1, the default: mqa_sim_grad_batch
+ * the first launch), "jit_min_amps" (launches of at least this many amplitudes use compiled passes),
+ * "jit_min_runs" (8: so do smaller ones once their plan has run that many times; 0: never),
+ * "checkpoints", "phase_free" (1, the default: mqa_sim_grad_batch
  * runs plans in which uncontrolled RX / RY / RZ / GP drop their global phase, see there; 0: exact
  * plans), "jit_single_buffer" (1, the default: compiled back-sweep passes keep psi and lambda in
  * registers between register groups and share one LDS tile buffer, so twice as many blocks fit a CU;

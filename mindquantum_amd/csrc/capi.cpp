@@ -75,6 +75,8 @@ struct Options {
     // per-circuit compiled passes: 0 off, 1 compile in the background and switch over when
     // ready, 2 compile before the first launch; only for launches of >= jit_min_amps amplitudes
     std::atomic<int64_t> jit{1}, jit_min_amps{int64_t(1) << 24};
+    // ... and for smaller launches once a plan has run this many times (0: never)
+    std::atomic<int64_t> jit_min_runs{8};
     // forward-state checkpoints of the batched gradient (0: none; the back-sweep uncomputes psi)
     std::atomic<int64_t> checkpoints{0};
     // 1: hermitian gradients run phase-free plans (uncontrolled rotations drop global phases)
@@ -100,7 +102,8 @@ std::vector<OptEntry> opt_table() {
             {"hbm_budget_bytes", &g_opt.budget, 0},      {"profile", &g_opt.profile, 0},
             {"ham_tile_qubits_c64", &g_opt.ham_tile_c64, 14}, {"ham_tile_qubits_c128", &g_opt.ham_tile_c128, 13},
             {"ham_tile_engine", &g_opt.ham_engine, 1}, {"jit", &g_opt.jit, 1},
-            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"checkpoints", &g_opt.checkpoints, 0},
+            {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"jit_min_runs", &g_opt.jit_min_runs, 8},
+            {"checkpoints", &g_opt.checkpoints, 0},
             {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1},
             {"dense_mfma", &g_opt.dense_mfma, 1}};
 }
@@ -166,6 +169,7 @@ struct DevPlan {
     std::mutex jit_mu;
     std::unique_ptr<JitPlan> jit;     // compiled passes (created on first large launch)
     std::unique_ptr<JitPlan> jit_sb;  // the same with one LDS tile buffer (option "jit_single_buffer")
+    std::atomic<int64_t> runs{0};     // run_plan calls so far (small plans compile once they recur)
 };
 
 // ---------------------------------------------------------------- kernel timing
@@ -533,8 +537,12 @@ struct CustomRows {
     uint64_t stride = 0;
 };
 
+// src0: the first pass (p0, forward plans) reads its input tiles from src0 (one state broadcast to
+// every sample) instead of psi.  skip_last: the outputs of the last pass (p1 - 1) are dead (the last
+// back-sweep pass of a gradient): it writes its taps only.
 void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, const double* angles, int n_slots,
-              double* partials, int n_tap_ids, int p0 = 0, int p1 = -1, CustomRows custom = {}) {
+              double* partials, int n_tap_ids, int p0 = 0, int p1 = -1, CustomRows custom = {},
+              const void* src0 = nullptr, bool skip_last = false) {
     const Plan& pl = dp->plan;
     if (p1 < 0 || p1 > int(pl.passes.size())) p1 = int(pl.passes.size());
     if (p0 >= p1) return;
@@ -558,7 +566,10 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     a.custom_stride = custom.stride;
     const double bytes = double(batch) * double(sim->state_bytes()) * (pl.adjoint ? 4.0 : 2.0);
     JitPlan* jp = nullptr;
-    if (g_opt.jit != 0 && double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps)) {
+    const int64_t runs = dp->runs.fetch_add(1) + 1;
+    const bool big = double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps);
+    const bool recurring = g_opt.jit_min_runs > 0 && runs >= g_opt.jit_min_runs;
+    if (g_opt.jit != 0 && (big || recurring)) {
         std::lock_guard<std::mutex> lk(dp->jit_mu);
         JitForm form = JitForm::kTile;
         if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, sim->dtype)) form = JitForm::kSingleBuffer;
@@ -582,7 +593,14 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         tile_pass_lds_bytes(sim->dtype, pl.adjoint && !one_tile, pl.K, pl.C, std::max(pl.max_taps, 1));
     for (int i = p0; i < p1; i++) {
         a.pass_index = i;
-        timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", bytes, [&] {
+        a.src = (i == p0 && !pl.adjoint) ? src0 : nullptr;
+        a.src_stride = 0;
+        a.skip_store = (skip_last && i == p1 - 1) ? 1 : 0;
+        // algorithmic bytes: a broadcast input is read once, dead outputs are not written
+        double pass_bytes = bytes;
+        if (a.src) pass_bytes = double(batch) * double(sim->state_bytes()) + double(sim->state_bytes());
+        if (a.skip_store) pass_bytes = bytes / 2;
+        timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", pass_bytes, [&] {
             if (jp) {
                 hip_check(jp->launch(i, a, batch, lds, sim->stream), "compiled tile pass");
             } else {
@@ -1736,17 +1754,23 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
             const double* ang = build_angles(sim, circ, n_enc ? enc + b0 * n_enc : nullptr, nb, n_enc, ans, n_ans);
             const CustomRows cust = custom_rows(sim, circ, b0, nb);
             // forward sweep into psi (and the checkpoints)
+            // the first forward pass reads the initial state itself (every sample starts from it), so
+            // psi is not filled first; a plan without passes still needs the copies
             auto forward = [&] {
-                hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff, sim->dim_eff,
-                                      nb, s),
-                          "fill");
+                DevPlan* first = ck ? ck->segs[0].get() : fwd;
+                const bool direct = !first->plan.passes.empty();
+                if (!direct)
+                    hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff,
+                                          sim->dim_eff, nb, s),
+                              "fill");
+                const void* src0 = direct ? sim->state.p : nullptr;
                 if (!ck) {
-                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust);
+                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust, src0);
                     return;
                 }
                 for (int k = 0; k <= n_ck; k++) {
                     run_plan(sim, ck->segs[size_t(k)].get(), sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0,
-                             -1, cust);
+                             -1, cust, k == 0 ? src0 : nullptr);
                     if (k < n_ck)
                         hip_check(hipMemcpyAsync(ckpt_at(n_ck - 1 - k), sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s),
                                   "checkpoint");
@@ -1778,7 +1802,7 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
                         cur = ckpt_at(i);
                     }
                     run_plan(sim, adj, cur, sim->lam.p, nb, ang, circ->n_slots, sim->partials.as<double>(), n_tap_ids,
-                             p0, -1, cust);
+                             p0, -1, cust, nullptr, true);
                     hip_check(launch_reduce_rows1(sim->partials.as<double>(), n_tiles, sim->taps.as<double>(),
                                                   uint64_t(nb) * n_tap_ids, s),
                               "reduce taps");
@@ -2015,21 +2039,45 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
     const auto* hargs_d = reinterpret_cast<const HamTileArgs*>(tab + h_at);
     const auto* prefix_d = reinterpret_cast<const uint64_t*>(tab + p_at);
 
+    // kernel timing (option "profile"): HIP events on this call's stream, with the algorithmic bytes
+    // of each shared launch (every item's state read and written once per pass; psi and lambda in
+    // back-sweep passes; the Hamiltonian passes read the ket and write H ket)
+    Timer timer;
+    auto prof = [&](const char* name, double bytes, auto&& fn) {
+        if (!g_opt.profile) {
+            fn();
+            return;
+        }
+        hipEvent_t a = timer.get(), b = timer.get();
+        hip_check(hipEventRecord(a, s), "hipEventRecord");
+        fn();
+        hip_check(hipEventRecord(b, s), "hipEventRecord");
+        timer.pending.push_back({a, b, name, bytes});
+    };
     auto run_tile_steps = [&](const std::vector<std::vector<StepGroup<TilePassArgs>>>& steps, bool adjoint) {
         for (auto& v : steps)
             for (auto& g : v) {
                 const size_t lds = interpreted_pass_lds_bytes(dtype, adjoint, g.full, g.K, g.C, g.max_taps);
-                hip_check(launch_tile_pass_items(dtype, adjoint, g.full, targs_d + g.args_at, prefix_d + g.prefix_at,
-                                                 int(g.args.size()), g.blocks, g.K, lds, s),
-                          "tile pass (items)");
+                double bytes = 0;
+                for (const auto& a : g.args) bytes += double(a.stride) * double(amp) * (adjoint ? 4.0 : 2.0);
+                prof(adjoint ? "items_tile_pass_adj" : "items_tile_pass_fwd", bytes, [&] {
+                    hip_check(launch_tile_pass_items(dtype, adjoint, g.full, targs_d + g.args_at,
+                                                     prefix_d + g.prefix_at, int(g.args.size()), g.blocks, g.K, lds, s),
+                              "tile pass (items)");
+                });
             }
     };
     run_tile_steps(fwd_steps, false);
     for (auto& v : ham_steps)
-        for (auto& g : v)
-            hip_check(launch_ham_tile_items(dtype, g.first, g.last, hargs_d + g.args_at, prefix_d + g.prefix_at,
-                                            int(g.args.size()), g.blocks, g.K, g.C, s),
-                      "hamiltonian (items)");
+        for (auto& g : v) {
+            double bytes = 0;
+            for (const auto& h : g.args) bytes += double(h.stride) * double(amp) * 2.0;
+            prof("items_hamiltonian", bytes, [&] {
+                hip_check(launch_ham_tile_items(dtype, g.first, g.last, hargs_d + g.args_at, prefix_d + g.prefix_at,
+                                                int(g.args.size()), g.blocks, g.K, g.C, s),
+                          "hamiltonian (items)");
+            });
+        }
     for (auto& it : items) {  // flip masks wider than a tile: gather kernel, one launch per item
         if (it.hp) continue;
         auto* ham = const_cast<mqa_ham*>(hams[it.index]);
@@ -2072,6 +2120,7 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
     hip_check(hipMemcpyAsync(taps_h.data(), taps_d.p, n_taps * 8, hipMemcpyDeviceToHost, s), "d2h taps");
     hip_check(hipMemcpyAsync(fpart_h.data(), fpart_d.p, n_fpart * 8, hipMemcpyDeviceToHost, s), "d2h f");
     hip_check(hipStreamSynchronize(s), "grad items");
+    timer.resolve();
 
     // f = sum of the partials; g = jacobian^T * 2 Re(taps) (gradient.h:124-133, 183-188)
     for (auto& it : items) {

@@ -763,20 +763,24 @@ __device__ __forceinline__ void st_stream(double2 v, double2* p) {
 // pair inside its 16-byte slot, and a per-lane swap in registers costs more than the second
 // LDS access).
 
-template <typename T>
+// NT = false: cached loads (a state that every sample of a launch reads).
+template <typename T, bool NT = true>
 __device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* __restrict__ g,
-                                          const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT) {
+                                          const uint64_t* __restrict__ offtab, int C, uint32_t tid, uint32_t NT_) {
     const uint32_t cmask = (1u << C) - 1u;
+    const uint32_t NT_threads = NT_;
     if constexpr (sizeof(T) == 4) {
         float4 buf[kAmpsPerThread / 2];
 #pragma unroll
         for (int j = 0; j < kAmpsPerThread / 2; j++) {
-            const uint32_t u = 2u * (tid + NT * j);
-            buf[j] = ld_stream(reinterpret_cast<const float4*>(g + ((u & cmask) | offtab[u >> C])));
+            const uint32_t u = 2u * (tid + NT_threads * j);
+            const float4* src = reinterpret_cast<const float4*>(g + ((u & cmask) | offtab[u >> C]));
+            if constexpr (NT) buf[j] = ld_stream(src);
+            else buf[j] = *src;
         }
 #pragma unroll
         for (int j = 0; j < kAmpsPerThread / 2; j++) {
-            const uint32_t a = swz<T>(2u * (tid + NT * j));
+            const uint32_t a = swz<T>(2u * (tid + NT_threads * j));
             lds[a] = make_float2(buf[j].x, buf[j].y);
             lds[a ^ 1u] = make_float2(buf[j].z, buf[j].w);
         }
@@ -787,11 +791,13 @@ __device__ __forceinline__ void tile_load(V2<T>* __restrict__ lds, const V2<T>* 
             V2<T> buf[kAmpsPerThread / 2];
 #pragma unroll
             for (int j = 0; j < kAmpsPerThread / 2; j++) {
-                const uint32_t u = tid + NT * (h + j);
-                buf[j] = ld_stream(g + ((u & cmask) | offtab[u >> C]));
+                const uint32_t u = tid + NT_threads * (h + j);
+                const V2<T>* src = g + ((u & cmask) | offtab[u >> C]);
+                if constexpr (NT) buf[j] = ld_stream(src);
+                else buf[j] = *src;
             }
 #pragma unroll
-            for (int j = 0; j < kAmpsPerThread / 2; j++) lds[swz<T>(tid + NT * (h + j))] = buf[j];
+            for (int j = 0; j < kAmpsPerThread / 2; j++) lds[swz<T>(tid + NT_threads * (h + j))] = buf[j];
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -861,7 +867,13 @@ __device__ __forceinline__ TileFrame<T> tile_begin(const TilePassArgs& args, con
     __syncthreads();
     f.g_psi = static_cast<V*>(args.psi) + f.sample * args.stride + f.base;
     f.g_lam = ADJ ? (static_cast<V*>(args.lam) + f.sample * args.stride + f.base) : nullptr;
-    tile_load<T>(f.t_psi, f.g_psi, f.offtab, f.C, f.tid, f.NT);
+    if (!ADJ && args.src != nullptr) {  // the input state comes from src (broadcast: cached loads)
+        const V* g_src = static_cast<const V*>(args.src) + f.sample * args.src_stride + f.base;
+        if (args.src_stride == 0) tile_load<T, false>(f.t_psi, g_src, f.offtab, f.C, f.tid, f.NT);
+        else tile_load<T>(f.t_psi, g_src, f.offtab, f.C, f.tid, f.NT);
+    } else {
+        tile_load<T>(f.t_psi, f.g_psi, f.offtab, f.C, f.tid, f.NT);
+    }
     if constexpr (ADJ) tile_load<T>(f.t_lam, f.g_lam, f.offtab, f.C, f.tid, f.NT);
     __syncthreads();
     f.wave = int(f.tid >> 6);
@@ -896,8 +908,10 @@ __device__ __forceinline__ void tile_end(const TilePassArgs& args, const DevPass
     // than kept live in registers through the whole group loop
     uint32_t tid_st = f.tid;
     asm volatile("" : "+v"(tid_st));
-    tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
-    if constexpr (ADJ) tile_store<T>(f.t_lam, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    if (!args.skip_store) {
+        tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
+        if constexpr (ADJ) tile_store<T>(f.t_lam, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    }
     if constexpr (ADJ) {
         // one partial per (sample, tap id, tile); summed over tiles by k_reduce_rows1
         for (int t = int(f.tid); t < pass.n_taps; t += int(f.NT)) {

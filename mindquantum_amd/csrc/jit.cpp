@@ -353,12 +353,14 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
     }
     if (sb) {  // psi, then lambda, back through the buffer to HBM; the tap partials
         const int last = ps.n_groups - 1;
-        o << "  uint32_t tid_st = f.tid;\n  asm volatile(\"\" : \"+v\"(tid_st));\n  __syncthreads();\n";
+        o << "  uint32_t tid_st = f.tid;\n  asm volatile(\"\" : \"+v\"(tid_st));\n  __syncthreads();\n"
+          << "  if (!args.skip_store) {\n";
         sb_move("store", last, "p");
         o << "  __syncthreads();\n  tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);\n"
           << "  __syncthreads();\n";
         sb_move("store", last, "l");
         o << "  __syncthreads();\n  tile_store<T>(f.t_psi, f.g_lam, f.offtab, f.C, tid_st, f.NT);\n"
+          << "  }\n"
           << "  taps_out<T>(args, pass, f);\n";
     } else {
         o << "  tile_end<T, " << A << ">(args, pass, f);\n";

@@ -95,13 +95,15 @@ __device__ __forceinline__ void tile_pass_adj_one_buffer(const TilePassArgs& arg
     asm volatile("" : "+v"(tid_st));
     const uint32_t ap = swz<T>(group_ctx(tid_st, prev.pos, prev.spos).ub);
     __syncthreads();
-    group_store1<T>(f.t_psi, ap, prev.off, p);
-    __syncthreads();
-    tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
-    __syncthreads();
-    group_store1<T>(f.t_psi, ap, prev.off, l);
-    __syncthreads();
-    tile_store<T>(f.t_psi, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    if (!args.skip_store) {  // uniform: dead outputs of the last back-sweep pass are not written
+        group_store1<T>(f.t_psi, ap, prev.off, p);
+        __syncthreads();
+        tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);
+        __syncthreads();
+        group_store1<T>(f.t_psi, ap, prev.off, l);
+        __syncthreads();
+        tile_store<T>(f.t_psi, f.g_lam, f.offtab, f.C, tid_st, f.NT);
+    }
     taps_out<T>(args, pass, f);
 }
 

@@ -188,6 +188,12 @@ struct TilePassArgs {
     const void* coefs;     // [batch][n_slots][2] in the real type, per CoefForm (compiled passes)
     const double* custom;  // per-sample matrices of parameterised dense ops (OPK_PM / OPK_PMTAP)
     uint64_t custom_stride;  // doubles between samples' rows (0: one row for every sample)
+    // Forward passes: read the input tiles from src (sample b at src + b * src_stride) instead of
+    // psi; src_stride 0 broadcasts one state to every sample (cached loads).  NULL: read psi.
+    const void* src;
+    uint64_t src_stride;
+    // 1: the pass's output states are dead (the last back-sweep pass): only its taps are written.
+    int32_t skip_store;
 };
 
 // Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are
