@@ -221,7 +221,7 @@ def run_hea16(reps, cpu):
 
 
 # ---------------------------------------------------------------- 28q gate application
-def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
+def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False, variants=True):
     import mindquantum_amd as mq
     from mindquantum_amd import _native, autotune, ir
     from mindquantum_amd.algorithm import layered_random_circuit
@@ -261,6 +261,8 @@ def run_gate_apply(dtype, n, depth, seed, unfused_depth, tune=False):
         "autotune": tuned, "norm_drift": abs(norm - 1.0),
     }
     log(f"gate apply {n}q {np.dtype(dtype).name}: {res}")
+    if not variants:
+        return res, sim
     # unfused: one gate per pass, on a shorter prefix of the same circuit.  Single-gate passes
     # stream memory; they run on the interpreted kernel, with the tile width that streams fastest
     # (timed here over the widths 10..12; the reference's schedule of one sweep per gate)
@@ -628,6 +630,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile widths (no autotuning)")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the device copy probe (counter calibration for profiling runs)")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="run only the complex64 gate application of the random circuit, fused passes (profiling)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="set a library option (mqa_set_option) before the workloads; repeatable")
     args = ap.parse_args()
@@ -645,6 +649,14 @@ def main():
         sys.exit(1)
     if args.copy_probe_only:
         print(json.dumps({"copy_probe": copy_probe()}), flush=True)
+        return
+    if args.apply_only:
+        _native.set_option("jit", 2)
+        for kv in args.option:
+            key, _, val = kv.partition("=")
+            _native.set_option(key.strip(), int(val))
+        res, _ = run_gate_apply(mq.complex64, args.apply_qubits, args.apply_depth, args.seed, 0, variants=False)
+        print(json.dumps({"gate_apply_c64": res}), flush=True)
         return
     # per-circuit compiled passes, compiled before the first launch (inside the warmup steps)
     _native.set_option("jit", 0 if args.interpreted else 2)
