@@ -185,8 +185,13 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
 
 // One pass step over several independent single-sample states (uneven circuits): block b runs
 // tile b - tile_prefix[i] of item i.  Each item carries its own schedule, buffers and pass index.
+// complex64: at least three waves per SIMD (at most 168 VGPRs); left alone, the interpreter's
+// back-sweep takes 187 and two waves per SIMD, a third of what the LDS of its tiles allows.
+template <typename T>
+constexpr int kItemsMinWaves = sizeof(T) == 4 ? 3 : 1;
+
 template <typename T, bool ADJ, bool FULL>
-__global__ void __launch_bounds__(512) k_tile_pass_items(const TilePassArgs* __restrict__ items,
+__global__ void __launch_bounds__(512, kItemsMinWaves<T>) k_tile_pass_items(const TilePassArgs* __restrict__ items,
                                                          const uint64_t* __restrict__ tile_prefix, int n_items) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int i = item_of_block(tile_prefix, n_items, blockIdx.x);
