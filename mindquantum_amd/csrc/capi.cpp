@@ -86,6 +86,9 @@ struct Options {
     std::atomic<int64_t> jit_single_buffer{1};
     // 1: complex64 5-qubit dense sweeps run on the FP32 matrix cores
     std::atomic<int64_t> dense_mfma{1};
+    // 1: batched gradients and gate application whose launches run compiled passes merge RZ, PS and
+    // RZZ gates into phase blocks (OPK_PHASE); 2: every launch of those (interpreted ones too)
+    std::atomic<int64_t> phase_ops{1};
 };
 Options g_opt;
 
@@ -105,7 +108,7 @@ std::vector<OptEntry> opt_table() {
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"jit_min_runs", &g_opt.jit_min_runs, 8},
             {"checkpoints", &g_opt.checkpoints, 0},
             {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1},
-            {"dense_mfma", &g_opt.dense_mfma, 1}};
+            {"dense_mfma", &g_opt.dense_mfma, 1}, {"phase_ops", &g_opt.phase_ops, 1}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -164,6 +167,7 @@ struct DevPlan {
     Plan plan;
     DevBuf passes, groups, ops, pool, tap_ids;
     DevBuf coef_forms;  // CoefForm per angle slot (the compiled passes' coefficient table)
+    DevBuf terms, phase_coefs;  // phase blocks: DevPhaseTerm and PhaseCoef arrays
     int n_coef_forms = 0;
     bool full = false;
     std::mutex jit_mu;
@@ -260,19 +264,21 @@ struct mqa_circuit {
 
     // phase_free: the plan may drop global phases of uncontrolled gates (OPK_PF_* kinds); only for
     // states whose results do not see them (the hermitian gradient's forward and back-sweep)
-    mqa::SchedOptions options(int dtype, bool adjoint, bool phase_free = false) const {
+    mqa::SchedOptions options(int dtype, bool adjoint, bool phase_free = false, bool phase_ops = false) const {
         auto o = mqa::sched_options(dtype, adjoint);
         const int k = tile_override[dtype == MQA_C64 ? 0 : 1][adjoint ? 1 : 0].load();
         if (k > 0) o.tile_qubits = k;
         o.phase_free = phase_free;
+        o.phase_ops = phase_ops;
         return o;
     }
 
-    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev, bool phase_free = false) {
-        auto o = options(dtype, adjoint, phase_free);
+    mqa::DevPlan* get_plan(int n_eff, int dtype, bool adjoint, hipStream_t s, bool upload_dev, bool phase_free = false,
+                           bool phase_ops = false) {
+        auto o = options(dtype, adjoint, phase_free, phase_ops);
         int dev = upload_dev ? mqa::current_device() : -1;
         auto key = std::make_tuple(n_eff, dtype, int(adjoint), o.tile_qubits, o.low_run, o.group_width,
-                                   o.max_ops_per_pass, o.taps_per_group, int(o.phase_free), dev);
+                                   o.max_ops_per_pass, o.taps_per_group, int(o.phase_free) | (int(o.phase_ops) << 1), dev);
         std::lock_guard<std::mutex> lk(mu);
         auto it = plans.find(key);
         if (it != plans.end()) return it->second.get();
@@ -285,16 +291,21 @@ struct mqa_circuit {
     static std::unique_ptr<mqa::DevPlan> make_devplan(mqa::Plan&& plan, int dtype, hipStream_t s, bool upload_dev) {
         auto dp = std::make_unique<mqa::DevPlan>();
         dp->plan = std::move(plan);
-        // dense blocks or controls that stay per-amplitude predicates need the FULL instantiation
+        // dense blocks, controls that stay per-amplitude predicates and phase blocks need the FULL
+        // instantiation
         for (auto& op : dp->plan.dev_ops)
             if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0 ||
-                mqa::opk_is_custom(op.kind))
+                mqa::opk_is_custom(op.kind) || op.kind == mqa::OPK_PHASE)
                 dp->full = true;
         if (upload_dev) {
             mqa::upload(dp->passes, dp->plan.dev_passes, s);
             mqa::upload(dp->groups, dp->plan.dev_groups, s);
             mqa::upload(dp->ops, dp->plan.dev_ops, s);
             mqa::upload(dp->tap_ids, dp->plan.dev_tap_slots, s);
+            if (!dp->plan.dev_terms.empty()) {
+                mqa::upload(dp->terms, dp->plan.dev_terms, s);
+                mqa::upload(dp->phase_coefs, dp->plan.phase_coefs, s);
+            }
             const std::vector<uint8_t> forms = mqa::jit_coef_forms(dp->plan);  // alive until the sync
             mqa::upload(dp->coef_forms, forms, s);
             dp->n_coef_forms = int(forms.size());
@@ -324,7 +335,7 @@ struct mqa_circuit {
     // The segments drop the same phases as the back-sweep plan adj (a phase-free back-sweep
     // continues from phase-free checkpoints, so psi and lambda keep one common phase).
     Checkpointed* checkpointed(int n_eff, int dtype, int n_ck, const mqa::DevPlan* adj, hipStream_t s) {
-        auto o = options(dtype, false, adj->plan.opts.phase_free);
+        auto o = options(dtype, false, adj->plan.opts.phase_free, adj->plan.opts.phase_ops);
         std::lock_guard<std::mutex> lk(mu);
         auto key = std::make_tuple(adj, n_ck, o.tile_qubits, o.low_run, o.group_width, o.max_ops_per_pass);
         auto it = ckpt_plans.find(key);
@@ -338,7 +349,11 @@ struct mqa_circuit {
         std::vector<int> pass_of(gates.size(), -1);  // back-sweep pass that undoes each gate
         for (int j = 0; j < P; j++)
             for (auto& g : adj->plan.passes[size_t(j)].groups)
-                for (auto& op : g.ops) pass_of[size_t(op.gate)] = j;
+                for (auto& op : g.ops) {
+                    std::vector<int> gs;
+                    mqa::op_gates(op, gs);
+                    for (int gi : gs) pass_of[size_t(gi)] = j;
+                }
         const int n = int(cp->bound.size());
         for (int k = 0; k <= n; k++) {
             std::vector<mqa::CGate> seg = gates;
@@ -477,6 +492,7 @@ struct mqa_sim {
     // workspaces (grow-only)
     mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
     mqa::DevBuf coefs;  // coefficient table of the compiled passes (rebuilt per run_plan call)
+    mqa::DevBuf phase;  // phase table of the running plan's phase blocks (rebuilt per run_plan call)
     mqa::DevBuf custom[2];  // rows of the custom-gate matrices of the running circuits (two: bra and ket sides)
     mqa::DevBuf dense;      // matrices of the dense blocks of mqa_sim_apply_dense_blocks
     mqa::Timer timer;
@@ -587,6 +603,18 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
                   "coefficient table");
         a.coefs = sim->coefs.p;
     }
+    // phase blocks: the table of the terms' e^{i phi} follows this call's angles
+    if (!pl.dev_terms.empty()) {
+        const size_t real = sim->dtype == MQA_C64 ? sizeof(float) : sizeof(double);
+        const int nt = int(pl.dev_terms.size());
+        sim->phase.ensure(size_t(batch) * size_t(nt) * 2 * real);
+        hip_check(launch_phase_coefs(sim->dtype, angles, dp->phase_coefs.as<PhaseCoef>(), nt, n_slots, batch,
+                                     sim->phase.p, sim->stream),
+                  "phase table");
+        a.terms = dp->terms.as<DevPhaseTerm>();
+        a.phase = sim->phase.p;
+        a.n_phase = nt;
+    }
     // a single-buffer pass holds one tile of LDS (the forward layout) plus the tap table
     const bool one_tile = jp && jp->form() == JitForm::kSingleBuffer;
     const size_t lds =
@@ -612,6 +640,17 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
         if (jp) count_jit_launch();
         if (one_tile) g_jit_sb_launches.fetch_add(1);
     }
+}
+
+// Phase blocks (option phase_ops) for plans that compile (jit on: large launches at once, smaller
+// ones once they recur); the interpreted pass applies a block term by term, which costs more than
+// the per-gate ops it replaces, so with jit off the plans keep one op per gate.  Value 2 uses
+// them whatever the jit setting.
+// Launches below 2^22 amplitudes (a single small state) are latency-bound: there the serial
+// products of a block cost more than the per-gate ops, so their plans keep one op per gate.
+bool want_phase_ops(uint64_t batch, uint64_t dim) {
+    if (g_opt.phase_ops == 2) return true;
+    return g_opt.phase_ops != 0 && g_opt.jit != 0 && double(batch) * double(dim) >= 4194304.0;
 }
 
 // Device rows [b0, b0 + nb) of a circuit's custom-gate matrices (a one-row table serves every sample),
@@ -1221,19 +1260,21 @@ int mqa_circuit_schedule(const mqa_circuit* circ_c, int32_t n_qubits, int32_t dt
         require(circ_c != nullptr && len != nullptr, "NULL argument");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ_c->max_qubit && n_qubits <= 40, "bad qubit count");
+        require(adjoint >= 0 && (adjoint & ~19) == 0, "adjoint must be 0..3, optionally + 16");
         auto* circ = const_cast<mqa_circuit*>(circ_c);
-        DevPlan* dp = circ->get_plan(n_eff_of(n_qubits), dtype, adjoint != 0, nullptr, false);
+        DevPlan* dp = circ->get_plan(n_eff_of(n_qubits), dtype, (adjoint & 1) != 0, nullptr, false, (adjoint & 2) != 0,
+                                     (adjoint & 16) != 0);
         std::vector<int64_t> w;
         w.push_back(int64_t(dp->plan.passes.size()));
         for (auto& ps : dp->plan.passes) {
             uint64_t L = 0;
             for (int q : ps.local_qubits) L |= uint64_t(1) << q;
             w.push_back(int64_t(L));
-            int64_t cnt = 0;
-            for (auto& g : ps.groups) cnt += int64_t(g.ops.size());
-            w.push_back(cnt);
+            std::vector<int> gs;
             for (auto& g : ps.groups)
-                for (auto& op : g.ops) w.push_back(op.gate);
+                for (auto& op : g.ops) op_gates(op, gs);
+            w.push_back(int64_t(gs.size()));
+            for (int gi : gs) w.push_back(gi);
         }
         *len = int64_t(w.size());
         if (out && cap >= int64_t(w.size())) std::memcpy(out, w.data(), w.size() * sizeof(int64_t));
@@ -1246,9 +1287,10 @@ int mqa_circuit_jit_compile(const mqa_circuit* circ, int32_t n_qubits, int32_t d
         require(circ != nullptr, "circuit is NULL");
         require(dtype == MQA_C64 || dtype == MQA_C128, "bad dtype");
         require(n_qubits > circ->max_qubit && n_qubits <= 40, "bad qubit count");
-        require(adjoint >= 0 && adjoint <= 11 && (adjoint & 4) == 0, "adjoint must be 0..3, optionally + 8");
+        require(adjoint >= 0 && (adjoint & ~27) == 0, "adjoint must be 0..3, optionally + 8 and + 16");
         const bool adj = (adjoint & 1) != 0, pf = (adjoint & 2) != 0, one_tile = (adjoint & 8) != 0;
-        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf));
+        const bool ops = (adjoint & 16) != 0;
+        const Plan pl = build_plan(circ->gates, n_eff_of(n_qubits), adj, circ->options(dtype, adj, pf, ops));
         require(!one_tile || jit_can_single_buffer(pl, dtype),
                 "single-buffer passes are back-sweeps without dense 3- or 4-qubit blocks");
         JitPlan jp(pl, dtype, one_tile ? JitForm::kSingleBuffer : JitForm::kTile);
@@ -1450,7 +1492,7 @@ int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double*
         // phase restored by one scaling pass; the angle table is computed here, so the phase is
         // decided on exactly the values the kernels read
         const bool pf = g_opt.phase_free.load() != 0 && circ->n_slots > 0;
-        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true, pf);
+        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true, pf, want_phase_ops(1, sim->dim_eff));
         const double* ang = nullptr;
         std::vector<double> tab;
         std::complex<double> phase(1.0, 0.0);
@@ -1709,8 +1751,9 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         bool pf = g_opt.phase_free.load() != 0;
         for (int h = 0; h < n_hams; h++) pf = pf && hams[h]->hermitian;
         g_pf_used.store(pf ? 1 : 0);
-        DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true, pf);
-        DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true, pf);
+        const bool ops = want_phase_ops(uint64_t(batch), sim->dim_eff);
+        DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true, pf, ops);
+        DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true, pf, ops);
         DevBuf* sl = circ->slots_on_device(s);
         const int n_tap_ids = circ->n_tap_ids;
         const uint64_t n_tiles = uint64_t(1) << (adj->plan.n_eff - adj->plan.K);

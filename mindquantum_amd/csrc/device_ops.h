@@ -1083,4 +1083,68 @@ __device__ __forceinline__ void taps4_to_lds(const TileFrame<T>& f, T t0, int i0
     if (idx >= 0) f.lds_taps[idx * f.n_waves + f.wave] = double(e);
 }
 
+
+// ---------------------------------------------------------------- phase blocks (OPK_PHASE)
+template <typename T>
+__device__ __forceinline__ void cmul(T& ar, T& ai, T br, T bi) {
+    const T r = ar * br - ai * bi;
+    ai = ar * bi + ai * br;
+    ar = r;
+}
+// a *= on ? b : 1
+template <typename T>
+__device__ __forceinline__ void cmul_if(T& ar, T& ai, T br, T bi, bool on) {
+    const T r = ar * br - ai * bi, i = ar * bi + ai * br;
+    ar = on ? r : ar;
+    ai = on ? i : ai;
+}
+
+// Interpreted phase block: its terms one after another, each over the 16 registers (the generated
+// passes factor the block into per-thread and per-register products instead).  z: the sample's
+// row of the phase table.  Back-sweep: the conjugate phases, and each tap as one reduction of its
+// terms' weighted sums of Im(conj(lambda) psi), a quantity the block's phases leave unchanged.
+template <typename T, bool ADJ>
+__device__ __forceinline__ void phase_block(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g,
+                                            const DevPhaseTerm* __restrict__ terms, const T* __restrict__ z,
+                                            const TileFrame<T>& f) {
+    T acc = T(0);
+    int cur = -1;
+    for (int k = 0; k < op.slot; k++) {
+        const DevPhaseTerm t = terms[op.mat + k];
+        const T zr = z[2 * t.coef];
+        const T zi = ADJ ? -z[2 * t.coef + 1] : z[2 * t.coef + 1];
+        const uint64_t gm = f.base & t.mg;
+        const bool tile_on = gm == t.mg;
+        const uint32_t tile_par = uint32_t(__popcll(gm)) & 1u;
+        const bool is_and = t.kind == PHT_AND;
+        T s = T(0);
+#pragma unroll
+        for (int v = 0; v < kAmpsPerThread; v++) {
+            const uint32_t u = (g.ub | ((v & 1) ? g.pm[0] : 0u) | ((v & 2) ? g.pm[1] : 0u) | ((v & 4) ? g.pm[2] : 0u) |
+                                ((v & 8) ? g.pm[3] : 0u)) &
+                               t.ml;
+            const bool on = is_and ? (tile_on && u == t.ml) : (((uint32_t(__popc(u)) + tile_par) & 1u) != 0u);
+            if constexpr (ADJ) {
+                const T m = l.r[v] * p.i[v] - l.i[v] * p.r[v];
+                s += on ? m : T(0);
+            }
+            cmul_if(p.r[v], p.i[v], zr, zi, on);
+            if constexpr (ADJ) cmul_if(l.r[v], l.i[v], zr, zi, on);
+        }
+        if constexpr (ADJ) {
+            if (t.tap >= 0) {  // a gate's terms are consecutive and share its tap
+                if (t.tap != cur && cur >= 0) {
+                    tap_to_lds(f, acc, cur);
+                    acc = T(0);
+                }
+                cur = t.tap;
+                acc += T(t.weight) * s;
+            }
+        }
+    }
+    if constexpr (ADJ) {
+        if (cur >= 0) tap_to_lds(f, acc, cur);
+    }
+}
+
 }  // namespace mqa

@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 
@@ -127,6 +129,246 @@ void emit_phase_free(std::ostringstream& b, const DevOp& op, bool adj, bool read
     b << "    }\n";
 }
 
+// Body of a phase block (OPK_PHASE) in a group with register positions grp.pos.  Each term's
+// indicator splits into register bits R (compile-time per register), thread bits (g.ub) and
+// tile bits (f.base): terms without register bits multiply a per-thread factor E, terms on one
+// register bit j a factor F_j, and register v's phase is E * prod_{j in v} F_j (times the rare
+// multi-register terms), built once per register and applied to psi (and lambda).  Back-sweep
+// taps: sums of m_v = Im(conj(lambda_v) psi_v) over each term's registers, weighted and reduced
+// four at a time.
+void emit_phase_block(std::ostringstream& b, const Plan& pl, const DevOp& op, const DevGroup& grp, bool adj) {
+    uint32_t posmask = 0;
+    for (int j = 0; j < 4; j++) posmask |= uint32_t(1) << grp.pos[j];
+    auto hex32 = [](uint32_t x) {
+        std::ostringstream o;
+        o << "0x" << std::hex << x << "u";
+        return o.str();
+    };
+    auto hex64 = [](uint64_t x) {
+        std::ostringstream o;
+        o << "0x" << std::hex << x << "ull";
+        return o.str();
+    };
+    auto num = [](double x) {
+        std::ostringstream o;
+        o << std::setprecision(17) << x;
+        return o.str();
+    };
+    struct Factor {
+        bool id = true;
+        std::string r, i;
+    };
+    Factor E{true, "er", "ei"};
+    Factor F[4];
+    for (int j = 0; j < 4; j++) F[j] = Factor{true, "f" + std::to_string(j) + "r", "f" + std::to_string(j) + "i"};
+    struct Extra {
+        uint32_t R;
+        bool parity;
+        std::string s;  // PARITY: runtime flip of the term's parity ("" when the flip is 0)
+        std::string x;  // variable name stem (x##r, x##i)
+    };
+    std::vector<Extra> extras;
+    std::ostringstream decl, body, taps_code;
+    body << "    {  // phase block, " << op.slot << " terms\n";
+    // multiply factor X by (zr, zi), when cond holds (cond empty: always)
+    auto mul = [&](Factor& X, const std::string& zr, const std::string& zi, const std::string& cond) {
+        if (X.id) {
+            if (cond.empty())
+                body << "      " << X.r << " = " << zr << ", " << X.i << " = " << zi << ";\n";
+            else
+                body << "      " << X.r << " = (" << cond << ") ? " << zr << " : T(1), " << X.i << " = (" << cond
+                     << ") ? " << zi << " : T(0);\n";
+            X.id = false;
+        } else if (cond.empty()) {
+            body << "      cmul<T>(" << X.r << ", " << X.i << ", " << zr << ", " << zi << ");\n";
+        } else {
+            body << "      cmul_if<T>(" << X.r << ", " << X.i << ", " << zr << ", " << zi << ", " << cond << ");\n";
+        }
+    };
+    // taps: pass tap index -> accumulator variable
+    std::vector<int> tap_idx;
+    std::map<std::string, bool> sums;  // named register sums of m_v emitted so far
+    bool want_m = false;
+    for (int k = 0; k < op.slot; k++) want_m = want_m || (adj && pl.dev_terms.at(size_t(op.mat + k)).tap >= 0);
+    if (want_m)
+        for (int v = 0; v < 16; v++)
+            taps_code << "      const T m" << v << " = l.r[" << v << "] * p.i[" << v << "] - l.i[" << v << "] * p.r["
+                      << v << "];\n";
+    auto sum_of = [&](const std::string& name, auto pred) {
+        if (!sums.count(name)) {
+            taps_code << "      const T " << name << " = ";
+            bool first = true;
+            for (int v = 0; v < 16; v++)
+                if (pred(v)) taps_code << (first ? "" : " + ") << "m" << v, first = false;
+            if (first) taps_code << "T(0)";
+            taps_code << ";\n";
+            sums[name] = true;
+        }
+        return name;
+    };
+    for (int k = 0; k < op.slot; k++) {
+        const DevPhaseTerm& t = pl.dev_terms.at(size_t(op.mat + k));
+        uint32_t R = 0;
+        for (int j = 0; j < 4; j++)
+            if ((t.ml >> grp.pos[j]) & 1u) R |= 1u << j;
+        const uint32_t H = t.ml & ~posmask;
+        const std::string zr = "zrow[" + std::to_string(2 * t.coef) + "]";
+        const std::string zi = std::string(adj ? "-" : "") + "zrow[" + std::to_string(2 * t.coef + 1) + "]";
+        const std::string zrn = "z" + std::to_string(k) + "r", zin = "z" + std::to_string(k) + "i";
+        body << "      const T " << zrn << " = " << zr << ", " << zin << " = " << zi << ";\n";
+        std::string cond;  // AND: the term holds on this thread's registers covering R
+        std::string sflip;  // PARITY: the term's parity over thread and tile bits
+        if (t.kind == PHT_AND) {
+            if (H) cond = "(g.ub & " + hex32(H) + ") == " + hex32(H);
+            if (t.mg) cond += std::string(cond.empty() ? "" : " && ") + "(f.base & " + hex64(t.mg) + ") == " + hex64(t.mg);
+        } else {
+            std::string par;
+            if (H) par = "__popc(g.ub & " + hex32(H) + ")";
+            if (t.mg) par += std::string(par.empty() ? "" : " + ") + "__popcll(f.base & " + hex64(t.mg) + ")";
+            if (!par.empty()) sflip = "((" + par + ") & 1) != 0";
+        }
+        const std::string condv = "c" + std::to_string(k);
+        if (!cond.empty()) body << "      const bool " << condv << " = " << cond << ";\n";
+        if (!sflip.empty()) body << "      const bool " << condv << " = " << sflip << ";\n";
+        const std::string cnd = (cond.empty() && sflip.empty()) ? std::string() : condv;
+        const int nr = __builtin_popcount(R);
+        if (t.kind == PHT_AND) {
+            if (nr == 0) mul(E, zrn, zin, cnd);
+            else if (nr == 1) mul(F[__builtin_ctz(R)], zrn, zin, cnd);
+            else {
+                const std::string x = "x" + std::to_string(k);
+                if (cnd.empty()) body << "      const T " << x << "r = " << zrn << ", " << x << "i = " << zin << ";\n";
+                else
+                    body << "      const T " << x << "r = " << cnd << " ? " << zrn << " : T(1), " << x << "i = " << cnd
+                         << " ? " << zin << " : T(0);\n";
+                extras.push_back(Extra{R, false, "", x});
+            }
+        } else {
+            if (nr == 0) {
+                mul(E, zrn, zin, cnd);
+            } else if (nr == 1) {
+                // parity = v_j ^ s: z on v_j = 1 when s = 0; when s = 1, z on every register and
+                // conj(z) on v_j = 1
+                Factor& Fj = F[__builtin_ctz(R)];
+                if (cnd.empty()) {
+                    mul(Fj, zrn, zin, "");
+                } else {
+                    mul(E, zrn, zin, cnd);
+                    const std::string zjn = "zj" + std::to_string(k);
+                    body << "      const T " << zjn << " = " << cnd << " ? -" << zin << " : " << zin << ";\n";
+                    mul(Fj, zrn, zjn, "");
+                }
+            } else {
+                const std::string x = "x" + std::to_string(k);
+                body << "      const T " << x << "r = " << zrn << ", " << x << "i = " << zin << ";\n";
+                extras.push_back(Extra{R, true, cnd, x});
+            }
+        }
+        // back-sweep tap of the term: weight * sum of m_v over the registers where it holds
+        if (adj && t.tap >= 0) {
+            std::string val;
+            if (t.kind == PHT_AND) {
+                const std::string M =
+                    R == 0 ? sum_of("sall", [](int) { return true; })
+                           : sum_of("ma" + std::to_string(R), [R](int v) { return (uint32_t(v) & R) == R; });
+                val = cnd.empty() ? M : "(" + cnd + " ? " + M + " : T(0))";
+            } else if (R == 0) {
+                const std::string S = sum_of("sall", [](int) { return true; });
+                val = "(" + cnd + " ? " + S + " : T(0))";
+            } else {
+                const std::string O =
+                    sum_of("mo" + std::to_string(R), [R](int v) { return (__builtin_popcount(uint32_t(v) & R) & 1) != 0; });
+                if (cnd.empty()) {
+                    val = O;
+                } else {
+                    const std::string S = sum_of("sall", [](int) { return true; });
+                    val = "(" + cnd + " ? (" + S + " - " + O + ") : " + O + ")";
+                }
+            }
+            size_t a = 0;
+            while (a < tap_idx.size() && tap_idx[a] != t.tap) a++;
+            if (a == tap_idx.size()) tap_idx.push_back(t.tap);
+            body << "      ptap" << a << " += T(" << num(t.weight) << ") * " << val << ";\n";
+        }
+    }
+    // register phases: Q[v] = Q[v without its highest bit] * F_hb (* multi-register AND terms),
+    // each applied as soon as it is built, in depth-first order over that tree (few live values)
+    std::vector<Factor> Q(16);
+    Q[0] = E;
+    auto apply_reg = [&](int v) {
+        Factor a = Q[size_t(v)];
+        bool copied = false;
+        for (const Extra& x : extras) {
+            if (!x.parity) continue;
+            const bool odd = (__builtin_popcount(uint32_t(v) & x.R) & 1) != 0;
+            if (x.s.empty() && !odd) continue;
+            if (!copied) {  // the multi-register parity terms of register v on a copy
+                const std::string nr = "qa" + std::to_string(v) + "r", ni = "qa" + std::to_string(v) + "i";
+                if (a.id) body << "      T " << nr << " = T(1), " << ni << " = T(0);\n";
+                else body << "      T " << nr << " = " << a.r << ", " << ni << " = " << a.i << ";\n";
+                a = Factor{false, nr, ni};
+                copied = true;
+            }
+            if (x.s.empty())
+                body << "      cmul<T>(" << a.r << ", " << a.i << ", " << x.x << "r, " << x.x << "i);\n";
+            else
+                body << "      cmul_if<T>(" << a.r << ", " << a.i << ", " << x.x << "r, " << x.x << "i, "
+                     << (odd ? "!" : "") << x.s << ");\n";
+        }
+        if (a.id) return;
+        body << "      cmul<T>(p.r[" << v << "], p.i[" << v << "], " << a.r << ", " << a.i << ");\n";
+        if (adj) body << "      cmul<T>(l.r[" << v << "], l.i[" << v << "], " << a.r << ", " << a.i << ");\n";
+    };
+    auto build = [&](auto&& self, int v) -> void {
+        if (v > 0) {
+            const int hb = 31 - __builtin_clz(uint32_t(v));
+            const Factor& P = Q[size_t(v & ~(1 << hb))];
+            std::vector<std::pair<std::string, std::string>> mults;
+            if (!F[hb].id) mults.push_back({F[hb].r, F[hb].i});
+            for (const Extra& x : extras)
+                if (!x.parity && ((x.R >> hb) & 1u) && (uint32_t(v) & x.R) == x.R) mults.push_back({x.x + "r", x.x + "i"});
+            if (mults.empty()) {
+                Q[size_t(v)] = P;
+            } else {
+                Factor q{false, "q" + std::to_string(v) + "r", "q" + std::to_string(v) + "i"};
+                size_t m0 = 0;
+                if (P.id) {
+                    body << "      T " << q.r << " = " << mults[0].first << ", " << q.i << " = " << mults[0].second
+                         << ";\n";
+                    m0 = 1;
+                } else {
+                    body << "      T " << q.r << " = " << P.r << ", " << q.i << " = " << P.i << ";\n";
+                }
+                for (size_t m = m0; m < mults.size(); m++)
+                    body << "      cmul<T>(" << q.r << ", " << q.i << ", " << mults[m].first << ", " << mults[m].second
+                         << ");\n";
+                Q[size_t(v)] = q;
+            }
+        }
+        apply_reg(v);
+        const int lo = v == 0 ? 0 : 32 - __builtin_clz(uint32_t(v));  // children add a higher bit
+        for (int k = lo; k < 4; k++) self(self, v | (1 << k));
+    };
+    build(build, 0);
+    // factor declarations, tap accumulators; taps before the phases (m_v does not change)
+    if (!E.id) decl << "      T er, ei;\n";
+    for (int j = 0; j < 4; j++)
+        if (!F[j].id) decl << "      T " << F[j].r << ", " << F[j].i << ";\n";
+    for (size_t a = 0; a < tap_idx.size(); a++) decl << "      T ptap" << a << " = T(0);\n";
+    std::string bs = body.str();
+    const std::string head = "    {  // phase block, " + std::to_string(op.slot) + " terms\n";
+    b << head << decl.str() << taps_code.str() << bs.substr(head.size());
+    for (size_t a = 0; a < tap_idx.size(); a += 4) {
+        b << "      taps4_to_lds<T>(f";
+        for (size_t j = a; j < a + 4; j++) {
+            if (j < tap_idx.size()) b << ", ptap" << j << ", " << tap_idx[j];
+            else b << ", T(0), -1";
+        }
+        b << ");\n";
+    }
+    b << "    }\n";
+}
+
 }  // namespace
 
 std::vector<uint8_t> jit_coef_forms(const Plan& pl) {
@@ -164,10 +406,15 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
     const char* T = dtype == 0 ? "float" : "double";
     const char* A = adj ? "true" : "false";
     const unsigned nt = 1u << (pl.K - kGroupWidth);
+    // waves per SIMD the LDS of a complex64 pass allows (forward and single-buffer tiles: four;
+    // two tile buffers: two), as the register budget the compiler must meet; complex128 passes
+    // hold twice the registers and are left to the compiler
+    const int min_waves = dtype != 0 ? 1 : (adj && !sb ? 2 : 4);
     std::ostringstream o;
     o << "#include \"device_ops.h\"\n"
       << "namespace mqa {\n"
-      << "extern \"C\" __global__ void __launch_bounds__(" << nt << ") " << jit_kernel_name(adj) << "(TilePassArgs args) {\n"
+      << "extern \"C\" __global__ void __launch_bounds__(" << nt << ", " << min_waves << ") " << jit_kernel_name(adj)
+      << "(TilePassArgs args) {\n"
       << "  using T = " << T << ";\n"
       << "  extern __shared__ __align__(16) unsigned char smem[];\n"
       << "  DevPass pass{};\n"
@@ -200,6 +447,10 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         o << "  const_ptr<T> coef_row = as_const(static_cast<const T*>(args.coefs) + 2 * f.sample * "
              "uint64_t(args.n_slots));\n"
           << "  (void)coef_row;\n";
+    if (!pl.dev_terms.empty())
+        o << "  const_ptr<T> zrow = as_const(static_cast<const T*>(args.phase) + 2 * f.sample * "
+             "uint64_t(args.n_phase));\n"
+          << "  (void)zrow;\n";
     // single buffer: every group's layout at function scope; psi, then lambda, into group 0's layout
     auto sb_move = [&](const char* what, int gi, const char* reg) {
         o << "  group_" << what << "1<T>(f.t_psi, swz<T>(group_ctx(f.tid, pos_" << gi << ", spos_" << gi
@@ -255,6 +506,10 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         std::ostringstream b;                   // the group's ops
         for (int oi = 0; oi < g.n_ops; oi++) {
             const DevOp& op = pl.dev_ops.at(size_t(g.op_begin + oi));
+            if (op.kind == OPK_PHASE) {
+                emit_phase_block(b, pl, op, g, adj);
+                continue;
+            }
             const bool gated = op.ctrl_global != 0;
             const char* ctrl = op.ctrl_local ? "true" : "false";
             // CNOT across the tile in a back-sweep pass: selects (a branch there can make the
@@ -333,7 +588,21 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
             for (size_t k = 0; k < taps.size(); k++) o << (k ? ", " : " ") << "tap" << k << " = T(0)";
             o << ";\n";
         }
-        o << b.str();
+        // each tap is pinned where it is taken: left free, the compiler sinks the tap products to the
+        // group's reduction, past the rotations that follow, and keeps both register states alive
+        {
+            std::istringstream in(b.str());
+            std::string line;
+            while (std::getline(in, line)) {
+                o << line << "\n";
+                const size_t t = line.find_first_not_of(' ');
+                if (t != std::string::npos && line.compare(t, 3, "tap") == 0) {
+                    const size_t eq = line.find(" = ", t);
+                    if (eq != std::string::npos && line.back() == ';')
+                        o << std::string(t, ' ') << "asm volatile(\"\" : \"+v\"(" << line.substr(t, eq - t) << "));\n";
+                }
+            }
+        }
         for (size_t k = 0; k < taps.size(); k += 4) {  // four taps per batched cross-lane reduction
             o << "    taps4_to_lds<T>(f";
             for (size_t j = k; j < k + 4; j++) {

@@ -132,7 +132,8 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
             for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0);
         }
         const DevOp op0 = args.ops[grp.op_begin];
-        if (FULL && (op0.kind == OPK_M3 || op0.kind == OPK_M4)) {  // a dense block alone in its group
+        int first = 0;
+        if (FULL && (op0.kind == OPK_M3 || op0.kind == OPK_M4)) {  // a dense block opens its group
             if ((f.base & op0.ctrl_global) == op0.ctrl_global) {
                 const T* m = static_cast<const T*>(args.pool) + op0.mat;
                 if (op0.kind == OPK_M3) {
@@ -143,11 +144,15 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
                     if constexpr (ADJ) dense_from_lds<T, 4>(l, g, op0.ctrl_local, m, f.t_lam);
                 }
             }
-        } else {
-            for (int oi = 0; oi < grp.n_ops; oi++) {
-                const DevOp op = args.ops[grp.op_begin + oi];
+            first = 1;  // phase blocks may follow it
+        }
+        for (int oi = first; oi < grp.n_ops; oi++) {
+            const DevOp op = args.ops[grp.op_begin + oi];
+            if (FULL && op.kind == OPK_PHASE)  // plans with phase blocks run the FULL instantiation
+                phase_block<T, ADJ>(op, p, l, g, args.terms,
+                                    static_cast<const T*>(args.phase) + 2 * f.sample * uint64_t(args.n_phase), f);
+            else
                 run_op<T, ADJ, FULL>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc, pm_row);
-            }
         }
         if constexpr (ADJ) {
             if (grp.n_taps > 0) {  // one batched cross-lane reduction per group
@@ -588,6 +593,29 @@ __global__ void __launch_bounds__(256) k_coefs(const double* __restrict__ ang, c
     out[2 * e + 1] = s;
 }
 
+// Phase table of a plan's phase blocks: per (sample, term), e^{i (k alpha + b)} in the real type,
+// alpha the angle the sample's angle table holds for the term's slot (PhaseCoef), evaluated in
+// double from its (cos, sin) and rounded once.
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase_coefs(const double* __restrict__ ang, const PhaseCoef* __restrict__ pc,
+                                                     int n_terms, int n_slots, uint64_t total, T* __restrict__ out) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const uint64_t b = e / uint64_t(n_terms);
+    const PhaseCoef c = pc[e % uint64_t(n_terms)];
+    double x = 1.0, y = 0.0;
+    if (c.slot >= 0 && ang != nullptr) {
+        const double ca = ang[2 * (b * uint64_t(n_slots) + uint64_t(c.slot))];
+        const double sa = ang[2 * (b * uint64_t(n_slots) + uint64_t(c.slot)) + 1];
+        const int k = c.k < 0 ? -c.k : c.k;
+        if (k == 1) x = ca, y = sa;
+        else if (k == 2) x = ca * ca - sa * sa, y = 2.0 * ca * sa;
+        if (c.k < 0) y = -y;
+    }
+    out[2 * e] = T(x * c.cos_b - y * c.sin_b);
+    out[2 * e + 1] = T(x * c.sin_b + y * c.cos_b);
+}
+
 // rows x n (re, im) -> rows (re, im), one wave per row, fixed order (deterministic)
 __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ in, uint64_t n, double* __restrict__ out,
                                                     uint64_t rows) {
@@ -872,6 +900,20 @@ hipError_t launch_coefs(int dtype, const double* angles, const uint8_t* forms, i
                            static_cast<float*>(out));
     else
         hipLaunchKernelGGL(k_coefs<double>, grid, dim3(256), 0, s, angles, forms, n_forms, n_slots, total,
+                           static_cast<double*>(out));
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_coefs(int dtype, const double* angles, const PhaseCoef* pc, int n_terms, int n_slots,
+                              uint64_t batch, void* out, hipStream_t s) {
+    const uint64_t total = batch * uint64_t(n_terms);
+    if (total == 0) return hipSuccess;
+    const dim3 grid(uint32_t((total + 255) / 256));
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_phase_coefs<float>, grid, dim3(256), 0, s, angles, pc, n_terms, n_slots, total,
+                           static_cast<float*>(out));
+    else
+        hipLaunchKernelGGL(k_phase_coefs<double>, grid, dim3(256), 0, s, angles, pc, n_terms, n_slots, total,
                            static_cast<double*>(out));
     return hipGetLastError();
 }

@@ -96,6 +96,9 @@ hipError_t launch_angles(const AngleArgs& a, hipStream_t s);
 // slots' CoefForm entries (forms[0, n_forms); slots beyond: the (cos, sin) pair).
 hipError_t launch_coefs(int dtype, const double* angles, const uint8_t* forms, int n_forms, int n_slots, uint64_t batch,
                         void* out, hipStream_t s);
+// Phase table of a plan's phase blocks ([batch][n_terms][2] in the real type; PhaseCoef).
+hipError_t launch_phase_coefs(int dtype, const double* angles, const PhaseCoef* pc, int n_terms, int n_slots,
+                              uint64_t batch, void* out, hipStream_t s);
 hipError_t launch_reduce_rows(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
 hipError_t launch_reduce_rows1(const double* in, uint64_t n, double* out, uint64_t rows, hipStream_t s);
 hipError_t launch_jacobian(const double* taps, int n_tap_ids, const int32_t* jac_begin, const int32_t* jac_tap,

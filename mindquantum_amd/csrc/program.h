@@ -26,6 +26,13 @@ constexpr int kMaxTileQubits = 13;                // largest tile: 512 threads
 // Device op kinds.
 enum OpKind : int32_t {
     OPK_X = 1,   // swap the pair (X, CNOT, Toffoli...)
+    // Diagonal phase block: a run of commuting diagonal gates (RZ, PS, RZZ) merged into one op
+    // whose qubits need not be register positions (nor local): DevOp::mat = index of the first
+    // DevPhaseTerm of the plan, DevOp::slot = number of terms.  Every amplitude i is multiplied by
+    // prod_t z_t^{f_t(i)} (z_t = e^{i phi_t} from the per-sample phase table, conjugated in the
+    // back-sweep); the back-sweep taps are Re(tap) += weight_t * sum_{f_t(i) = 1} Im(conj(lambda_i)
+    // psi_i), which no phase of the block changes.
+    OPK_PHASE = 2,
     OPK_M1 = 5,  // dense 2x2 from the pool (Y, Z, H, S, T, SX, 1-qubit matrices)
     OPK_SWAP = 6,
     OPK_M2 = 7,  // dense 4x4 from the pool (ISWAP, 2-qubit matrices)
@@ -119,6 +126,30 @@ struct DevOp {             // 32 bytes
     uint64_t ctrl_global;  // control qubits that are not local, as a global index mask
 };
 
+// One term of a phase block (OPK_PHASE): its indicator f(i) over the amplitude index is
+//   PHT_AND:    every qubit of the term set in i (an empty term: every amplitude);
+//   PHT_PARITY: the parity of i over the term's qubits;
+// the qubits split into tile positions (ml) and qubits outside the tile (mg, global index bits,
+// uniform over a tile).
+enum PhaseTermKind : int32_t { PHT_AND = 0, PHT_PARITY = 1 };
+struct DevPhaseTerm {      // 32 bytes
+    uint64_t mg;           // qubits outside the tile (global index mask)
+    uint32_t ml;           // qubits inside the tile (tile-position mask)
+    int32_t kind;          // PhaseTermKind
+    int32_t coef;          // column of z_t in the per-sample phase table
+    int32_t tap;           // pass tap index (back-sweep), -1: none
+    double weight;         // Re(tap) += weight * sum_{f_t(i) = 1} Im(conj(lambda_i) psi_i)
+};
+// Phase of a term from the per-sample angle table: phi = k * alpha_slot + b, where alpha_slot is
+// the angle the table holds for the slot ((cos, sin) of the half angle, or of the angle for PS /
+// GP); slot < 0: the constant e^{i b}.
+struct PhaseCoef {         // 32 bytes
+    int32_t slot;
+    int32_t k;             // -2..2
+    double cos_b, sin_b;
+    double pad;
+};
+
 // XOR swizzle of a tile index into its LDS slot.  It folds higher index bits into the low bits
 // so that register groups whose positions include low qubits do not send every lane of a wave
 // to one bank.  It is linear over GF(2): swz(a ^ b) == swz(a) ^ swz(b), which lets the kernel
@@ -194,6 +225,11 @@ struct TilePassArgs {
     uint64_t src_stride;
     // 1: the pass's output states are dead (the last back-sweep pass): only its taps are written.
     int32_t skip_store;
+    // phase blocks (OPK_PHASE): the plan's terms and the per-sample table of their z_t
+    // ([batch][n_phase][2] in the real type; the row of sample b at phase + 2 * b * n_phase)
+    const DevPhaseTerm* terms;
+    const void* phase;
+    int32_t n_phase;
 };
 
 // Hamiltonian tile pass: the flip groups whose flip mask lies inside one tile's local qubits are

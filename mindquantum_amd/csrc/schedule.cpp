@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <stdexcept>
 
 #include "../../include/mqamd.h"
@@ -97,6 +98,67 @@ static int lower_kind(const CGate& g, bool adjoint, std::vector<cplx>* mat) {
     }
 }
 
+// Qubits a gate acts on other than diagonally: two gates commute when neither one's non-diagonal
+// qubits meet the other's qubits (a diagonal gate commutes with diagonal gates and with gates
+// that only use its qubits as controls).
+static uint64_t nondiag_targets(const CGate& g) {
+    switch (g.kind) {
+        case MQA_GATE_Z:
+        case MQA_GATE_S:
+        case MQA_GATE_SDAG:
+        case MQA_GATE_T:
+        case MQA_GATE_TDAG:
+        case MQA_GATE_RZ:
+        case MQA_GATE_PS:
+        case MQA_GATE_GP:
+        case MQA_GATE_RZZ:
+            return 0;
+        default:
+            return g.targets();
+    }
+}
+
+// Terms of a gate that joins phase blocks (RZ, PS, uncontrolled RZZ; false for other gates).  The
+// phase of a term is k * alpha + 0 with alpha the angle the slot's table entry holds (the half
+// angle of RZ / RZZ, the angle of PS), and its tap weight -dphi / dtheta.  Phase-free plans drop
+// the global phase of uncontrolled RZ / RZZ, as their OPK_PF_* forms do.
+static bool phase_terms(const CGate& g, bool phase_free, std::vector<HostPhaseTerm>* out) {
+    out->clear();
+    if (g.slot < 0) return false;
+    auto add = [&](int kind, uint64_t mask, int k, double dalpha) {
+        HostPhaseTerm t;
+        t.kind = kind;
+        t.mask = mask;
+        t.coef.slot = g.slot;
+        t.coef.k = k;
+        t.coef.cos_b = 1.0;
+        t.coef.sin_b = 0.0;
+        t.weight = -double(k) * dalpha;
+        out->push_back(t);
+    };
+    const uint64_t t0 = uint64_t(1) << g.objs[0];
+    switch (g.kind) {
+        case MQA_GATE_RZ:  // diag(e^{-i theta / 2}, e^{i theta / 2}) on the control subspace
+            if (phase_free && g.ctrl == 0) {
+                add(PHT_AND, t0, 2, 0.5);
+            } else {
+                add(PHT_AND, g.ctrl, -1, 0.5);
+                add(PHT_AND, g.ctrl | t0, 2, 0.5);
+            }
+            return true;
+        case MQA_GATE_PS:  // diag(1, e^{i theta})
+            add(PHT_AND, g.ctrl | t0, 1, 1.0);
+            return true;
+        case MQA_GATE_RZZ:  // e^{-i theta / 2} on even parity, e^{i theta / 2} on odd
+            if (g.ctrl != 0 || g.objs.size() != 2) return false;
+            if (!phase_free) add(PHT_AND, 0, -1, 0.5);
+            add(PHT_PARITY, t0 | (uint64_t(1) << g.objs[1]), 2, 0.5);
+            return true;
+        default:
+            return false;
+    }
+}
+
 namespace {
 
 struct PassBuilder {
@@ -105,26 +167,192 @@ struct PassBuilder {
     int W;
     int taps_per_group;
     bool phase_free;
+    bool phase_ops;
     std::vector<int> tap_id_of_slot;  // angle slot -> global tap id (-1: no tap)
+    std::vector<char> phase_ok;       // gate joins phase blocks (phase_ops plans)
+    std::vector<uint64_t> nd;         // nondiag_targets (phase_ops plans), else every qubit
+
+    // Fewest blocks for the pass's register groups: every gate of a phase block may move later,
+    // up to the first op that does not commute with it, so its terms can sit anywhere in an
+    // interval of op positions; one block at the right end of the interval that ends first covers
+    // every interval reaching it (interval stabbing), and so on.
+    void merge_blocks(HostPass* pass) const {
+        struct Seq {
+            int g, k;
+            uint64_t nd;
+        };
+        std::vector<Seq> seq;  // the regular ops in execution order
+        struct Item {
+            int a, b;  // legal positions: after regular op a .. after regular op b - 1 (-1: before all)
+            std::vector<HostPhaseTerm> terms;
+        };
+        std::vector<Item> items;  // one per gate, in execution order
+        for (int g = 0; g < int(pass->groups.size()); g++) {
+            for (int k = 0; k < int(pass->groups[size_t(g)].ops.size()); k++) {
+                const HostOp& op = pass->groups[size_t(g)].ops[size_t(k)];
+                if (op.dev.kind != OPK_PHASE) {
+                    seq.push_back(Seq{g, k, nd[size_t(op.gate)]});
+                    continue;
+                }
+                for (const HostPhaseTerm& t : op.terms) {
+                    if (items.empty() || items.back().terms.back().gate != t.gate || items.back().a != int(seq.size()) - 1)
+                        items.push_back(Item{int(seq.size()) - 1, 0, {}});
+                    items.back().terms.push_back(t);
+                }
+            }
+        }
+        if (items.empty()) return;
+        for (Item& it : items) {
+            const uint64_t q = gates[size_t(it.terms.front().gate)].qubits();
+            it.b = int(seq.size());
+            for (int j = it.a + 1; j < int(seq.size()); j++)
+                if (seq[size_t(j)].nd & q) {
+                    it.b = j;
+                    break;
+                }
+        }
+        std::vector<int> order(items.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return items[size_t(x)].b < items[size_t(y)].b; });
+        std::vector<int> at(items.size(), -2);  // chosen position of each item
+        for (int i : order) {
+            if (at[size_t(i)] != -2) continue;
+            const int p = items[size_t(i)].b - 1;
+            for (size_t j = 0; j < items.size(); j++)
+                if (at[j] == -2 && items[j].a <= p && items[j].b > p) at[j] = p;
+        }
+        // rebuild the groups: the regular ops, each followed by the block of its position
+        std::map<int, HostOp> blocks;
+        for (size_t i = 0; i < items.size(); i++) {
+            HostOp& op = blocks[at[i]];
+            if (op.terms.empty()) {
+                op.gate = items[i].terms.front().gate;
+                op.dev.kind = OPK_PHASE;
+                op.dev.tap = -1;
+                op.dev.mat = -1;
+            }
+            for (auto& t : items[i].terms) op.terms.push_back(t);
+        }
+        for (auto& kv : blocks) kv.second.dev.slot = int(kv.second.terms.size());
+        std::vector<std::vector<HostOp>> ops(pass->groups.size());
+        auto emit = [&](int pos, int g) {
+            auto f = blocks.find(pos);
+            if (f != blocks.end()) ops[size_t(g)].push_back(std::move(f->second)), blocks.erase(f);
+        };
+        // a block before everything goes into the first group, unless that group is a dense 3- / 4-qubit
+        // block (which reads its inputs from the tile): then into a group of its own
+        HostOp lead;
+        bool has_lead = false;
+        for (int i = 0; i < int(seq.size()); i++) {
+            const Seq& sq = seq[size_t(i)];
+            HostOp& cur = pass->groups[size_t(sq.g)].ops[size_t(sq.k)];
+            if (i == 0) {
+                if (cur.dev.kind == OPK_M3 || cur.dev.kind == OPK_M4) {
+                    auto f = blocks.find(-1);
+                    if (f != blocks.end()) lead = std::move(f->second), blocks.erase(f), has_lead = true;
+                } else {
+                    emit(-1, sq.g);
+                }
+            }
+            ops[size_t(sq.g)].push_back(std::move(cur));
+            emit(i, sq.g);
+        }
+        if (seq.empty()) emit(-1, 0);
+        std::vector<HostGroup> kept;
+        if (has_lead) {
+            HostGroup grp;
+            std::vector<int> gpos;
+            pad_positions(grp, gpos, int(pass->local_qubits.size()));
+            grp.tap_begin = pass->groups.front().tap_begin;
+            grp.ops.push_back(std::move(lead));
+            kept.push_back(std::move(grp));
+        }
+        for (size_t g = 0; g < pass->groups.size(); g++) {
+            pass->groups[g].ops = std::move(ops[g]);
+            if (!pass->groups[g].ops.empty()) kept.push_back(std::move(pass->groups[g]));  // no empty groups
+        }
+        pass->groups = std::move(kept);
+    }
+
+    static void pad_positions(HostGroup& grp, std::vector<int>& gpos, int K) {
+        for (int p = 0; int(gpos.size()) < kGroupWidth && p < K; p++)
+            if (std::find(gpos.begin(), gpos.end(), p) == gpos.end()) gpos.push_back(p);
+        for (int i = 0; i < kGroupWidth; i++) grp.pos[i] = gpos[i];
+    }
 
     // Split the pass's gates (dependency-ordered) into register groups of <= W target positions.
+    // Split the pass's gates (dependency-ordered) into register groups of <= W target positions.
+    // Phase-block gates (phase_ops) wait as pending terms while everything applied after them
+    // commutes with them; a gate that does not commute with a pending term waits for the block,
+    // and the block is applied (at the end of the last group) once nothing else can proceed.
     void groups(HostPass* pass, const std::vector<int>& chosen, const std::vector<int>& pos_of, uint64_t nonlocal,
                 std::vector<double>* pool) {
+        const int K = int(pass->local_qubits.size());
         std::vector<int> todo = chosen;
+        std::vector<HostPhaseTerm> pending;
+        uint64_t pending_q = 0;
+        std::vector<int> phase_taps;  // global tap ids of the pass's phase-block taps, in tap order
+        std::vector<HostPhaseTerm> terms;
+        // the pending terms as one op at the end of group `into` (null: the pass's last group)
+        auto flush = [&](HostGroup* into) {
+            if (pending.empty()) return;
+            if (!into && pass->groups.empty()) {  // a pass that opens with a block: a group of its own
+                HostGroup grp;
+                grp.tap_begin = int(pass->tap_slots.size());
+                std::vector<int> gpos;
+                pad_positions(grp, gpos, K);
+                pass->groups.push_back(std::move(grp));
+            }
+            HostOp op;
+            op.gate = pending.front().gate;
+            op.dev.kind = OPK_PHASE;
+            op.dev.slot = int(pending.size());
+            op.dev.tap = -1;
+            op.dev.mat = -1;
+            op.terms = std::move(pending);
+            (into ? into : &pass->groups.back())->ops.push_back(std::move(op));
+            pending.clear();
+            pending_q = 0;
+        };
         while (!todo.empty()) {
             HostGroup grp;
             grp.tap_begin = int(pass->tap_slots.size());
             std::vector<int> gpos;  // positions in register order
             std::vector<int> keep;
-            uint64_t blocked = 0;
+            uint64_t bq = 0, bt = 0;  // qubits of the deferred gates, and their non-diagonal qubits
             bool closed = false;
+            int consumed = 0;
             for (int gi : todo) {
                 const CGate& g = gates[gi];
                 uint64_t q = g.qubits();
-                const bool taps_full = adjoint && g.grad && g.slot >= 0 && grp.n_taps >= taps_per_group;
-                if (closed || (q & blocked) || taps_full) {
-                    blocked |= q;
+                const uint64_t ndq = nd[size_t(gi)];
+                const bool in_block = phase_ok[size_t(gi)] != 0;
+                const bool taps_full =
+                    !in_block && adjoint && g.grad && g.slot >= 0 && grp.n_taps >= taps_per_group;
+                // a gate waits behind a deferred gate it does not commute with; one that does not
+                // commute with pending terms (phase_ops) has them applied first, in this group,
+                // when it joins the group, and waits for them otherwise
+                const bool after_block = !in_block && (ndq & pending_q) != 0;
+                if (closed || (ndq & bq) || (q & bt) || taps_full) {
+                    bq |= q;
+                    bt |= ndq;
                     keep.push_back(gi);
+                    continue;
+                }
+                if (in_block) {
+                    phase_terms(g, phase_free, &terms);
+                    int tap = -1;
+                    if (adjoint && g.grad) {
+                        tap = int(phase_taps.size());
+                        phase_taps.push_back(tap_id_of_slot[size_t(g.slot)]);
+                    }
+                    for (auto& t : terms) {
+                        t.gate = gi;
+                        t.tap = tap;
+                        pending.push_back(t);
+                    }
+                    pending_q |= q;
+                    consumed++;
                     continue;
                 }
                 std::vector<cplx> m;
@@ -132,14 +360,16 @@ struct PassBuilder {
                 // a parameterised dense op opens a register group, so that its targets take register
                 // positions 0 (and 1): the interpreted pass then needs one instantiation per shape
                 if (opk_is_custom(kind) && !grp.ops.empty()) {
-                    blocked |= q;
+                    bq |= q;
+                    bt |= ndq;
                     keep.push_back(gi);
                     continue;
                 }
                 int reg_ctrl = -1;  // control qubit bound to a register position (CX / CZ)
                 if (kind == OPK_M3 || kind == OPK_M4) {
                     if (!grp.ops.empty()) {  // dense multi-qubit blocks take a group of their own
-                        blocked |= q;
+                        bq |= q;
+                        bt |= ndq;
                         keep.push_back(gi);
                         continue;
                     }
@@ -163,7 +393,8 @@ struct PassBuilder {
                             merged.push_back(pos_of[want_ctrl]);
                     }
                     if (int(merged.size()) > W && !grp.ops.empty()) {
-                        blocked |= q;
+                        bq |= q;
+                        bt |= ndq;
                         keep.push_back(gi);
                         continue;
                     }
@@ -174,6 +405,9 @@ struct PassBuilder {
                     }
                     gpos = merged;
                 }
+                // dense 3- / 4-qubit blocks read their inputs from the tile, so a block before one
+                // goes to the end of the previous group
+                if (after_block) flush((kind == OPK_M3 || kind == OPK_M4) ? nullptr : &grp);
                 HostOp op;
                 op.gate = gi;
                 DevOp& d = op.dev;
@@ -216,15 +450,24 @@ struct PassBuilder {
                 for (int c = 0; c < 64; c++)
                     if ((rest >> c) & 1) d.ctrl_local |= uint32_t(1) << pos_of[c];
                 grp.ops.push_back(op);
+                consumed++;
             }
-            // pad the group to 4 distinct positions
-            int K = int(pass->local_qubits.size());
-            for (int p = 0; int(gpos.size()) < kGroupWidth && p < K; p++)
-                if (std::find(gpos.begin(), gpos.end(), p) == gpos.end()) gpos.push_back(p);
-            for (int i = 0; i < kGroupWidth; i++) grp.pos[i] = gpos[i];
-            pass->groups.push_back(std::move(grp));
+            if (!grp.ops.empty()) {  // pad the group to 4 distinct positions
+                pad_positions(grp, gpos, K);
+                pass->groups.push_back(std::move(grp));
+            }
+            if (consumed == 0) flush(nullptr);  // nothing could proceed past the pending terms
             todo = keep;
         }
+        flush(nullptr);
+        if (phase_ops) merge_blocks(pass);
+        // the phase-block taps follow the register groups' taps of the pass
+        const int off = int(pass->tap_slots.size());
+        for (auto& grp : pass->groups)
+            for (auto& op : grp.ops)
+                for (auto& t : op.terms)
+                    if (t.tap >= 0) t.tap += off;
+        pass->tap_slots.insert(pass->tap_slots.end(), phase_taps.begin(), phase_taps.end());
     }
 };
 
@@ -259,7 +502,17 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     if (adjoint) std::reverse(remaining.begin(), remaining.end());
 
     PassBuilder pb{gates, adjoint, W, std::min(std::max(opts.taps_per_group, 1), kMaxTapsPerGroup), opts.phase_free,
-                   {}};
+                   opts.phase_ops, {}, {}, {}};
+    pb.phase_ok.assign(gates.size(), 0);
+    pb.nd.resize(gates.size());
+    {
+        std::vector<HostPhaseTerm> scratch;
+        for (size_t i = 0; i < gates.size(); i++) {
+            // without phase blocks, any shared qubit orders two gates (the original packing)
+            pb.nd[i] = opts.phase_ops ? nondiag_targets(gates[i]) : gates[i].qubits();
+            pb.phase_ok[i] = (opts.phase_ops && phase_terms(gates[i], opts.phase_free, &scratch)) ? 1 : 0;
+        }
+    }
     int n_slots = 0;
     for (auto& g : gates) n_slots = std::max(n_slots, g.slot + 1);
     pb.tap_id_of_slot.assign(n_slots, -1);
@@ -268,6 +521,7 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
     const uint64_t lowmask = (C >= 64) ? all : ((uint64_t(1) << C) - 1);
     // Greedy partition into passes: each pass takes, in order, every gate whose targets still fit
     // its local set and that no deferred gate blocks; cap > 0 closes a pass after cap gates.
+    // Phase-block gates need no local qubit.
     struct Part {
         std::vector<int> chosen;
         uint64_t L;
@@ -277,26 +531,29 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
         std::vector<int> todo = remaining;
         while (!todo.empty()) {
             uint64_t L = whole ? all : lowmask;
-            uint64_t blocked = 0;
+            uint64_t bq = 0, bt = 0;  // qubits of the deferred gates, and their non-diagonal qubits
             std::vector<int> chosen, rest;
             int taps = 0;
             for (int gi : todo) {
                 const CGate& g = gates[gi];
                 uint64_t q = g.qubits();
+                const uint64_t ndq = pb.nd[size_t(gi)];
                 bool needs_tap = adjoint && g.grad && g.slot >= 0;
                 bool full = (cap > 0 && int(chosen.size()) >= cap) || (needs_tap && taps >= kMaxTapsPerPass);
-                if (full || (q & blocked)) {
-                    blocked |= q;
+                if (full || (ndq & bq) || (q & bt)) {
+                    bq |= q;
+                    bt |= ndq;
                     rest.push_back(gi);
                     continue;
                 }
-                uint64_t nl = L | g.targets();
+                uint64_t nl = pb.phase_ok[size_t(gi)] ? L : (L | g.targets());
                 if (popc(nl) <= K) {
                     L = nl;
                     chosen.push_back(gi);
                     taps += needs_tap ? 1 : 0;
                 } else {
-                    blocked |= q;
+                    bq |= q;
+                    bt |= ndq;
                     rest.push_back(gi);
                 }
             }
@@ -307,17 +564,17 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
         return parts;
     };
     // Model of a pass: max(1, work / B) streams of the state, work = parameterised ops + 0.3 per
-    // other op (register renames and fixed matrices), B = the work at which a pass's arithmetic
-    // takes as long as its streaming (measured on MI355X: about 32 for a complex64 back-sweep,
-    // 48 for a forward sweep; twice that for complex128, which streams twice the bytes).  Greedy
-    // packing can leave one arithmetic-bound pass and stragglers; without an explicit cap, the
-    // partition of lowest modelled time among a few caps is kept.
+    // other op (register renames, fixed matrices and phase-block terms), B = the work at which a
+    // pass's arithmetic takes as long as its streaming (measured on MI355X: about 32 for a
+    // complex64 back-sweep, 48 for a forward sweep; twice that for complex128, which streams twice
+    // the bytes).  Greedy packing can leave one arithmetic-bound pass and stragglers; without an
+    // explicit cap, the partition of lowest modelled time among a few caps is kept.
     auto cost = [&](const std::vector<Part>& parts) {
         const double B = (adjoint ? 32.0 : 48.0) * (opts.amp_bytes > 8 ? 2.0 : 1.0);
         double t = 0;
         for (const auto& p : parts) {
             double w = 0;
-            for (int gi : p.chosen) w += gates[gi].slot >= 0 ? 1.0 : 0.3;
+            for (int gi : p.chosen) w += (gates[gi].slot >= 0 && !pb.phase_ok[size_t(gi)]) ? 1.0 : 0.3;
             t += std::max(1.0, w / B);
         }
         return t;
@@ -378,7 +635,27 @@ Plan build_plan(const std::vector<CGate>& gates, int n_eff, bool adjoint, const 
                     if ((v >> i) & 1) u |= uint32_t(1) << grp.pos[i];
                 dg.off[v] = uint16_t(lds_swizzle(u, opts.amp_bytes));
             }
-            for (auto& op : grp.ops) plan.dev_ops.push_back(op.dev);
+            for (auto& op : grp.ops) {
+                DevOp d = op.dev;
+                if (d.kind == OPK_PHASE) {  // terms: tile positions and the qubits outside the tile
+                    d.mat = int(plan.dev_terms.size());
+                    d.slot = int(op.terms.size());
+                    for (const HostPhaseTerm& t : op.terms) {
+                        DevPhaseTerm dt{};
+                        dt.mg = t.mask & dp.nonlocal_mask;
+                        dt.ml = 0;
+                        for (size_t p = 0; p < pass.local_qubits.size(); p++)
+                            if ((t.mask >> pass.local_qubits[p]) & 1) dt.ml |= uint32_t(1) << p;
+                        dt.kind = t.kind;
+                        dt.coef = int(plan.phase_coefs.size());
+                        dt.tap = t.tap;
+                        dt.weight = t.weight;
+                        plan.phase_coefs.push_back(t.coef);
+                        plan.dev_terms.push_back(dt);
+                    }
+                }
+                plan.dev_ops.push_back(d);
+            }
             plan.dev_groups.push_back(dg);
         }
         plan.dev_passes.push_back(dp);

@@ -34,16 +34,30 @@ struct SchedOptions {
     int taps_per_group = 8;    // derivative taps a register group may hold (<= kMaxTapsPerGroup)
     int amp_bytes = 8;         // 8 (complex64) or 16 (complex128): selects the LDS swizzle
     bool phase_free = false;   // uncontrolled RX / RY / RZ / GP lower to their OPK_PF_* forms
+    // RZ, PS and uncontrolled RZZ become terms of phase blocks (OPK_PHASE): they need no local
+    // qubit, and a run of them that commutes with the gates around it costs one block
+    bool phase_ops = false;
     bool operator==(const SchedOptions& o) const {
         return tile_qubits == o.tile_qubits && low_run == o.low_run && group_width == o.group_width &&
                max_ops_per_pass == o.max_ops_per_pass && taps_per_group == o.taps_per_group &&
-               amp_bytes == o.amp_bytes && phase_free == o.phase_free;
+               amp_bytes == o.amp_bytes && phase_free == o.phase_free && phase_ops == o.phase_ops;
     }
 };
 
+// A term of a phase block before flattening: the gate it comes from, its indicator over global
+// qubits, its phase (PhaseCoef) and its tap (index among the pass's phase-block taps, -1: none).
+struct HostPhaseTerm {
+    int gate = -1;
+    int kind = PHT_AND;
+    uint64_t mask = 0;
+    PhaseCoef coef{};
+    int tap = -1;
+    double weight = 0;
+};
 struct HostOp {
-    int gate = -1;  // index into the circuit
+    int gate = -1;  // index into the circuit (a phase block: its first gate)
     DevOp dev{};
+    std::vector<HostPhaseTerm> terms;  // OPK_PHASE: the block's terms
 };
 struct HostGroup {
     int pos[4] = {0, 1, 2, 3};
@@ -69,10 +83,22 @@ struct Plan {
     std::vector<DevGroup> dev_groups;
     std::vector<DevOp> dev_ops;
     std::vector<int32_t> dev_tap_slots;
+    std::vector<DevPhaseTerm> dev_terms;  // terms of the phase blocks (OPK_PHASE)
+    std::vector<PhaseCoef> phase_coefs;   // phase of term i (column i of the per-sample phase table)
     std::vector<double> pool;  // matrix pool (interleaved complex)
     int max_taps = 0;
     int n_tap_ids = 0;  // gates with a derivative tap; tap ids follow circuit order
 };
+
+// Every circuit gate an op of a plan applies (a phase block applies several).
+inline void op_gates(const HostOp& op, std::vector<int>& out) {
+    if (op.dev.kind != OPK_PHASE) {
+        out.push_back(op.gate);
+        return;
+    }
+    for (const HostPhaseTerm& t : op.terms)
+        if (out.empty() || out.back() != t.gate) out.push_back(t.gate);
+}
 
 // Build the schedule of `gates` (adjoint: the reverse order, each op applying G^dagger).
 // n_eff: qubits of the state.  Throws std::invalid_argument on unsupported gates.

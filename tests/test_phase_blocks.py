@@ -1,0 +1,148 @@
+"""Phase blocks of the scheduler (CPU only): with option "phase_ops", RZ, PS and uncontrolled RZZ
+gates join diagonal blocks that need no local qubit and may move past every gate they commute with.
+
+The schedule (mqa_circuit_schedule, mode + 16) must hold every gate exactly once, keep the other
+gates' targets local, and be a valid reordering: the dense simulation of the gates in schedule
+order (forward) or in reversed schedule order (back-sweep) gives the circuit's state.  The code
+generator must compile every phase-block form.  Parity on the GPU: test_gpu_phase_blocks.py.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mindquantum_amd as mq
+from mindquantum_amd import _native, ir
+from mindquantum_amd.algorithm.builders import encoder_rx, hardware_efficient_ansatz, max_cut_ansatz, qaoa_graph
+from mindquantum_amd.core import gates as G
+from oracle import dense
+from tests.conftest import hea, params_for, random_circuit
+
+PHASE_OPS = 16
+
+
+def diag_kinds():
+    return (G.RZ, G.PhaseShift)
+
+
+def in_block(g):
+    """Gates that join phase blocks."""
+    if isinstance(g, diag_kinds()):
+        return True
+    return isinstance(g, G.Rzz) and not g.ctrl_qubits
+
+
+def phase_circuit(n, depth, seed):
+    """Random circuit rich in RZ / PS / RZZ (controlled and not) among the other kinds."""
+    kinds = ["RZ", "RZ", "PS", "PS", "Rzz", "Rzz", "RX", "RY", "H", "X", "CNOT", "Z", "S", "T", "SWAP", "MAT1",
+             "GP", "Rxx"]
+    return random_circuit(n, depth, seed, kinds=kinds, ctrl_prob=0.3)
+
+
+def check_phase_schedule(circ, sched, n_eff, K, adjoint):
+    gates = [i for i, g in enumerate(circ) if not (hasattr(g, "ir_kind") and g.ir_kind == "I")]
+    order = [gi for _, ops in sched for gi in ops]
+    assert sorted(order) == sorted(gates)
+    for mask, ops in sched:
+        assert bin(mask).count("1") == min(K, n_eff)
+        for gi in ops:
+            if in_block(circ[gi]):
+                continue
+            for q in circ[gi].obj_qubits:
+                assert (mask >> q) & 1, "target outside the pass's local qubits"
+    return order
+
+
+@pytest.mark.parametrize("n,depth,seed", [(8, 150, 1), (8, 200, 2), (9, 200, 3)])
+@pytest.mark.parametrize("pf", [0, 2])
+def test_phase_schedule_is_a_valid_reordering(n, depth, seed, pf):
+    _native.set_option("tile_qubits_c64", 6)
+    _native.set_option("adj_tile_qubits_c64", 6)
+    _native.set_option("lowq_c64", 1)
+    try:
+        circ = phase_circuit(n, depth, seed)
+        names, vals = params_for(circ, seed)
+        values = dict(zip(names, vals))
+        nc = ir.NativeCircuit(circ, names)
+        fwd = nc.schedule(n, 0, PHASE_OPS | pf)
+        adj = nc.schedule(n, 0, PHASE_OPS | pf | 1)
+    finally:
+        for k in ("tile_qubits_c64", "adj_tile_qubits_c64", "lowq_c64"):
+            _native.set_option(k, -1)
+    ref = dense.state(circ, values, n)
+    for sched, adjoint in ((fwd, False), (adj, True)):
+        order = check_phase_schedule(circ, sched, n, 6, adjoint)
+        if adjoint:
+            order = order[::-1]
+        got = dense.state(mq.Circuit([circ[i] for i in order]), values, n)
+        assert np.allclose(got, ref, atol=1e-12)
+
+
+def test_phase_blocks_leave_other_schedules_alone():
+    """Without phase_ops the schedule is the per-gate one (mode bit 4 only adds blocks)."""
+    circ = phase_circuit(10, 120, 7)
+    nc = ir.NativeCircuit(circ, list(circ.params_name))
+    assert nc.schedule(10, 0, 0) == ir.NativeCircuit(circ, list(circ.params_name)).schedule(10, 0, 0)
+    with_blocks = nc.schedule(10, 0, PHASE_OPS)
+    assert sorted(gi for _, ops in with_blocks for gi in ops) == sorted(gi for _, ops in nc.schedule(10, 0, 0)
+                                                                       for gi in ops)
+
+
+def nonlocal_block_gates(circ, sched):
+    """Phase-block gates scheduled in a pass where one of their qubits is not local."""
+    return [gi for mask, ops in sched for gi in ops if in_block(circ[gi]) and
+            any(not (mask >> q) & 1 for q in circ[gi].obj_qubits + circ[gi].ctrl_qubits)]
+
+
+def test_hea_rz_layers_run_outside_the_local_qubits():
+    """The 20-qubit encoder + HEA-8 gradient circuit: RZ gates join blocks in passes where their
+    qubit indexes the tiles (a per-tile phase), in both sweeps."""
+    circ = encoder_rx(20) + hardware_efficient_ansatz(20, 8)
+    names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
+    nc = ir.NativeCircuit(circ, names)
+    for adjoint in (0, 1):
+        blocks = nc.schedule(20, 0, adjoint | 2 | PHASE_OPS)
+        check_phase_schedule(circ, blocks, 20, 12, bool(adjoint))
+        assert nonlocal_block_gates(circ, blocks)
+
+
+def test_qaoa_rzz_layers_run_outside_the_local_qubits():
+    graph = qaoa_graph(16, 5)
+    circ = max_cut_ansatz(graph, 3)
+    nc = ir.NativeCircuit(circ, list(circ.params_name))
+    _native.set_option("tile_qubits_c64", 10)
+    try:
+        blocks = nc.schedule(16, 0, 2 | PHASE_OPS)
+    finally:
+        _native.set_option("tile_qubits_c64", -1)
+    check_phase_schedule(circ, blocks, 16, 10, False)
+    assert nonlocal_block_gates(circ, blocks)
+
+
+def _compile(circ, n, dtype_code, mode):
+    names = list(circ.encoder_params_name) + list(circ.ansatz_params_name)
+    nc = ir.NativeCircuit(circ, names)
+    sec = ctypes.c_double(0)
+    _native.check(_native.lib().mqa_circuit_jit_compile(nc.handle, n, dtype_code, mode, ctypes.byref(sec)))
+
+
+@pytest.mark.parametrize("dtype_code", [_native.MQA_C64, _native.MQA_C128])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 11])
+def test_phase_block_passes_compile(dtype_code, mode):
+    """Every phase-block form compiles: terms on register, thread and tile bits, AND and parity
+    terms, controlled RZ / PS, exact (global-phase) terms and the single-buffer back-sweep."""
+    if mode == 11 and dtype_code != _native.MQA_C64:
+        pytest.skip("single-buffer passes are a complex64 back-sweep form")
+    n = 14
+    circ = phase_circuit(n, 160, 11)
+    circ += G.RZ("c").on(2, [7, 13])
+    circ += G.PhaseShift("d").on(12, [0, 1])
+    circ += G.Rzz("e").on([0, 13])
+    circ += G.Rzz("f").on([1, 2])
+    _compile(circ, n, dtype_code, mode | PHASE_OPS)
+
+
+def test_hea_phase_block_passes_compile():
+    circ = hea(16, 3, encoder=True)
+    _compile(circ, 16, _native.MQA_C64, 3 | 8 | PHASE_OPS)
+    _compile(circ, 16, _native.MQA_C64, 2 | PHASE_OPS)
