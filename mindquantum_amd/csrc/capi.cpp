@@ -170,6 +170,7 @@ struct DevPlan {
     DevBuf terms, phase_coefs;  // phase blocks: DevPhaseTerm and PhaseCoef arrays
     int n_coef_forms = 0;
     bool full = false;
+    bool phase = false;  // the plan has phase blocks (the interpreted pass's PHASE instantiation)
     std::mutex jit_mu;
     std::unique_ptr<JitPlan> jit;     // compiled passes (created on first large launch)
     std::unique_ptr<JitPlan> jit_sb;  // the same with one LDS tile buffer (option "jit_single_buffer")
@@ -291,12 +292,14 @@ struct mqa_circuit {
     static std::unique_ptr<mqa::DevPlan> make_devplan(mqa::Plan&& plan, int dtype, hipStream_t s, bool upload_dev) {
         auto dp = std::make_unique<mqa::DevPlan>();
         dp->plan = std::move(plan);
-        // dense blocks, controls that stay per-amplitude predicates and phase blocks need the FULL
-        // instantiation
-        for (auto& op : dp->plan.dev_ops)
+        // dense blocks or controls that stay per-amplitude predicates need the FULL instantiation,
+        // phase blocks the PHASE one
+        for (auto& op : dp->plan.dev_ops) {
             if (op.kind == mqa::OPK_M2 || op.kind == mqa::OPK_M3 || op.kind == mqa::OPK_M4 || op.ctrl_local != 0 ||
-                mqa::opk_is_custom(op.kind) || op.kind == mqa::OPK_PHASE)
+                mqa::opk_is_custom(op.kind))
                 dp->full = true;
+            if (op.kind == mqa::OPK_PHASE) dp->phase = true;
+        }
         if (upload_dev) {
             mqa::upload(dp->passes, dp->plan.dev_passes, s);
             mqa::upload(dp->groups, dp->plan.dev_groups, s);
@@ -632,7 +635,7 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
             if (jp) {
                 hip_check(jp->launch(i, a, batch, lds, sim->stream), "compiled tile pass");
             } else {
-                hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, a, batch, pl.C,
+                hip_check(launch_tile_pass(sim->dtype, pl.adjoint, dp->full, dp->phase, a, batch, pl.C,
                                            std::max(pl.max_taps, 1), sim->stream),
                           "tile pass");
             }
@@ -1893,8 +1896,27 @@ struct GradItem {
     uint64_t ang = 0;                 // offset (doubles) in the angle table
     uint64_t taps_part = 0, taps = 0; // offsets in the tap partials / reduced taps
     uint64_t fpart = 0, f_len = 0;    // offset (doubles) and (re, im) pairs of the <psi|H psi> partials
+    uint64_t ph_fwd = 0, ph_adj = 0;  // offsets (reals) of its plans' phase tables
     size_t bytes = 0;                 // device bytes this item needs
 };
+
+// Row of a plan's phase table for one sample from its angle table (k_phase_coefs on the host).
+template <typename T>
+void phase_row(const Plan& pl, const double* ang, T* out) {
+    for (size_t t = 0; t < pl.phase_coefs.size(); t++) {
+        const PhaseCoef& c = pl.phase_coefs[t];
+        double x = 1.0, y = 0.0;
+        if (c.slot >= 0) {
+            const double ca = ang[2 * c.slot], sa = ang[2 * c.slot + 1];
+            const int k = c.k < 0 ? -c.k : c.k;
+            if (k == 1) x = ca, y = sa;
+            else if (k == 2) x = ca * ca - sa * sa, y = 2.0 * ca * sa;
+            if (c.k < 0) y = -y;
+        }
+        out[2 * t] = T(x * c.cos_b - y * c.sin_b);
+        out[2 * t + 1] = T(x * c.sin_b + y * c.cos_b);
+    }
+}
 
 struct StreamOwner {
     hipStream_t s = nullptr;
@@ -1908,7 +1930,7 @@ struct StreamOwner {
 template <typename A>
 struct StepGroup {
     int K = 0, C = 0, max_taps = 0;
-    bool full = false, first = false, last = false;
+    bool full = false, phase = false, first = false, last = false;
     std::vector<A> args;
     std::vector<uint64_t> prefix;
     uint64_t blocks = 0;
@@ -1958,6 +1980,34 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
         }
     }
     upload(ang_d, ang, s);
+    // phase tables of the items' plans (phase blocks), in the state's real type
+    DevBuf ph_d;
+    {
+        uint64_t n_ph = 0;
+        for (auto& it : items) {
+            it.ph_fwd = n_ph;
+            n_ph += 2 * it.fwd->plan.phase_coefs.size();
+            it.ph_adj = n_ph;
+            n_ph += 2 * it.adj->plan.phase_coefs.size();
+        }
+        std::vector<float> pf32;
+        std::vector<double> pf64;
+        if (dtype == MQA_C64) pf32.assign(std::max<uint64_t>(n_ph, 1), 0.0f);
+        else pf64.assign(std::max<uint64_t>(n_ph, 1), 0.0);
+        for (auto& it : items) {
+            const double* a = ang.data() + it.ang;
+            if (dtype == MQA_C64) {
+                phase_row(it.fwd->plan, a, pf32.data() + it.ph_fwd);
+                phase_row(it.adj->plan, a, pf32.data() + it.ph_adj);
+            } else {
+                phase_row(it.fwd->plan, a, pf64.data() + it.ph_fwd);
+                phase_row(it.adj->plan, a, pf64.data() + it.ph_adj);
+            }
+        }
+        if (dtype == MQA_C64) upload(ph_d, pf32, s);
+        else upload(ph_d, pf64, s);
+        hip_check(hipStreamSynchronize(s), "phase tables");  // the host tables go out of scope
+    }
 
     // |0...0> for every item
     std::vector<uint64_t> unit;
@@ -1985,6 +2035,11 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
         a.n_tap_ids = circs[it.index]->n_tap_ids;
         a.tap_ids = dp->tap_ids.as<int32_t>();
         a.partials = adjoint ? part_d.as<double>() + it.taps_part : nullptr;
+        if (!dp->plan.dev_terms.empty()) {
+            a.terms = dp->terms.as<DevPhaseTerm>();
+            a.phase = static_cast<char*>(ph_d.p) + (adjoint ? it.ph_adj : it.ph_fwd) * (amp / 2);
+            a.n_phase = int32_t(dp->plan.dev_terms.size());
+        }
         return a;
     };
 
@@ -1995,15 +2050,16 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
         size_t max_p = 0;
         for (auto& it : items) max_p = std::max(max_p, (adjoint ? it.adj : it.fwd)->plan.passes.size());
         for (size_t st = 0; st < max_p; st++) {
-            std::map<std::tuple<int, int, int>, StepGroup<TilePassArgs>> by;
+            std::map<std::tuple<int, int, int, int>, StepGroup<TilePassArgs>> by;
             for (auto& it : items) {
                 DevPlan* dp = adjoint ? it.adj : it.fwd;
                 if (st >= dp->plan.passes.size()) continue;
                 if (adjoint && circs[it.index]->n_tap_ids == 0) continue;  // nothing to differentiate
-                auto& g = by[std::make_tuple(dp->plan.K, dp->plan.C, int(dp->full))];
+                auto& g = by[std::make_tuple(dp->plan.K, dp->plan.C, int(dp->full), int(dp->phase))];
                 g.K = dp->plan.K;
                 g.C = dp->plan.C;
                 g.full = dp->full;
+                g.phase = dp->phase;
                 g.max_taps = std::max(g.max_taps, std::max(dp->plan.max_taps, 1));
                 g.prefix.push_back(g.blocks);
                 g.args.push_back(tile_args(it, dp, int(st), adjoint));
@@ -2104,7 +2160,7 @@ void run_grad_chunk(std::vector<GradItem>& items, const mqa_circuit* const* circ
                 double bytes = 0;
                 for (const auto& a : g.args) bytes += double(a.stride) * double(amp) * (adjoint ? 4.0 : 2.0);
                 prof(adjoint ? "items_tile_pass_adj" : "items_tile_pass_fwd", bytes, [&] {
-                    hip_check(launch_tile_pass_items(dtype, adjoint, g.full, targs_d + g.args_at,
+                    hip_check(launch_tile_pass_items(dtype, adjoint, g.full, g.phase, targs_d + g.args_at,
                                                      prefix_d + g.prefix_at, int(g.args.size()), g.blocks, g.K, lds, s),
                               "tile pass (items)");
                 });
@@ -2219,8 +2275,10 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             it.dim = uint64_t(1) << it.n_eff;
             // Pauli observables (the only kind items take) are hermitian: phase-free plans
             const bool pf = g_opt.phase_free.load() != 0;
-            it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true, pf);
-            it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true, pf);
+            // the item passes always run interpreted: phase blocks unless the option is off
+            const bool ops = g_opt.phase_ops != 0;
+            it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true, pf, ops);
+            it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true, pf, ops);
             if (g_opt.ham_engine) {
                 auto* tp = ham_plan_for(const_cast<mqa_ham*>(h), it.n_eff, dtype, so.s);
                 it.hp = tp->ok ? tp : nullptr;

@@ -1099,51 +1099,107 @@ __device__ __forceinline__ void cmul_if(T& ar, T& ai, T br, T bi, bool on) {
     ai = on ? i : ai;
 }
 
-// Interpreted phase block: its terms one after another, each over the 16 registers (the generated
-// passes factor the block into per-thread and per-register products instead).  z: the sample's
-// row of the phase table.  Back-sweep: the conjugate phases, and each tap as one reduction of its
-// terms' weighted sums of Im(conj(lambda) psi), a quantity the block's phases leave unchanged.
+// Interpreted phase block (terms read at run time, uniform over the block): as in the generated
+// passes, a term without register bits only updates a per-thread factor E and a term on one
+// register bit j a factor F_j; register v's phase, E times the F_j of its bits, is applied once at
+// the end.  Terms on two or more register bits are applied to the registers directly.  Back-sweep:
+// conjugate phases; each term's tap value from the register sums of m_v = Im(conj(lambda_v) psi_v)
+// (over all registers and per bit), which the block's phases leave unchanged; a gate's terms are
+// consecutive and share its tap.
 template <typename T, bool ADJ>
 __device__ __forceinline__ void phase_block(const DevOp& op, Regs<T>& p, Regs<T>& l, const GroupCtx& g,
                                             const DevPhaseTerm* __restrict__ terms, const T* __restrict__ z,
                                             const TileFrame<T>& f) {
+    const uint32_t posm = g.pm[0] | g.pm[1] | g.pm[2] | g.pm[3];
+    T er = T(1), ei = T(0);
+    T f0r = T(1), f0i = T(0), f1r = T(1), f1i = T(0), f2r = T(1), f2i = T(0), f3r = T(1), f3i = T(0);
+    T S = T(0), M0 = T(0), M1 = T(0), M2 = T(0), M3 = T(0);  // register sums of m_v (back-sweep taps)
+    if constexpr (ADJ) {
+#pragma unroll
+        for (int v = 0; v < kAmpsPerThread; v++) {
+            const T m = l.r[v] * p.i[v] - l.i[v] * p.r[v];
+            S += m;
+            if (v & 1) M0 += m;
+            if (v & 2) M1 += m;
+            if (v & 4) M2 += m;
+            if (v & 8) M3 += m;
+        }
+    }
+    // F_j *= (ur, ui) for the block-uniform bit j, written on the named factors (a helper taking them
+    // by reference lets the compiler pick one through a pointer, which puts them in scratch memory)
+#define MQA_FMUL(J, UR, UI)                                                                      \
+    switch (J) {                                                                                 \
+        case 0: { const T r_ = f0r * (UR) - f0i * (UI); f0i = f0r * (UI) + f0i * (UR); f0r = r_; } break; \
+        case 1: { const T r_ = f1r * (UR) - f1i * (UI); f1i = f1r * (UI) + f1i * (UR); f1r = r_; } break; \
+        case 2: { const T r_ = f2r * (UR) - f2i * (UI); f2i = f2r * (UI) + f2i * (UR); f2r = r_; } break; \
+        default: { const T r_ = f3r * (UR) - f3i * (UI); f3i = f3r * (UI) + f3i * (UR); f3r = r_; } break; \
+    }
     T acc = T(0);
     int cur = -1;
     for (int k = 0; k < op.slot; k++) {
         const DevPhaseTerm t = terms[op.mat + k];
         const T zr = z[2 * t.coef];
         const T zi = ADJ ? -z[2 * t.coef + 1] : z[2 * t.coef + 1];
+        const uint32_t R = ((t.ml & g.pm[0]) ? 1u : 0u) | ((t.ml & g.pm[1]) ? 2u : 0u) | ((t.ml & g.pm[2]) ? 4u : 0u) |
+                           ((t.ml & g.pm[3]) ? 8u : 0u);
+        const uint32_t H = t.ml & ~posm;
+        const int nr = __popc(R);
+        const int j0 = __ffs(int(R)) - 1;
         const uint64_t gm = f.base & t.mg;
-        const bool tile_on = gm == t.mg;
-        const uint32_t tile_par = uint32_t(__popcll(gm)) & 1u;
         const bool is_and = t.kind == PHT_AND;
-        T s = T(0);
-#pragma unroll
-        for (int v = 0; v < kAmpsPerThread; v++) {
-            const uint32_t u = (g.ub | ((v & 1) ? g.pm[0] : 0u) | ((v & 2) ? g.pm[1] : 0u) | ((v & 4) ? g.pm[2] : 0u) |
-                                ((v & 8) ? g.pm[3] : 0u)) &
-                               t.ml;
-            const bool on = is_and ? (tile_on && u == t.ml) : (((uint32_t(__popc(u)) + tile_par) & 1u) != 0u);
-            if constexpr (ADJ) {
-                const T m = l.r[v] * p.i[v] - l.i[v] * p.r[v];
-                s += on ? m : T(0);
+        const bool on = is_and ? (((g.ub & H) == H) && gm == t.mg)
+                               : (((uint32_t(__popc(g.ub & H)) + uint32_t(__popcll(gm))) & 1u) != 0u);
+        T val = T(0);  // this thread's tap value of the term
+        if (nr == 0) {
+            cmul_if(er, ei, zr, zi, on);
+            val = on ? S : T(0);
+        } else if (nr == 1) {
+            const T Mj = j0 == 0 ? M0 : j0 == 1 ? M1 : j0 == 2 ? M2 : M3;
+            if (is_and) {  // z on v_j = 1 where the term's other bits hold
+                const T ur = on ? zr : T(1), ui = on ? zi : T(0);
+                MQA_FMUL(j0, ur, ui)
+                val = on ? Mj : T(0);
+            } else {  // z^{v_j ^ s}: z on E and conj(z) on F_j when s = 1
+                cmul_if(er, ei, zr, zi, on);
+                const T ui = on ? -zi : zi;
+                MQA_FMUL(j0, zr, ui)
+                val = on ? S - Mj : Mj;
             }
-            cmul_if(p.r[v], p.i[v], zr, zi, on);
-            if constexpr (ADJ) cmul_if(l.r[v], l.i[v], zr, zi, on);
+        } else {
+#pragma unroll
+            for (int v = 0; v < kAmpsPerThread; v++) {
+                const uint32_t rv = uint32_t(v) & R;
+                const bool hit = is_and ? (on && rv == R) : (((uint32_t(__popc(rv)) + (on ? 1u : 0u)) & 1u) != 0u);
+                if constexpr (ADJ) val += hit ? (l.r[v] * p.i[v] - l.i[v] * p.r[v]) : T(0);
+                cmul_if(p.r[v], p.i[v], zr, zi, hit);
+                if constexpr (ADJ) cmul_if(l.r[v], l.i[v], zr, zi, hit);
+            }
         }
         if constexpr (ADJ) {
-            if (t.tap >= 0) {  // a gate's terms are consecutive and share its tap
+            if (t.tap >= 0) {
                 if (t.tap != cur && cur >= 0) {
                     tap_to_lds(f, acc, cur);
                     acc = T(0);
                 }
                 cur = t.tap;
-                acc += T(t.weight) * s;
+                acc += T(t.weight) * val;
             }
         }
     }
+#undef MQA_FMUL
     if constexpr (ADJ) {
         if (cur >= 0) tap_to_lds(f, acc, cur);
+    }
+    // register v: E times the F_j of its bits
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) {
+        T qr = er, qi = ei;
+        if (v & 1) cmul(qr, qi, f0r, f0i);
+        if (v & 2) cmul(qr, qi, f1r, f1i);
+        if (v & 4) cmul(qr, qi, f2r, f2i);
+        if (v & 8) cmul(qr, qi, f3r, f3i);
+        cmul(p.r[v], p.i[v], qr, qi);
+        if constexpr (ADJ) cmul(l.r[v], l.i[v], qr, qi);
     }
 }
 

@@ -31,7 +31,7 @@ namespace mqa {
 // Back-sweep without dense blocks: psi and lambda stay in registers between groups and share one
 // LDS tile buffer (the single-buffer form of the generated passes, device_ops.h), so twice as many
 // blocks fit a CU.
-template <typename T>
+template <typename T, bool PHASE>
 __device__ __forceinline__ void tile_pass_adj_one_buffer(const TilePassArgs& args, const DevPass& pass,
                                                          unsigned char* smem, uint64_t tile, uint64_t sample) {
     TileFrame<T> f = sb_begin<T, 0>(args, pass, smem, tile, sample);
@@ -67,7 +67,11 @@ __device__ __forceinline__ void tile_pass_adj_one_buffer(const TilePassArgs& arg
         for (int j = 0; j < kTapSlots<T>; j++) acc.r[j] = T(0);
         for (int oi = 0; oi < grp.n_ops; oi++) {
             const DevOp op = args.ops[grp.op_begin + oi];
-            run_op<T, true, false>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc);
+            if (PHASE && op.kind == OPK_PHASE)
+                phase_block<T, true>(op, p, l, g, args.terms,
+                                     static_cast<const T*>(args.phase) + 2 * f.sample * uint64_t(args.n_phase), f);
+            else
+                run_op<T, true, false>(op, p, l, g, f.base, static_cast<const T*>(args.pool), ang_row, acc);
         }
         if (grp.n_taps > 0) {  // one batched cross-lane reduction per group
             if (f.wave_width == 64) {
@@ -107,12 +111,14 @@ __device__ __forceinline__ void tile_pass_adj_one_buffer(const TilePassArgs& arg
     taps_out<T>(args, pass, f);
 }
 
-template <typename T, bool ADJ, bool FULL>
+// PHASE: the plan has phase blocks (an instantiation of its own, so that plans without them keep
+// the register allocation of the plain kernels).
+template <typename T, bool ADJ, bool FULL, bool PHASE>
 __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigned char* smem, uint64_t tile,
                                                uint64_t sample) {
     const DevPass pass = args.passes[args.pass_index];
     if constexpr (ADJ && !FULL) {
-        tile_pass_adj_one_buffer<T>(args, pass, smem, tile, sample);
+        tile_pass_adj_one_buffer<T, PHASE>(args, pass, smem, tile, sample);
         return;
     }
     TileFrame<T> f = tile_begin<T, ADJ>(args, pass, smem, tile, sample);
@@ -148,7 +154,7 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
         }
         for (int oi = first; oi < grp.n_ops; oi++) {
             const DevOp op = args.ops[grp.op_begin + oi];
-            if (FULL && op.kind == OPK_PHASE)  // plans with phase blocks run the FULL instantiation
+            if (PHASE && op.kind == OPK_PHASE)
                 phase_block<T, ADJ>(op, p, l, g, args.terms,
                                     static_cast<const T*>(args.phase) + 2 * f.sample * uint64_t(args.n_phase), f);
             else
@@ -182,10 +188,10 @@ __device__ __forceinline__ void tile_pass_body(const TilePassArgs& args, unsigne
     tile_end<T, ADJ>(args, pass, f);
 }
 
-template <typename T, bool ADJ, bool FULL>
+template <typename T, bool ADJ, bool FULL, bool PHASE>
 __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
     extern __shared__ __align__(16) unsigned char smem[];
-    tile_pass_body<T, ADJ, FULL>(args, smem, blockIdx.x, blockIdx.y);
+    tile_pass_body<T, ADJ, FULL, PHASE>(args, smem, blockIdx.x, blockIdx.y);
 }
 
 // One pass step over several independent single-sample states (uneven circuits): block b runs
@@ -195,13 +201,13 @@ __global__ void __launch_bounds__(512) k_tile_pass(TilePassArgs args) {
 template <typename T>
 constexpr int kItemsMinWaves = sizeof(T) == 4 ? 3 : 1;
 
-template <typename T, bool ADJ, bool FULL>
+template <typename T, bool ADJ, bool FULL, bool PHASE>
 __global__ void __launch_bounds__(512, kItemsMinWaves<T>) k_tile_pass_items(const TilePassArgs* __restrict__ items,
                                                          const uint64_t* __restrict__ tile_prefix, int n_items) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int i = item_of_block(tile_prefix, n_items, blockIdx.x);
     const TilePassArgs args = items[i];
-    tile_pass_body<T, ADJ, FULL>(args, smem, uint64_t(blockIdx.x) - tile_prefix[i], 0);
+    tile_pass_body<T, ADJ, FULL, PHASE>(args, smem, uint64_t(blockIdx.x) - tile_prefix[i], 0);
 }
 
 // ---------------------------------------------------------------- Hamiltonian / reductions
@@ -690,65 +696,74 @@ size_t interpreted_pass_lds_bytes(int dtype, bool adj, bool full, int K, int C, 
     return tile_pass_lds_bytes(dtype, adj && full, K, C, max_taps);  // one tile buffer unless dense blocks
 }
 
-template <typename T, bool ADJ, bool FULL>
+template <typename T, bool ADJ, bool FULL, bool PHASE>
 static hipError_t launch_tp(const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<T, ADJ, FULL>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<T, ADJ, FULL, PHASE>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsLimit));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     dim3 grid(uint32_t(a.n_tiles), uint32_t(batch));
     dim3 block(1u << (a.K - kGroupWidth));
-    hipLaunchKernelGGL((k_tile_pass<T, ADJ, FULL>), grid, block, lds, s, a);
+    hipLaunchKernelGGL((k_tile_pass<T, ADJ, FULL, PHASE>), grid, block, lds, s, a);
     return hipGetLastError();
 }
 
-template <typename T>
-static hipError_t launch_tp_t(bool adj, bool full, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
-    if (adj) return full ? launch_tp<T, true, true>(a, batch, lds, s) : launch_tp<T, true, false>(a, batch, lds, s);
-    return full ? launch_tp<T, false, true>(a, batch, lds, s) : launch_tp<T, false, false>(a, batch, lds, s);
+template <typename T, bool ADJ>
+static hipError_t launch_tp_m(bool full, bool phase, const TilePassArgs& a, uint64_t batch, size_t lds, hipStream_t s) {
+    if (full) return phase ? launch_tp<T, ADJ, true, true>(a, batch, lds, s) : launch_tp<T, ADJ, true, false>(a, batch, lds, s);
+    return phase ? launch_tp<T, ADJ, false, true>(a, batch, lds, s) : launch_tp<T, ADJ, false, false>(a, batch, lds, s);
 }
 
-hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
+hipError_t launch_tile_pass(int dtype, bool adj, bool full, bool phase, const TilePassArgs& a, uint64_t batch, int C,
                             int max_taps, hipStream_t s) {
     size_t lds = interpreted_pass_lds_bytes(dtype, adj, full, a.K, C, max_taps);
     if (lds > kLdsLimit) return hipErrorInvalidValue;
-    if (dtype == 0) return launch_tp_t<float>(adj, full, a, batch, lds, s);
-    return launch_tp_t<double>(adj, full, a, batch, lds, s);
+    if (dtype == 0)
+        return adj ? launch_tp_m<float, true>(full, phase, a, batch, lds, s)
+                   : launch_tp_m<float, false>(full, phase, a, batch, lds, s);
+    return adj ? launch_tp_m<double, true>(full, phase, a, batch, lds, s)
+               : launch_tp_m<double, false>(full, phase, a, batch, lds, s);
 }
 
-template <typename T, bool ADJ, bool FULL>
+template <typename T, bool ADJ, bool FULL, bool PHASE>
 static hipError_t launch_tpi(const TilePassArgs* items, const uint64_t* prefix, int n_items, uint64_t blocks,
                              uint32_t threads, size_t lds, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass_items<T, ADJ, FULL>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass_items<T, ADJ, FULL, PHASE>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsLimit));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_tile_pass_items<T, ADJ, FULL>), dim3(uint32_t(blocks)), dim3(threads), lds, s, items, prefix,
-                       n_items);
+    hipLaunchKernelGGL((k_tile_pass_items<T, ADJ, FULL, PHASE>), dim3(uint32_t(blocks)), dim3(threads), lds, s, items,
+                       prefix, n_items);
     return hipGetLastError();
 }
 
-hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, const TilePassArgs* items, const uint64_t* prefix,
-                                  int n_items, uint64_t blocks, int K, size_t lds, hipStream_t s) {
+template <typename T, bool ADJ>
+static hipError_t launch_tpi_m(bool full, bool phase, const TilePassArgs* items, const uint64_t* prefix, int n_items,
+                               uint64_t blocks, uint32_t threads, size_t lds, hipStream_t s) {
+    if (full)
+        return phase ? launch_tpi<T, ADJ, true, true>(items, prefix, n_items, blocks, threads, lds, s)
+                     : launch_tpi<T, ADJ, true, false>(items, prefix, n_items, blocks, threads, lds, s);
+    return phase ? launch_tpi<T, ADJ, false, true>(items, prefix, n_items, blocks, threads, lds, s)
+                 : launch_tpi<T, ADJ, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+}
+
+hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, bool phase, const TilePassArgs* items,
+                                  const uint64_t* prefix, int n_items, uint64_t blocks, int K, size_t lds,
+                                  hipStream_t s) {
     if (lds > kLdsLimit || blocks == 0 || blocks > 0x7fffffffull || K < kMinTileQubits || K > kMaxTileQubits)
         return hipErrorInvalidValue;
     const uint32_t threads = 1u << (K - kGroupWidth);
-    if (dtype == 0) {
-        if (adj) return full ? launch_tpi<float, true, true>(items, prefix, n_items, blocks, threads, lds, s)
-                             : launch_tpi<float, true, false>(items, prefix, n_items, blocks, threads, lds, s);
-        return full ? launch_tpi<float, false, true>(items, prefix, n_items, blocks, threads, lds, s)
-                    : launch_tpi<float, false, false>(items, prefix, n_items, blocks, threads, lds, s);
-    }
-    if (adj) return full ? launch_tpi<double, true, true>(items, prefix, n_items, blocks, threads, lds, s)
-                         : launch_tpi<double, true, false>(items, prefix, n_items, blocks, threads, lds, s);
-    return full ? launch_tpi<double, false, true>(items, prefix, n_items, blocks, threads, lds, s)
-                : launch_tpi<double, false, false>(items, prefix, n_items, blocks, threads, lds, s);
+    if (dtype == 0)
+        return adj ? launch_tpi_m<float, true>(full, phase, items, prefix, n_items, blocks, threads, lds, s)
+                   : launch_tpi_m<float, false>(full, phase, items, prefix, n_items, blocks, threads, lds, s);
+    return adj ? launch_tpi_m<double, true>(full, phase, items, prefix, n_items, blocks, threads, lds, s)
+               : launch_tpi_m<double, false>(full, phase, items, prefix, n_items, blocks, threads, lds, s);
 }
 
 hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s) {

@@ -72,16 +72,18 @@ size_t tile_pass_lds_bytes(int dtype, bool adj, int K, int C, int max_taps);
 // LDS of an interpreted pass: back-sweeps without dense blocks share one tile buffer between
 // psi and lambda (two only when `full`).
 size_t interpreted_pass_lds_bytes(int dtype, bool adj, bool full, int K, int C, int max_taps);
-// full: the pass holds dense 4x4 / 8x8 / 16x16 blocks (a heavier instantiation)
-hipError_t launch_tile_pass(int dtype, bool adj, bool full, const TilePassArgs& a, uint64_t batch, int C,
+// full: the pass holds dense 4x4 / 8x8 / 16x16 blocks (a heavier instantiation); phase: phase
+// blocks (OPK_PHASE)
+hipError_t launch_tile_pass(int dtype, bool adj, bool full, bool phase, const TilePassArgs& a, uint64_t batch, int C,
                             int max_taps, hipStream_t s);
 hipError_t launch_ham(int dtype, bool write, const HamArgs& a, uint64_t batch, hipStream_t s);
 hipError_t launch_ham_tile(int dtype, bool first, bool last, const HamTileArgs& a, int C, uint64_t n_tiles,
                            uint64_t batch, hipStream_t s);
 // Several independent single-sample states in one launch: items (device array) with ascending
 // first-block offsets prefix (device array, n_items entries); blocks = total tiles.
-hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, const TilePassArgs* items, const uint64_t* prefix,
-                                  int n_items, uint64_t blocks, int K, size_t lds, hipStream_t s);
+hipError_t launch_tile_pass_items(int dtype, bool adj, bool full, bool phase, const TilePassArgs* items,
+                                  const uint64_t* prefix, int n_items, uint64_t blocks, int K, size_t lds,
+                                  hipStream_t s);
 hipError_t launch_ham_tile_items(int dtype, bool first, bool last, const HamTileArgs* items, const uint64_t* prefix,
                                  int n_items, uint64_t blocks, int K, int C, hipStream_t s);
 hipError_t launch_unit_items(int dtype, void* base, const uint64_t* offset, int n, hipStream_t s);
