@@ -240,6 +240,7 @@ struct mqa_circuit {
     int n_slots = 0;
     int max_qubit = -1;
     bool unitary = true;  // every dense matrix is unitary (the other kinds always are)
+    bool dense_ops = false;  // multi-qubit or parameterised matrix gates (plans of the FULL interpreter)
     // parameterised matrices (MQA_GATE_PMATRIX / _TAP): doubles a sample's row needs, and the table
     // of the calls that follow (mqa_circuit_set_custom_matrices)
     int64_t custom_need = 0;
@@ -650,10 +651,12 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
 // the per-gate ops it replaces, so with jit off the plans keep one op per gate.  Value 2 uses
 // them whatever the jit setting.
 // Launches below 2^22 amplitudes (a single small state) are latency-bound: there the serial
-// products of a block cost more than the per-gate ops, so their plans keep one op per gate.
-bool want_phase_ops(uint64_t batch, uint64_t dim) {
+// products of a block cost more than the per-gate ops, so their plans keep one op per gate; so do
+// circuits with multi-qubit or parameterised matrix gates, whose plans run the FULL interpreted
+// instantiation until compiled, which phase blocks would make heavier still.
+bool want_phase_ops(const mqa_circuit* c, uint64_t batch, uint64_t dim) {
     if (g_opt.phase_ops == 2) return true;
-    return g_opt.phase_ops != 0 && g_opt.jit != 0 && double(batch) * double(dim) >= 4194304.0;
+    return g_opt.phase_ops != 0 && g_opt.jit != 0 && !c->dense_ops && double(batch) * double(dim) >= 4194304.0;
 }
 
 // Device rows [b0, b0 + nb) of a circuit's custom-gate matrices (a one-row table serves every sample),
@@ -1166,6 +1169,7 @@ int mqa_circuit_create(const mqa_gate* gates, int32_t n_gates, const int32_t* te
             } else if (s.kind != MQA_GATE_MATRIX) {
                 require(s.n_obj == want, "gate " + std::to_string(i) + ": wrong number of target qubits");
             }
+            if ((s.kind == MQA_GATE_MATRIX && s.n_obj >= 2) || custom_kind) c->dense_ops = true;
             if (s.kind == MQA_GATE_MATRIX) {
                 int64_t m = int64_t(1) << s.n_obj;
                 require(mats != nullptr && s.mat_begin >= 0 && s.mat_begin + m * m <= n_mat_entries,
@@ -1495,7 +1499,7 @@ int mqa_sim_apply_circuit(mqa_sim* sim, const mqa_circuit* circ_c, const double*
         // phase restored by one scaling pass; the angle table is computed here, so the phase is
         // decided on exactly the values the kernels read
         const bool pf = g_opt.phase_free.load() != 0 && circ->n_slots > 0;
-        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true, pf, want_phase_ops(1, sim->dim_eff));
+        DevPlan* dp = circ->get_plan(sim->n_eff, sim->dtype, false, sim->stream, true, pf, want_phase_ops(circ, 1, sim->dim_eff));
         const double* ang = nullptr;
         std::vector<double> tab;
         std::complex<double> phase(1.0, 0.0);
@@ -1754,7 +1758,7 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
         bool pf = g_opt.phase_free.load() != 0;
         for (int h = 0; h < n_hams; h++) pf = pf && hams[h]->hermitian;
         g_pf_used.store(pf ? 1 : 0);
-        const bool ops = want_phase_ops(uint64_t(batch), sim->dim_eff);
+        const bool ops = want_phase_ops(circ, uint64_t(batch), sim->dim_eff);
         DevPlan* fwd = circ->get_plan(sim->n_eff, sim->dtype, false, s, true, pf, ops);
         DevPlan* adj = circ->get_plan(sim->n_eff, sim->dtype, true, s, true, pf, ops);
         DevBuf* sl = circ->slots_on_device(s);
@@ -2275,8 +2279,9 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             it.dim = uint64_t(1) << it.n_eff;
             // Pauli observables (the only kind items take) are hermitian: phase-free plans
             const bool pf = g_opt.phase_free.load() != 0;
-            // the item passes always run interpreted: phase blocks unless the option is off
-            const bool ops = g_opt.phase_ops != 0;
+            // the item passes always run interpreted: phase blocks unless the option is off or the
+            // circuit's plans need the FULL instantiation
+            const bool ops = g_opt.phase_ops != 0 && !c->dense_ops;
             it.fwd = c->get_plan(it.n_eff, dtype, false, so.s, true, pf, ops);
             it.adj = c->get_plan(it.n_eff, dtype, true, so.s, true, pf, ops);
             if (g_opt.ham_engine) {
