@@ -8,7 +8,7 @@ ansatz_params_name, all_encoder, all_ansatz), as_encoder / as_ansatz / no_grad, 
 import copy
 
 from . import gates as G
-from .mutation import bump
+from .mutation import bump, epoch
 from .parameterresolver import ParameterResolver
 
 
@@ -127,26 +127,37 @@ class Circuit(list):
         qs = [q for g in self for q in g.obj_qubits + g.ctrl_qubits]
         return max(qs) + 1 if qs else 0
 
-    def _names(self, pred):
-        out = _NameList()
-        for g in self:
-            for pr in g.param_resolvers():
-                for k in pr.keys():
-                    if pred(pr, k) and k not in out:
-                        out.append(k)
-        return out
+    def _names(self, which, pred):
+        # the views are kept until the mutation epoch advances (any change to any circuit, gate or
+        # parameter resolver): batched calls read them for every circuit on every call
+        now = epoch()
+        views = self.__dict__.get("_name_views")
+        if views is None or views[0] != now:
+            views = (now, {})
+            self._name_views = views
+        names = views[1].get(which)
+        if names is None:
+            names, seen = [], set()
+            for g in self:
+                for pr in g.param_resolvers():
+                    for k in pr.keys():
+                        if k not in seen and pred(pr, k):
+                            seen.add(k)
+                            names.append(k)
+            views[1][which] = names
+        return _NameList(names)
 
     @property
     def params_name(self):
-        return self._names(lambda pr, k: True)
+        return self._names(0, lambda pr, k: True)
 
     @property
     def encoder_params_name(self):
-        return self._names(lambda pr, k: k in pr.encoder_parameters)
+        return self._names(1, lambda pr, k: k in pr.encoder_parameters)
 
     @property
     def ansatz_params_name(self):
-        return self._names(lambda pr, k: k not in pr.encoder_parameters)
+        return self._names(2, lambda pr, k: k not in pr.encoder_parameters)
 
     @property
     def all_encoder(self):
