@@ -557,12 +557,13 @@ struct CustomRows {
     uint64_t stride = 0;
 };
 
-// src0: the first pass (p0, forward plans) reads its input tiles from src0 (one state broadcast to
-// every sample) instead of psi.  skip_last: the outputs of the last pass (p1 - 1) are dead (the last
-// back-sweep pass of a gradient): it writes its taps only.
+// src0: the first pass (p0, forward plans) reads its input tiles from src0 instead of psi: sample b
+// at src0 + b * src_stride amplitudes (src_stride 0: one state broadcast to every sample).
+// skip_last: the outputs of the last pass (p1 - 1) are dead (the last back-sweep pass of a
+// gradient): it writes its taps only.
 void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, const double* angles, int n_slots,
               double* partials, int n_tap_ids, int p0 = 0, int p1 = -1, CustomRows custom = {},
-              const void* src0 = nullptr, bool skip_last = false) {
+              const void* src0 = nullptr, bool skip_last = false, uint64_t src_stride = 0) {
     const Plan& pl = dp->plan;
     if (p1 < 0 || p1 > int(pl.passes.size())) p1 = int(pl.passes.size());
     if (p0 >= p1) return;
@@ -626,11 +627,11 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     for (int i = p0; i < p1; i++) {
         a.pass_index = i;
         a.src = (i == p0 && !pl.adjoint) ? src0 : nullptr;
-        a.src_stride = 0;
+        a.src_stride = a.src ? src_stride : 0;
         a.skip_store = (skip_last && i == p1 - 1) ? 1 : 0;
         // algorithmic bytes: a broadcast input is read once, dead outputs are not written
         double pass_bytes = bytes;
-        if (a.src) pass_bytes = double(batch) * double(sim->state_bytes()) + double(sim->state_bytes());
+        if (a.src && a.src_stride == 0) pass_bytes = double(batch) * double(sim->state_bytes()) + double(sim->state_bytes());
         if (a.skip_store) pass_bytes = bytes / 2;
         timed(sim, pl.adjoint ? "tile_pass_adj" : "tile_pass_fwd", pass_bytes, [&] {
             if (jp) {
@@ -1807,23 +1808,34 @@ int mqa_sim_grad_batch(const mqa_sim* sim_c, const mqa_ham* const* hams, int32_t
             // the first forward pass reads the initial state itself (every sample starts from it), so
             // psi is not filled first; a plan without passes still needs the copies
             auto forward = [&] {
-                DevPlan* first = ck ? ck->segs[0].get() : fwd;
-                const bool direct = !first->plan.passes.empty();
-                if (!direct)
-                    hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff,
-                                          sim->dim_eff, nb, s),
-                              "fill");
-                const void* src0 = direct ? sim->state.p : nullptr;
                 if (!ck) {
-                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust, src0);
+                    const bool direct = !fwd->plan.passes.empty();
+                    if (!direct)
+                        hip_check(launch_fill(sim->dtype, sim->psi.p, sim->state.p, sim->dim_eff, sim->dim_eff,
+                                              sim->dim_eff, nb, s),
+                                  "fill");
+                    run_plan(sim, fwd, sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust,
+                             direct ? sim->state.p : nullptr);
                     return;
                 }
+                // segment k reads the previous one's output and writes its own buffer (its first
+                // pass out of place): segment k < n_ck fills checkpoint slot n_ck - 1 - k, the last
+                // one psi, so the checkpoints cost no copies
                 for (int k = 0; k <= n_ck; k++) {
-                    run_plan(sim, ck->segs[size_t(k)].get(), sim->psi.p, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0,
-                             -1, cust, k == 0 ? src0 : nullptr);
-                    if (k < n_ck)
-                        hip_check(hipMemcpyAsync(ckpt_at(n_ck - 1 - k), sim->psi.p, sb * nb, hipMemcpyDeviceToDevice, s),
-                                  "checkpoint");
+                    DevPlan* seg = ck->segs[size_t(k)].get();
+                    void* out = k < n_ck ? ckpt_at(n_ck - 1 - k) : sim->psi.p;
+                    if (seg->plan.passes.empty()) {
+                        if (k == 0)
+                            hip_check(launch_fill(sim->dtype, out, sim->state.p, sim->dim_eff, sim->dim_eff,
+                                                  sim->dim_eff, nb, s),
+                                      "fill");
+                        else
+                            hip_check(hipMemcpyAsync(out, ckpt_at(n_ck - k), sb * nb, hipMemcpyDeviceToDevice, s),
+                                      "checkpoint");
+                        continue;
+                    }
+                    run_plan(sim, seg, out, nullptr, nb, ang, circ->n_slots, nullptr, 0, 0, -1, cust,
+                             k == 0 ? sim->state.p : ckpt_at(n_ck - k), false, k == 0 ? 0 : sim->dim_eff);
                 }
             };
             forward();
