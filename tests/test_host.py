@@ -33,6 +33,25 @@ def test_circuit_names_and_hermitian():
     assert s.hermitian().ir_kind == "Sdag" and s.hermitian().hermitian().ir_kind == "S"
 
 
+def test_circuit_name_views_follow_every_change():
+    """The name views are kept between reads; any change to a circuit, gate or resolver renews them."""
+    circ = mq.Circuit().rx("a", 0).ry("b", 1)
+    first = circ.params_name
+    assert first == ["a", "b"] and circ.ansatz_params_name == ["a", "b"] and circ.encoder_params_name == []
+    first.append("z")  # a caller's copy: the kept view is not touched
+    assert circ.params_name == ["a", "b"]
+    circ += mq.Circuit().rz("c", 0)
+    assert circ.params_name == ["a", "b", "c"]
+    circ[0] = G.RX("d").on(0)
+    assert circ.params_name == ["d", "b", "c"]
+    circ.as_encoder()
+    assert circ.encoder_params_name == ["d", "b", "c"] and circ.ansatz_params_name == []
+    other = mq.Circuit().ry("e", 0)
+    assert other.params_name == ["e"] and circ.params_name == ["d", "b", "c"]
+    other[0].coeff.as_encoder()  # a resolver's flag on another circuit's gate
+    assert other.encoder_params_name == ["e"]
+
+
 def test_gate_matrices_are_unitary_and_diff_is_derivative():
     for g in [G.RX(0.3), G.RY(0.3), G.RZ(0.3), G.Rxx(0.3), G.Ryy(0.3), G.Rzz(0.3), G.PhaseShift(0.3),
               G.GlobalPhase(0.3)]:
