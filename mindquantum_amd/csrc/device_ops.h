@@ -200,6 +200,22 @@ __device__ __forceinline__ T hmul(T x) {
     }
 }
 
+// A register set times a constant (a compiled pass's sqrt(2)^-n after n unnormalised Hadamards),
+// given as hi + lo (lo: the rounding error of hi, zero for a power of two), so that the product is
+// as close to exact as the per-gate 1 / sqrt(2) of hmul.
+template <typename T>
+__device__ __forceinline__ void scale_regs(Regs<T>& a, T hi, T lo) {
+#pragma unroll
+    for (int v = 0; v < kAmpsPerThread; v++) {
+        if (lo == T(0)) {
+            a.r[v] *= hi, a.i[v] *= hi;
+        } else {
+            a.r[v] = fma(a.r[v], hi, a.r[v] * lo);
+            a.i[v] = fma(a.i[v], hi, a.i[v] * lo);
+        }
+    }
+}
+
 // One-target op on a register set.  c, s: cos / sin of the half angle (RX, RY, RZ) or of the
 // angle (PS, GP); m: dense 2x2 (M1) in the matrix pool.  CTRL: some control qubits are local;
 // their predicate is evaluated per pair and applied with selects (no divergent branches).
@@ -214,6 +230,9 @@ __device__ __forceinline__ void one_target(Regs<T>& a, const GroupCtx& g, uint32
         } else if constexpr (KIND == OPK_H) {  // [[1, 1], [1, -1]] / sqrt(2)
             xr = hmul<T>(ar + br), xi = hmul<T>(ai + bi);
             yr = hmul<T>(ar - br), yi = hmul<T>(ai - bi);
+        } else if constexpr (KIND == OPK_HU) {  // [[1, 1], [1, -1]]
+            xr = ar + br, xi = ai + bi;
+            yr = ar - br, yi = ai - bi;
         } else if constexpr (KIND == OPK_RX) {  // [[c, -is], [-is, c]]
             xr = c * ar + s * bi, xi = c * ai - s * br;
             yr = c * br + s * ai, yi = c * bi - s * ar;

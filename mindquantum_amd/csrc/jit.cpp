@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cctype>
 #include <cinttypes>
 #include <cerrno>
 #include <cstdio>
@@ -33,6 +35,7 @@ const char* kind_name(int k) {
     switch (k) {
         case OPK_X: return "OPK_X";
         case OPK_H: return "OPK_H";
+        case OPK_HU: return "OPK_HU";
         case OPK_M1: return "OPK_M1";
         case OPK_SWAP: return "OPK_SWAP";
         case OPK_M2: return "OPK_M2";
@@ -473,6 +476,28 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         o << "  __syncthreads();\n  tile_load<T>(f.t_psi, f.g_lam, f.offtab, f.C, f.tid, f.NT);\n  __syncthreads();\n";
         sb_move("load", 0, "l");
     }
+    // unnormalised Hadamards applied so far (OPK_HU): the tile is sqrt(2)^n_hu times the state, so
+    // each tap taken from here on is 2^n_hu times its value
+    int n_hu = 0;
+    auto pow2 = [](int e) {
+        std::ostringstream s;
+        s << "T(" << std::setprecision(17) << std::ldexp(1.0, e) << ")";
+        return s.str();
+    };
+    auto unscale = [&](const char* indent) {  // sqrt(2)^-n_hu on the registers, before the tile leaves
+        if (n_hu == 0) return;
+        // hi + lo: the factor in the state's real type and its rounding error (complex64: a float
+        // pair, as hmul; zero for even counts, a power of two)
+        const double c = std::ldexp(n_hu % 2 ? 0.70710678118654752440084436210484903928 : 1.0, -(n_hu / 2));
+        const double hi = dtype == 0 ? double(float(c)) : c;
+        const double lo = dtype == 0 ? double(float(c - hi)) : 0.0;
+        std::ostringstream s;
+        s << std::setprecision(17) << indent << "{\n" << indent << "  const T hh = T(" << hi << "), hl = T(" << lo
+          << ");\n" << indent << "  scale_regs<T>(p, hh, hl);\n";
+        if (adj) s << indent << "  scale_regs<T>(l, hh, hl);\n";
+        s << indent << "}\n";
+        o << s.str();
+    };
     for (int gi = 0; gi < ps.n_groups; gi++) {
         const DevGroup& g = pl.dev_groups.at(size_t(ps.group_begin + gi));
         if (sb) {
@@ -551,7 +576,9 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
                 emit_phase_free(b, op, adj, ready(op), taps);
                 continue;
             }
-            const char* kn = kind_name(op.kind);
+            // an uncontrolled Hadamard runs without its 1 / sqrt(2) (the pass rescales the tile once)
+            const bool hu = op.kind == OPK_H && op.ctrl_local == 0 && op.ctrl_global == 0;
+            const char* kn = kind_name(hu ? int(OPK_HU) : op.kind);
             if (!kn) throw std::invalid_argument("code generation: unknown op kind " + std::to_string(op.kind));
             const bool two = two_target_kind(op.kind);
             const int q1 = two ? int(op.q1) : 0;
@@ -594,12 +621,32 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
             std::istringstream in(b.str());
             std::string line;
             while (std::getline(in, line)) {
+                if (n_hu > 0 && line.find("taps4_to_lds<T>(f, ptap") != std::string::npos) {
+                    // phase-block taps: each accumulator over 2^n_hu
+                    std::string out;
+                    for (size_t i = 0; i < line.size();) {
+                        if (line.compare(i, 4, "ptap") == 0) {
+                            size_t e = i + 4;
+                            while (e < line.size() && std::isdigit(static_cast<unsigned char>(line[e]))) e++;
+                            out += line.substr(i, e - i) + " * " + pow2(-n_hu);
+                            i = e;
+                        } else {
+                            out += line[i++];
+                        }
+                    }
+                    line = out;
+                }
                 o << line << "\n";
+                if (line.find("apply_body<T, OPK_HU,") != std::string::npos && line.find("(p, g") != std::string::npos)
+                    n_hu++;
                 const size_t t = line.find_first_not_of(' ');
                 if (t != std::string::npos && line.compare(t, 3, "tap") == 0) {
                     const size_t eq = line.find(" = ", t);
-                    if (eq != std::string::npos && line.back() == ';')
-                        o << std::string(t, ' ') << "asm volatile(\"\" : \"+v\"(" << line.substr(t, eq - t) << "));\n";
+                    if (eq != std::string::npos && line.back() == ';') {
+                        const std::string var = line.substr(t, eq - t);
+                        if (n_hu > 0) o << std::string(t, ' ') << var << " *= " << pow2(-n_hu) << ";\n";
+                        o << std::string(t, ' ') << "asm volatile(\"\" : \"+v\"(" << var << "));\n";
+                    }
                 }
             }
         }
@@ -615,6 +662,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
             o << "  }\n";
             continue;
         }
+        if (gi == ps.n_groups - 1) unscale("    ");
         // the next group reads the tile in its own layout
         o << "    group_store<T, " << A << ">(f, a0, off, p, l);\n"
           << "    __syncthreads();\n"
@@ -624,6 +672,7 @@ std::string jit_pass_source(const Plan& pl, int pass_index, int dtype, JitForm f
         const int last = ps.n_groups - 1;
         o << "  uint32_t tid_st = f.tid;\n  asm volatile(\"\" : \"+v\"(tid_st));\n  __syncthreads();\n"
           << "  if (!args.skip_store) {\n";
+        unscale("  ");
         sb_move("store", last, "p");
         o << "  __syncthreads();\n  tile_store<T>(f.t_psi, f.g_psi, f.offtab, f.C, tid_st, f.NT);\n"
           << "  __syncthreads();\n";

@@ -41,6 +41,10 @@ enum OpKind : int32_t {
     OPK_CX = 10, // X on q0 controlled by register position q1 (CNOT), plus any other controls
     OPK_CZ = 11, // Z on q0 controlled by register position q1 (CZ), plus any other controls
     OPK_H = 12,  // Hadamard: (a + b, a - b) / sqrt(2), half the operations of a dense 2x2
+    // Compiled passes only (never in a plan): the uncontrolled Hadamard without its 1 / sqrt(2),
+    // (a + b, a - b); the pass divides the tile by sqrt(2)^n once, before it stores it (n: the
+    // pass's count), and each tap by 2^n (n: the count before the tap)
+    OPK_HU = 13,
     // parameterised: the angle comes from the per-sample angle table
     OPK_RX = 16,
     OPK_RY = 17,
