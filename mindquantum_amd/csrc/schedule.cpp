@@ -118,12 +118,36 @@ static uint64_t nondiag_targets(const CGate& g) {
     }
 }
 
-// Terms of a gate that joins phase blocks (RZ, PS, uncontrolled RZZ; false for other gates).  The
-// phase of a term is k * alpha + 0 with alpha the angle the slot's table entry holds (the half
-// angle of RZ / RZZ, the angle of PS), and its tap weight -dphi / dtheta.  Phase-free plans drop
-// the global phase of uncontrolled RZ / RZZ, as their OPK_PF_* forms do.
+// Terms of a gate that joins phase blocks (RZ, PS, uncontrolled RZZ, and the constant phases Z, S,
+// T, their inverses and their controlled forms such as CZ; false for other gates).  The phase of a
+// parameterised term is k * alpha + 0 with alpha the angle the slot's table entry holds (the half
+// angle of RZ / RZZ, the angle of PS), and its tap weight -dphi / dtheta; a constant term has no
+// slot (k = 0) and the phase b.  Phase-free plans drop the global phase of uncontrolled RZ / RZZ,
+// as their OPK_PF_* forms do.
 static bool phase_terms(const CGate& g, bool phase_free, std::vector<HostPhaseTerm>* out) {
     out->clear();
+    auto constant = [&](double cb, double sb) {  // e^{i b} = (cb, sb) on the control subspace's |1>
+        if (g.objs.size() != 1) return false;
+        HostPhaseTerm t;
+        t.kind = PHT_AND;
+        t.mask = g.ctrl | (uint64_t(1) << g.objs[0]);
+        t.coef.slot = -1;
+        t.coef.k = 0;
+        t.coef.cos_b = cb;
+        t.coef.sin_b = sb;
+        t.weight = 0.0;
+        out->push_back(t);
+        return true;
+    };
+    const double h = 0.70710678118654752440084436210484903928;
+    switch (g.kind) {
+        case MQA_GATE_Z: return constant(-1.0, 0.0);
+        case MQA_GATE_S: return constant(0.0, 1.0);
+        case MQA_GATE_SDAG: return constant(0.0, -1.0);
+        case MQA_GATE_T: return constant(h, h);
+        case MQA_GATE_TDAG: return constant(h, -h);
+        default: break;
+    }
     if (g.slot < 0) return false;
     auto add = [&](int kind, uint64_t mask, int k, double dalpha) {
         HostPhaseTerm t;
@@ -342,7 +366,7 @@ struct PassBuilder {
                 if (in_block) {
                     phase_terms(g, phase_free, &terms);
                     int tap = -1;
-                    if (adjoint && g.grad) {
+                    if (adjoint && g.grad && g.slot >= 0) {
                         tap = int(phase_taps.size());
                         phase_taps.push_back(tap_id_of_slot[size_t(g.slot)]);
                     }

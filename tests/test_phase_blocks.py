@@ -1,5 +1,6 @@
-"""Phase blocks of the scheduler (CPU only): with option "phase_ops", RZ, PS and uncontrolled RZZ
-gates join diagonal blocks that need no local qubit and may move past every gate they commute with.
+"""Phase blocks of the scheduler (CPU only): with option "phase_ops", RZ, PS, uncontrolled RZZ and
+the constant phases (Z, S, T, their inverses and controlled forms) join diagonal blocks that need
+no local qubit and may move past every gate they commute with.
 
 The schedule (mqa_circuit_schedule, mode + 16) must hold every gate exactly once, keep the other
 gates' targets local, and be a valid reordering: the dense simulation of the gates in schedule
@@ -26,8 +27,11 @@ def diag_kinds():
 
 
 def in_block(g):
-    """Gates that join phase blocks."""
+    """Gates that join phase blocks: RZ, PS, uncontrolled RZZ and the constant phases (Z, S, T, their
+    inverses and controlled forms)."""
     if isinstance(g, diag_kinds()):
+        return True
+    if getattr(g, "ir_kind", None) in ("Z", "S", "Sdag", "T", "Tdag"):
         return True
     return isinstance(g, G.Rzz) and not g.ctrl_qubits
 
