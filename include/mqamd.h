@@ -121,9 +121,9 @@ int mqa_device_count(void);
  * registers between register groups and share one LDS tile buffer, so twice as many blocks fit a CU;
  * 0: two tile buffers), "dense_mfma" (1, the default: complex64 5-qubit blocks of
  * mqa_sim_apply_dense_blocks run on the FP32 matrix cores; 0: on the vector ALUs), "phase_ops" (1, the
- * default: with "jit" on, the plans of batched gradients and gate application merge RZ, PS and
- * uncontrolled RZZ gates into diagonal phase blocks that need no local qubit; 2: also with "jit"
- * off; 0: one op per gate).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
+ * default: with "jit" on, the plans of batched gradients and gate application merge RZ, PS,
+ * uncontrolled RZZ and the constant phases Z, S, T (their inverses and controlled forms such as CZ)
+ * into diagonal phase blocks that need no local qubit; 2: also with "jit" off; 0: one op per gate).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
  * "grad_recomputes", "checkpoints_used", "phase_free_used" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
@@ -170,8 +170,8 @@ int mqa_circuit_get_tile_qubits(const mqa_circuit* circ, int32_t dtype, int32_t 
 /* Export the tile-pass schedule (host only, no GPU needed) for inspection and tests.
  * Layout of out (int64 words): [n_passes, then per pass: local_mask, n_gates, gate ids...].
  * adjoint bit 0 exports the reverse (back-sweep) schedule, bit 1 its phase-free form, bit 4
- * (+ 16) the form with phase blocks (option "phase_ops": RZ, PS and RZZ gates merged into
- * diagonal blocks that need no local qubit).  *len gets the words needed. */
+ * (+ 16) the form with phase blocks (option "phase_ops": RZ, PS, RZZ and constant phase gates
+ * merged into diagonal blocks that need no local qubit).  *len gets the words needed. */
 int mqa_circuit_schedule(const mqa_circuit* circ, int32_t n_qubits, int32_t dtype, int32_t adjoint, int64_t* out,
                          int64_t cap, int64_t* len);
 /* Generate and compile (hipRTC, host only, no GPU needed) the per-circuit kernels of the
