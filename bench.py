@@ -403,7 +403,8 @@ def gate_apply_cpu(dtype, n, layers, depth, seed):
 def run_uneven_items(count, seed, cpu_sample):
     """Uneven MaxCut batch: `count` QAOA circuits (p = 4) on 12..24 qubits, expectation and
     adjoint gradient of every circuit in complex64 through one mqa_grad_items call (shared
-    per-step launches over the tiles of all circuits).  Optionally the CPU oracle on a subset
+    per-step launches over the tiles of the smaller circuits, the largest on their own compiled
+    passes; the compiles run in the untimed warm-up call).  Optionally the CPU oracle on a subset
     (two circuits of every size from 12 to 20 qubits) beside the GPU on the same subset, and the
     subset's error against the complex128 oracle."""
     import mindquantum_amd as mq
@@ -435,8 +436,12 @@ def run_uneven_items(count, seed, cpu_sample):
     finally:
         _native.set_option("profile", 0)
     kern = {}
-    for name in ("items_tile_pass_fwd", "items_tile_pass_adj", "items_hamiltonian"):
+    # shared launches (items_*) and the large items' own compiled launches (tile_pass_*, hamiltonian)
+    for name in ("items_tile_pass_fwd", "items_tile_pass_adj", "items_hamiltonian", "tile_pass_fwd", "tile_pass_adj",
+                 "hamiltonian"):
         launches, ms, byts = _native.profile(name)
+        if launches == 0:
+            continue
         kern[name] = {"launches": launches, "ms": round(ms, 3), "gb": round(byts / 1e9, 2),
                       "gbs": round(byts / (ms * 1e-3) / 1e9, 1) if ms else None}
     tot_ms = sum(k["ms"] for k in kern.values())

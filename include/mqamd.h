@@ -123,7 +123,9 @@ int mqa_device_count(void);
  * mqa_sim_apply_dense_blocks run on the FP32 matrix cores; 0: on the vector ALUs), "phase_ops" (1, the
  * default: with "jit" on, the plans of batched gradients and gate application merge RZ, PS,
  * uncontrolled RZZ and the constant phases Z, S, T (their inverses and controlled forms such as CZ)
- * into diagonal phase blocks that need no local qubit; 2: also with "jit" off; 0: one op per gate).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
+ * into diagonal phase blocks that need no local qubit; 2: also with "jit" off; 0: one op per gate).
+ * "items_jit_qubits" (23: items of an uneven batch with at least this many qubits run on their own
+ * compiled passes; 0: every item in the shared interpreted launches).  mqa_get_option also reads the counters "jit_launches", "jit_compile_ms",
  * "grad_recomputes", "checkpoints_used", "phase_free_used" and "jit_single_buffer_launches".  No counterpart
  * in the reference (its CPU kernels have no tuning surface). */
 int mqa_set_option(const char* key, int64_t value);
@@ -288,7 +290,9 @@ int mqa_sim_grad_nonherm(const mqa_sim* sim, const mqa_sim* left, const mqa_ham*
  * (<psi|H|psi>), g_out[param_offset[i] ..] (real gradient).  Items run in chunks that fit the HBM
  * budget.  The reference has no batched entry for different circuits: a caller makes one
  * get_expectation_with_grad call per circuit and simulator (mqsim.py:255-439); this is the same
- * result for a whole list. */
+ * result for a whole list.  With "jit" on, items of at least "items_jit_qubits" qubits (23 by
+ * default; 0: none) run one by one as the batched gradient of one sample on their per-circuit
+ * compiled passes (a launch of one such state fills the device), the rest in the shared launches. */
 int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_ham* const* hams,
                    const int32_t* n_qubits, const double* params, const int64_t* param_offset, int32_t dtype,
                    double* f_out, double* g_out);

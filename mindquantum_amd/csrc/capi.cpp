@@ -89,6 +89,10 @@ struct Options {
     // 1: batched gradients and gate application whose launches run compiled passes merge RZ, PS and
     // RZZ gates into phase blocks (OPK_PHASE); 2: every launch of those (interpreted ones too)
     std::atomic<int64_t> phase_ops{1};
+    // uneven batches (mqa_grad_items): items of at least this many qubits run one by one on the
+    // per-circuit compiled passes (with "jit" on) instead of in the shared interpreted launches; 0:
+    // every item shares them
+    std::atomic<int64_t> items_jit_qubits{23};
 };
 Options g_opt;
 
@@ -108,7 +112,8 @@ std::vector<OptEntry> opt_table() {
             {"jit_min_amps", &g_opt.jit_min_amps, int64_t(1) << 24}, {"jit_min_runs", &g_opt.jit_min_runs, 8},
             {"checkpoints", &g_opt.checkpoints, 0},
             {"phase_free", &g_opt.phase_free, 1}, {"jit_single_buffer", &g_opt.jit_single_buffer, 1},
-            {"dense_mfma", &g_opt.dense_mfma, 1}, {"phase_ops", &g_opt.phase_ops, 1}};
+            {"dense_mfma", &g_opt.dense_mfma, 1}, {"phase_ops", &g_opt.phase_ops, 1},
+            {"items_jit_qubits", &g_opt.items_jit_qubits, 23}};
 }
 
 SchedOptions sched_options(int dtype, bool adjoint) {
@@ -497,6 +502,7 @@ struct mqa_sim {
     mqa::DevBuf psi, lam, psi_keep, ckpt, angles, enc, ans, partials, taps, grads, fpart, fsum, scratch_state;
     mqa::DevBuf coefs;  // coefficient table of the compiled passes (rebuilt per run_plan call)
     mqa::DevBuf phase;  // phase table of the running plan's phase blocks (rebuilt per run_plan call)
+    bool force_jit = false;  // compiled passes at any size (the solo items of an uneven batch)
     mqa::DevBuf custom[2];  // rows of the custom-gate matrices of the running circuits (two: bra and ket sides)
     mqa::DevBuf dense;      // matrices of the dense blocks of mqa_sim_apply_dense_blocks
     mqa::Timer timer;
@@ -590,7 +596,7 @@ void run_plan(mqa_sim* sim, DevPlan* dp, void* psi, void* lam, uint64_t batch, c
     const int64_t runs = dp->runs.fetch_add(1) + 1;
     const bool big = double(batch) * double(sim->dim_eff) >= double(g_opt.jit_min_amps);
     const bool recurring = g_opt.jit_min_runs > 0 && runs >= g_opt.jit_min_runs;
-    if (g_opt.jit != 0 && (big || recurring)) {
+    if (g_opt.jit != 0 && (big || recurring || sim->force_jit)) {
         std::lock_guard<std::mutex> lk(dp->jit_mu);
         JitForm form = JitForm::kTile;
         if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, sim->dtype)) form = JitForm::kSingleBuffer;
@@ -2272,6 +2278,10 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
         std::vector<GradItem> all;
         std::vector<const mqa_ham*> costate(size_t(n_items), nullptr);  // the co-state observable of each item
         std::vector<double> shift(size_t(n_items), 0.0);
+        // large items run one by one on their compiled passes (a launch of one such state fills the
+        // device, and the compiled passes outrun the shared interpreted ones); the rest share launches
+        const int64_t solo_q = g_opt.jit != 0 ? g_opt.items_jit_qubits.load() : 0;
+        std::vector<int> solo;
         for (int i = 0; i < n_items; i++) {
             auto* c = const_cast<mqa_circuit*>(circs[i]);
             require(c != nullptr && hams[i] != nullptr, "item " + std::to_string(i) + ": NULL circuit or hamiltonian");
@@ -2285,6 +2295,10 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
                                                                          "in item batches");
             require((h->qubit_mask >> n) == 0, "item " + std::to_string(i) + ": hamiltonian is wider than its qubit count");
             require(c->n_params == 0 || params != nullptr, "params is NULL");
+            if (solo_q > 0 && n >= solo_q) {
+                solo.push_back(i);
+                continue;
+            }
             GradItem it;
             it.index = i;
             it.n_eff = n_eff_of(n);
@@ -2305,6 +2319,28 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
                        uint64_t(c->n_slots) * 16 + 2 * ((it.dim + 2047) / 2048 + 1) * 8 + 4 * sizeof(TilePassArgs) *
                        (it.fwd->plan.passes.size() + it.adj->plan.passes.size() + 4);
             all.push_back(it);
+        }
+        // the solo items: the batched gradient of one sample on a simulator of their size (one per
+        // size, released at the end of the call; the compiled passes stay with the circuits)
+        {
+            std::map<int, std::unique_ptr<mqa_sim, int (*)(mqa_sim*)>> sims;
+            for (int i : solo) {
+                const int n = n_qubits[i];
+                auto it = sims.find(n);
+                if (it == sims.end()) {
+                    mqa_sim* s = nullptr;
+                    if (mqa_sim_create(n, dtype, 0, &s) != MQA_OK) throw Error(MQA_ERR_HIP, mqa_last_error());
+                    s->force_jit = true;
+                    it = sims.emplace(n, std::unique_ptr<mqa_sim, int (*)(mqa_sim*)>(s, &mqa_sim_destroy)).first;
+                }
+                const mqa_circuit* c = circs[i];
+                const mqa_ham* h = hams[i];
+                const int st = mqa_sim_grad_batch(it->second.get(), &h, 1, c, nullptr, 1, 0,
+                                                  c->n_params ? params + param_offset[i] : nullptr, c->n_params,
+                                                  f_out + 2 * i, g_out + param_offset[i]);
+                if (st != MQA_OK) throw Error(st, "item " + std::to_string(i) + ": " + mqa_last_error());
+                shift[size_t(i)] = 0.0;  // the batched gradient adds the identity part itself
+            }
         }
         // chunks that fit the HBM budget, largest states first
         std::stable_sort(all.begin(), all.end(), [](const GradItem& a, const GradItem& b) { return a.n_eff > b.n_eff; });

@@ -18,6 +18,7 @@
 #include <fstream>
 #include <iomanip>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 
@@ -752,10 +753,15 @@ void JitPlan::compile_all() {
         for (int p = 0; p < n; p++) std::ofstream(std::string(dir) + "/pass" + std::to_string(p) + ".hip") << src[size_t(p)];
     }
 
-    // parallel: helper processes, each compiling every W-th pass
+    // parallel: helper processes, each compiling every W-th pass; one plan at a time holds them
+    // (plans compiling in the background - many circuits at once - queue here instead of each
+    // starting its own set of processes)
     const std::string helper = helper_path();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const int workers = std::max(1, std::min<int>({n, int(hw), 16}));
+    static std::mutex helpers_mu;
+    std::unique_lock<std::mutex> helpers_lock(helpers_mu, std::defer_lock);
+    if (!helper.empty() && workers > 1) helpers_lock.lock();
     if (!helper.empty() && workers > 1) {
         const char* tmp = std::getenv("TMPDIR");
         std::string tmpl = std::string(tmp && *tmp ? tmp : "/tmp") + "/mqamd_jit_XXXXXX";

@@ -94,3 +94,40 @@ def test_items_in_budget_chunks():
     for a, b in zip(g0, g1):
         assert np.max(np.abs(a - b)) < 1e-6
     check_items(circs, ops, nqs, params, dtype)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_solo_items_on_compiled_passes(dtype):
+    """Items at or above the "items_jit_qubits" size run one by one on the per-circuit compiled
+    passes (the batched gradient of one sample), the rest in the shared launches: both kinds in one
+    call match the oracle and the all-shared call (identity parts, a qubit count above the circuit
+    width, MaxCut Hamiltonians with their constant)."""
+    rng = np.random.default_rng(12)
+    circs, ops, nqs, params = [], [], [], []
+    for i, n in enumerate([12, 8, 14, 10, 16]):
+        graph = random_regular_graph(n, 3, 40 + i)
+        c = max_cut_ansatz(graph, 2)
+        circs.append(c), ops.append(max_cut_hamiltonian(graph)), nqs.append(n)
+    c = hea(11, 2)
+    circs.append(c), ops.append(random_hamiltonian(11, 10, 5) + mq.QubitOperator("", 0.25)), nqs.append(13)
+    for c in circs:
+        params.append(rng.uniform(0, 2 * np.pi, size=len(c.params_name)))
+    hams = [mq.Hamiltonian(o, dtype) for o in ops]
+    jit0 = _native.get_option("jit")
+    _native.set_option("items_jit_qubits", 0)
+    try:
+        f0, g0 = mq.expectation_with_grad_batch(circs, hams, params, nqs, dtype)
+        _native.set_option("jit", 2)
+        _native.set_option("items_jit_qubits", 12)
+        launches = _native.get_option("jit_launches")
+        f1, g1 = mq.expectation_with_grad_batch(circs, hams, params, nqs, dtype)
+        assert _native.get_option("jit_launches") > launches  # the solo items ran compiled passes
+    finally:
+        _native.set_option("items_jit_qubits", -1)
+        _native.set_option("jit", jit0)
+    tol = 1e-5 if dtype == mq.complex64 else 1e-11
+    scale = max(1.0, float(np.max(np.abs(f0))))
+    assert np.max(np.abs(f1 - f0)) < tol * scale
+    for a, b in zip(g0, g1):
+        assert np.max(np.abs(a - b)) < tol * max(1.0, float(np.max(np.abs(b))))
+    check_items(circs, ops, nqs, params, dtype)
