@@ -190,6 +190,7 @@ def run_hea16(reps, cpu):
         return (time.perf_counter() - t0) / reps * 1e3, out
 
     # interpreted passes: what the first calls of a plan run while its kernels compile
+    jit_prev = _native.get_option("jit")
     _native.set_option("jit", 0)
     try:
         interp_ms, _ = timed_calls()
@@ -209,6 +210,7 @@ def run_hea16(reps, cpu):
     launches0 = _native.get_option("jit_launches")
     gpu_ms, (f, g) = timed_calls()
     assert _native.get_option("jit_launches") > launches0
+    _native.set_option("jit", jit_prev)  # the later workloads run with the caller's setting again
     res = {"qubits": 16, "layers": 10, "params": len(vals), "terms": len(op.terms), "dtype": "complex128",
            "gpu_ms_per_grad": round(gpu_ms, 3), "calls_timed": reps, "compiled_passes": True,
            "compile_seconds_untimed": round(compile_s, 1), "gpu_ms_per_grad_interpreted": round(interp_ms, 3)}

@@ -666,6 +666,20 @@ bool want_phase_ops(const mqa_circuit* c, uint64_t batch, uint64_t dim) {
     return g_opt.phase_ops != 0 && g_opt.jit != 0 && !c->dense_ops && double(batch) * double(dim) >= 4194304.0;
 }
 
+// Whether a plan's compiled passes are ready, in the form run_plan would launch; starts their
+// compile (in the background) if it has not started.  jit 2 waits for it.
+bool jit_ready(DevPlan* dp, int dtype) {
+    if (g_opt.jit == 0) return false;
+    std::lock_guard<std::mutex> lk(dp->jit_mu);
+    const Plan& pl = dp->plan;
+    JitForm form = JitForm::kTile;
+    if (g_opt.jit_single_buffer != 0 && jit_can_single_buffer(pl, dtype)) form = JitForm::kSingleBuffer;
+    auto& slot = form == JitForm::kSingleBuffer ? dp->jit_sb : dp->jit;
+    if (!slot) slot = std::make_unique<JitPlan>(pl, dtype, form);
+    if (g_opt.jit == 2) slot->wait();
+    return slot->ready();
+}
+
 // Device rows [b0, b0 + nb) of a circuit's custom-gate matrices (a one-row table serves every sample),
 // in the simulator's workspace `which` (0 or 1: the two circuits of one call use separate ones).
 // one_row: every state of the launch belongs to sample b0 (row b0 for all of them).
@@ -2296,8 +2310,18 @@ int mqa_grad_items(int32_t n_items, const mqa_circuit* const* circs, const mqa_h
             require((h->qubit_mask >> n) == 0, "item " + std::to_string(i) + ": hamiltonian is wider than its qubit count");
             require(c->n_params == 0 || params != nullptr, "params is NULL");
             if (solo_q > 0 && n >= solo_q) {
-                solo.push_back(i);
-                continue;
+                // on their own only once their compiled passes are ready (jit 2 compiles them here;
+                // jit 1 in the background, the item sharing the interpreted launches meanwhile) -
+                // the plans the batched gradient of one sample runs
+                const bool pf1 = g_opt.phase_free.load() != 0 && h->hermitian && hams[i]->hermitian;
+                const bool ops1 = want_phase_ops(c, 1, uint64_t(1) << n_eff_of(n));
+                DevPlan* f1 = c->get_plan(n_eff_of(n), dtype, false, so.s, true, pf1, ops1);
+                DevPlan* a1 = c->get_plan(n_eff_of(n), dtype, true, so.s, true, pf1, ops1);
+                const bool rf = jit_ready(f1, dtype), ra = jit_ready(a1, dtype);
+                if (rf && ra) {
+                    solo.push_back(i);
+                    continue;
+                }
             }
             GradItem it;
             it.index = i;
