@@ -716,6 +716,13 @@ def main():
     share = {k: round(v[1], 2) for k, v in stats.items()}
     extra = {"kernel_ms_in_timed_region": share, "compile_seconds_untimed": _native.get_option("jit_compile_ms") / 1e3,
              "autotune": h64["tiles"]}
+    # the other two kernel families of the step, on the same basis (algorithmic bytes / launch time)
+    for name in ("tile_pass_fwd", "hamiltonian"):
+        if name in stats and stats[name][0] > 0:
+            n_l, ms_l, b_l = stats[name]
+            gbs = b_l / n_l / (ms_l / n_l * 1e-3) / 1e9
+            extra[f"roofline_{name}"] = {"launches": n_l, "avg_launch_ms": round(ms_l / n_l, 4),
+                                         "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     samples = [h64["sample"]]
     if not args.no_extra:
         mark("copy_probe_4GiB")
