@@ -3,8 +3,8 @@
 Encoder RX(x_i) on 20 qubits + HEA-8 (RY, RZ on every qubit, CNOT ladder; 320 ansatz parameters),
 H = sum_i w_i Z_i Z_{i+1 mod 20} + sum_i v_i X_i (40 terms), batch 1024, seed 2024 - exactly
 bench.py's workload (mindquantum_amd.algorithm.batched_adjoint) - through the torch.autograd operator
-(MQOps) on compiled passes, at the forward / back-sweep tile widths the bench's tuner takes (13 / 12)
-and at 12 / 12.  Sixteen sampled rows of f, g_enc and g_ans are compared with the complex128 oracle
+(MQOps) on compiled passes, at the forward / back-sweep tile widths the bench's tuner takes (13 / 12),
+at 12 / 12, and at 13 / 13 (a width the library's tuner may take).  Sixteen sampled rows of f, g_enc and g_ans are compared with the complex128 oracle
 (gradient.h:100-197 restated; framework/operations.py:132-146 for the operator):
 
 * complex64: |f - f_ref| <= 1e-5 * sum |c|, max |g - g_ref| / max |g_ref| <= 1e-5;
@@ -47,7 +47,7 @@ def compiled():
         _native.set_option(k, v)
 
 
-@pytest.mark.parametrize("dtype,tiles", [(mq.complex64, (13, 12)), (mq.complex64, (12, 12)),
+@pytest.mark.parametrize("dtype,tiles", [(mq.complex64, (13, 13)), (mq.complex64, (13, 12)), (mq.complex64, (12, 12)),
                                          (mq.complex128, (12, 12))])
 def test_headline_workload_against_c128_oracle(workload, compiled, dtype, tiles):
     circ, op, enc, ans, f_ref, g_ref = workload
